@@ -1,0 +1,261 @@
+"""ctypes binding of the C ABI in include/cghip.h (libcghip.so).
+
+The library is built in-tree by ``cgxx_amd.build`` (``__graft_entry__.build``)
+into ``cgxx_amd/lib/libcghip.so``.  Loading fails loudly when it is missing:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcghip.so")
+
+# CG::Vector (CG.h:22-35)
+VEC = {"K": 0, "X": 1, "P": 2, "Q": 3, "R": 4, "Z": 5}
+FORMAT = {"CRS": 1, "ELL": 2}
+PRECOND = {"none": 0, "jacobi": 1}
+DOT = {"tree": 0, "serial": 1}
+
+SYMBOLS = [
+    "cghip_create", "cghip_destroy", "cghip_last_error", "cghip_geometry",
+    "cghip_set_size", "cghip_upload_crs", "cghip_upload_ell", "cghip_upload_jacobi",
+    "cghip_set_format", "cghip_upload_vector", "cghip_download_vector", "cghip_set_dot_mode",
+    "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
+    "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
+]
+
+
+class CGHIPError(RuntimeError):
+    pass
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precond", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
+                ("batch", C.c_int), ("profile", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("r2", C.c_double),
+                ("wall_s", C.c_double), ("ms_matvec", C.c_double), ("ms_update", C.c_double),
+                ("ms_xpay", C.c_double), ("ms_init", C.c_double), ("n_matvec", C.c_int),
+                ("n_update", C.c_int), ("n_xpay", C.c_int), ("n_init", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libcghip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CGHIPError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.cghip_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.cghip_destroy.argtypes = [vp]
+    L.cghip_last_error.argtypes = [vp]
+    L.cghip_last_error.restype = C.c_char_p
+    L.cghip_geometry.argtypes = [C.POINTER(C.c_int)] * 5
+    L.cghip_set_size.argtypes = [vp, C.c_int]
+    L.cghip_upload_crs.argtypes = [vp, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+    L.cghip_upload_ell.argtypes = [vp, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+    L.cghip_upload_jacobi.argtypes = [vp, _f64p]
+    L.cghip_set_format.argtypes = [vp, C.c_int]
+    L.cghip_upload_vector.argtypes = [vp, C.c_int, _f64p]
+    L.cghip_download_vector.argtypes = [vp, C.c_int, _f64p]
+    L.cghip_set_dot_mode.argtypes = [vp, C.c_int]
+    L.cghip_cpy.argtypes = [vp, C.c_int, C.c_int]
+    L.cghip_matvec.argtypes = [vp, C.c_int, C.c_int]
+    L.cghip_axpy.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+    L.cghip_xpay.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    L.cghip_vector_dot.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.cghip_apply_preconditioner.argtypes = [vp, C.c_int, C.c_int]
+    L.cghip_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]
+    L.cghip_history.argtypes = [vp, C.c_int, _f64p, C.c_int, C.POINTER(C.c_int)]
+    L.cghip_time_matvec.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def geometry() -> dict:
+    vals = [C.c_int() for _ in range(5)]
+    load().cghip_geometry(*[C.byref(v) for v in vals])
+    return dict(zip(["block", "vec_gmax", "rows_max", "nnz_cap", "spmv_gmax"], [v.value for v in vals]))
+
+
+class Context:
+    """One device context: the GPU-side state of a CG backend."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        self._h = C.c_void_p()
+        rc = self.L.cghip_create(device, C.byref(self._h))
+        if rc:
+            raise CGHIPError(f"cghip_create failed ({rc}): {self.L.cghip_last_error(None).decode()}")
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            self.L.cghip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc: int, what: str):
+        if rc:
+            raise CGHIPError(f"{what} failed ({rc}): {self.L.cghip_last_error(self._h).decode()}")
+
+    # data
+    def set_size(self, n: int):
+        self._chk(self.L.cghip_set_size(self._h, n), "cghip_set_size")
+        self.n = n
+
+    def upload_crs(self, ptr, index, value):
+        ptr = np.ascontiguousarray(ptr, np.int32)
+        index = np.ascontiguousarray(index, np.int32)
+        value = np.ascontiguousarray(value, np.float64)
+        n = ptr.size - 1
+        if index.size == 0:
+            index, value = np.zeros(1, np.int32), np.zeros(1, np.float64)
+        self._chk(self.L.cghip_upload_crs(self._h, n, int(ptr[-1]), ptr, index, value), "cghip_upload_crs")
+        self.n = n
+
+    def upload_ell(self, width, length, index, data):
+        length = np.ascontiguousarray(length, np.int32)
+        index = np.ascontiguousarray(index, np.int32)
+        data = np.ascontiguousarray(data, np.float64)
+        n = length.size
+        if index.size == 0:
+            index, data = np.zeros(1, np.int32), np.zeros(1, np.float64)
+        self._chk(self.L.cghip_upload_ell(self._h, n, int(width), length, index, data), "cghip_upload_ell")
+        self.n = n
+
+    def upload_jacobi(self, c):
+        self._chk(self.L.cghip_upload_jacobi(self._h, np.ascontiguousarray(c, np.float64)), "cghip_upload_jacobi")
+
+    def set_format(self, fmt: str):
+        self._chk(self.L.cghip_set_format(self._h, FORMAT[fmt]), "cghip_set_format")
+
+    def set_dot_mode(self, mode: str):
+        self._chk(self.L.cghip_set_dot_mode(self._h, DOT[mode]), "cghip_set_dot_mode")
+
+    def upload(self, vec: str, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.n
+        self._chk(self.L.cghip_upload_vector(self._h, VEC[vec], a), "cghip_upload_vector")
+
+    def download(self, vec: str) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        self._chk(self.L.cghip_download_vector(self._h, VEC[vec], out), "cghip_download_vector")
+        return out
+
+    # hooks (CG.h:224-237)
+    def cpy(self, dst, src):
+        self._chk(self.L.cghip_cpy(self._h, VEC[dst], VEC[src]), "cghip_cpy")
+
+    def matvec(self, x, y):
+        self._chk(self.L.cghip_matvec(self._h, VEC[x], VEC[y]), "cghip_matvec")
+
+    def axpy(self, a, x, y):
+        self._chk(self.L.cghip_axpy(self._h, a, VEC[x], VEC[y]), "cghip_axpy")
+
+    def xpay(self, x, a, y):
+        self._chk(self.L.cghip_xpay(self._h, VEC[x], a, VEC[y]), "cghip_xpay")
+
+    def dot(self, a, b) -> float:
+        out = C.c_double()
+        self._chk(self.L.cghip_vector_dot(self._h, VEC[a], VEC[b], C.byref(out)), "cghip_vector_dot")
+        return out.value
+
+    def apply_preconditioner(self, x, y):
+        self._chk(self.L.cghip_apply_preconditioner(self._h, VEC[x], VEC[y]), "cghip_apply_preconditioner")
+
+    # fused solver
+    def solve(self, precond="jacobi", max_iter=1000, tol=1e-9, batch=0, profile=False) -> dict:
+        o = SolveOpts(PRECOND[precond], max_iter, tol, batch, int(profile))
+        info = SolveInfo()
+        self._chk(self.L.cghip_solve(self._h, C.byref(o), C.byref(info)), "cghip_solve")
+        return info.as_dict()
+
+    def history(self, which: str, count: int) -> np.ndarray:
+        idx = {"r2": 0, "rho": 1, "pq": 2, "alpha": 3, "beta": 4}[which]
+        out = np.full(max(count, 1), np.nan)
+        w = C.c_int()
+        self._chk(self.L.cghip_history(self._h, idx, out, count, C.byref(w)), "cghip_history")
+        return out[: w.value]
+
+    def time_matvec(self, x="P", y="Q", reps=20) -> float:
+        ms = C.c_double()
+        self._chk(self.L.cghip_time_matvec(self._h, VEC[x], VEC[y], reps, C.byref(ms)), "cghip_time_matvec")
+        return ms.value
+
+
+def reference_loop(ctx: Context, precond: str, max_iter=1000, tol=1e-9):
+    """CG::solve (CG.cpp:284-385) driven hook by hook through the C ABI.
+
+    This is the reference's loop structure unchanged (8 synchronous hooks and
+    3 host round trips per iteration); returns (iterations, residual, history).
+    """
+    import math
+
+    hist = {"r2": [], "rho": [], "pq": [], "alpha": [], "beta": []}
+    ctx.matvec("X", "R")
+    ctx.xpay("K", -1.0, "R")
+    if precond == "none":
+        ctx.cpy("P", "R")
+    else:
+        ctx.apply_preconditioner("R", "P")
+    r2 = ctx.dot("R", "R")
+    hist["r2"].append(r2)
+    nrm2_0 = math.sqrt(r2)
+    rho = r2 if precond == "none" else ctx.dot("P", "R")
+    hist["rho"].append(rho)
+    residual = float("nan")
+    it = 0
+    while it < max_iter:
+        ctx.matvec("P", "Q")
+        dot_pq = ctx.dot("P", "Q")
+        a = rho / dot_pq
+        hist["pq"].append(dot_pq)
+        hist["alpha"].append(a)
+        ctx.axpy(a, "P", "X")
+        ctx.axpy(-a, "Q", "R")
+        r2 = ctx.dot("R", "R")
+        hist["r2"].append(r2)
+        residual = math.sqrt(r2) / nrm2_0
+        if residual <= tol:
+            it += 1
+            break
+        rho_old = rho
+        if precond == "none":
+            rho = r2
+        else:
+            ctx.apply_preconditioner("R", "Z")
+            rho = ctx.dot("R", "Z")
+        hist["rho"].append(rho)
+        b = rho / rho_old
+        hist["beta"].append(b)
+        ctx.xpay("R" if precond == "none" else "Z", b, "P")
+        it += 1
+    return it, residual, {k: np.array(v) for k, v in hist.items()}

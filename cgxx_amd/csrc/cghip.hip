@@ -1,0 +1,640 @@
+// C ABI of the MI355X CG backend (declared in include/cghip.h).
+//
+// The context owns one stream and every device buffer: the six CG vectors
+// (CG::Vector, CG.h:22-35), the CRS and/or ELL matrix, the Jacobi diagonal,
+// the reduction scratch and the device-resident solver state.  Each entry
+// point validates its arguments on the host before anything is launched
+// (indices in range, sizes consistent), so malformed input becomes an error
+// code instead of an out-of-bounds access on the GPU.
+
+#include "../../include/cghip.h"
+#include "cghip_kernels.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace cghip;
+
+struct cghip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int n = 0;
+  double *vec[6] = {};
+
+  // CRS (MatrixDataCRS) + row blocks of the CSR-stream kernel
+  bool has_crs = false;
+  int crs_nnz = 0, crs_nrb = 0;
+  int *crs_ptr = nullptr, *crs_index = nullptr, *crs_rb = nullptr;
+  double *crs_value = nullptr;
+
+  // ELL (MatrixDataELL), column-major
+  bool has_ell = false;
+  int ell_width = 0;
+  int *ell_length = nullptr, *ell_index = nullptr;
+  double *ell_data = nullptr;
+
+  bool has_jac = false;
+  double *jac = nullptr;
+
+  int format = 0;
+  int dot_mode = CGHIP_DOT_TREE;
+
+  // reduction scratch
+  double *partial = nullptr;  // 2 x kVecGMax (>= kSpmvGMax)
+  unsigned *ticket = nullptr;
+  double *dot_out = nullptr;
+
+  // solver
+  SolveState *d_state = nullptr;
+  SolveState *h_state = nullptr;  // pinned
+  double *h_scalar = nullptr;     // pinned
+  double *d_hist = nullptr;
+  int hist_cap = 0;
+  int last_iterations = 0;
+
+  // graph of `graph_batch` fused iterations
+  hipGraphExec_t graph = nullptr;
+  int graph_batch = 0, graph_format = 0, graph_precond = -1;
+
+  std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+std::string g_create_err = "no error";
+
+int fail(cghip_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c)
+    c->err = buf;
+  else
+    g_create_err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((ctx), CGHIP_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(cghip_ctx *c, T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, CGHIP_ERR_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                hipGetErrorString(e));
+  }
+  return CGHIP_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+void drop_graph(cghip_ctx *c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graph_batch = 0;
+  c->graph_precond = -1;
+}
+
+void free_crs(cghip_ctx *c) {
+  dfree(c->crs_ptr);
+  dfree(c->crs_index);
+  dfree(c->crs_value);
+  dfree(c->crs_rb);
+  c->has_crs = false;
+}
+
+void free_ell(cghip_ctx *c) {
+  dfree(c->ell_length);
+  dfree(c->ell_index);
+  dfree(c->ell_data);
+  c->has_ell = false;
+}
+
+bool valid_vec(int id) { return id >= CGHIP_VEC_K && id <= CGHIP_VEC_Z; }
+
+Reduce reduce_of(cghip_ctx *c) { return Reduce{c->partial, c->partial + kVecGMax, c->ticket}; }
+
+Crs<double> crs_of(cghip_ctx *c) {
+  return Crs<double>{c->crs_ptr, c->crs_index, c->crs_value, c->crs_rb, c->n, c->crs_nrb};
+}
+
+Ell<double> ell_of(cghip_ctx *c) {
+  return Ell<double>{c->ell_length, c->ell_index, c->ell_data, c->n, c->ell_width};
+}
+
+int check_ready(cghip_ctx *c) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no vectors: call cghip_set_size first");
+  if (c->format == CGHIP_FORMAT_CRS && !c->has_crs)
+    return fail(c, CGHIP_ERR_STATE, "CRS format selected but no CRS matrix uploaded");
+  if (c->format == CGHIP_FORMAT_ELL && !c->has_ell)
+    return fail(c, CGHIP_ERR_STATE, "ELL format selected but no ELL matrix uploaded");
+  if (c->format != CGHIP_FORMAT_CRS && c->format != CGHIP_FORMAT_ELL)
+    return fail(c, CGHIP_ERR_STATE, "no matrix format selected");
+  return CGHIP_OK;
+}
+
+hipError_t launch_matvec(cghip_ctx *c, const double *x, double *y) {
+  if (c->format == CGHIP_FORMAT_CRS) return matvec_crs(crs_of(c), x, y, c->stream);
+  return matvec_ell(ell_of(c), x, y, c->stream);
+}
+
+int sync(cghip_ctx *c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CGHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap, int *spmv_gmax) {
+  if (block) *block = kBlock;
+  if (vec_gmax) *vec_gmax = kVecGMax;
+  if (rows_max) *rows_max = kRowsMax;
+  if (nnz_cap) *nnz_cap = kNnzCap;
+  if (spmv_gmax) *spmv_gmax = kSpmvGMax;
+  return CGHIP_OK;
+}
+
+const char *cghip_last_error(const cghip_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int cghip_create(int device, cghip_ctx **out) {
+  if (!out) return fail(nullptr, CGHIP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0)
+    return fail(nullptr, CGHIP_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= count)
+    return fail(nullptr, CGHIP_ERR_ARG, "device %d out of range [0, %d)", device, count);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(nullptr, CGHIP_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  cghip_ctx *c = new cghip_ctx;
+  c->device = device;
+  int rc = CGHIP_OK;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+    rc = fail(nullptr, CGHIP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  if (!rc) rc = dalloc(c, &c->partial, 2 * (size_t)kVecGMax);
+  if (!rc) rc = dalloc(c, &c->ticket, 1);
+  if (!rc) rc = dalloc(c, &c->dot_out, 1);
+  if (!rc) rc = dalloc(c, &c->d_state, 1);
+  if (!rc && (e = hipMemset(c->ticket, 0, sizeof(unsigned))) != hipSuccess)
+    rc = fail(nullptr, CGHIP_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
+  if (!rc && (e = hipHostMalloc((void **)&c->h_state, sizeof(SolveState), 0)) != hipSuccess)
+    rc = fail(nullptr, CGHIP_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  if (!rc && (e = hipHostMalloc((void **)&c->h_scalar, sizeof(double) * 8, 0)) != hipSuccess)
+    rc = fail(nullptr, CGHIP_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  if (!rc && (e = hipDeviceSynchronize()) != hipSuccess)
+    rc = fail(nullptr, CGHIP_ERR_HIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));
+  if (rc) {
+    if (!c->err.empty()) g_create_err = c->err;
+    cghip_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return CGHIP_OK;
+}
+
+int cghip_destroy(cghip_ctx *c) {
+  if (!c) return CGHIP_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  for (auto &ev : c->events) (void)hipEventDestroy(ev);
+  for (auto &v : c->vec) dfree(v);
+  free_crs(c);
+  free_ell(c);
+  dfree(c->jac);
+  dfree(c->partial);
+  dfree(c->ticket);
+  dfree(c->dot_out);
+  dfree(c->d_state);
+  dfree(c->d_hist);
+  if (c->h_state) (void)hipHostFree(c->h_state);
+  if (c->h_scalar) (void)hipHostFree(c->h_scalar);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CGHIP_OK;
+}
+
+int cghip_set_size(cghip_ctx *c, int n) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (n <= 0) return fail(c, CGHIP_ERR_ARG, "n must be positive (got %d)", n);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  if (c->n != n) {
+    for (auto &v : c->vec) dfree(v);
+    free_crs(c);
+    free_ell(c);
+    dfree(c->jac);
+    c->has_jac = false;
+    c->format = 0;
+    c->n = 0;
+    for (auto &v : c->vec)
+      if (int rc = dalloc(c, &v, (size_t)n)) return rc;
+    c->n = n;
+  }
+  for (auto &v : c->vec) HIPCHK(c, hipMemsetAsync(v, 0, sizeof(double) * (size_t)n, c->stream));
+  return sync(c);
+}
+
+int cghip_upload_crs(cghip_ctx *c, int n, int nnz, const int32_t *ptr, const int32_t *index,
+                     const double *value) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (n <= 0 || nnz < 0 || !ptr || (nnz > 0 && (!index || !value)))
+    return fail(c, CGHIP_ERR_ARG, "invalid CRS arguments (n=%d, nnz=%d)", n, nnz);
+  if (ptr[0] != 0 || ptr[n] != nnz)
+    return fail(c, CGHIP_ERR_ARG, "CRS ptr must start at 0 and end at nnz");
+  for (int i = 0; i < n; i++)
+    if (ptr[i + 1] < ptr[i]) return fail(c, CGHIP_ERR_ARG, "CRS ptr decreases at row %d", i);
+  for (int e = 0; e < nnz; e++)
+    if (index[e] < 0 || index[e] >= n)
+      return fail(c, CGHIP_ERR_ARG, "CRS column index %d out of range at entry %d", index[e], e);
+  if (c->n != n)
+    if (int rc = cghip_set_size(c, n)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  free_crs(c);
+  std::vector<int> rb((size_t)n + 1);
+  const int nrb = crs_rowblocks(n, ptr, rb.data());
+  int rc = dalloc(c, &c->crs_ptr, (size_t)n + 1);
+  if (!rc) rc = dalloc(c, &c->crs_index, (size_t)nnz);
+  if (!rc) rc = dalloc(c, &c->crs_value, (size_t)nnz);
+  if (!rc) rc = dalloc(c, &c->crs_rb, (size_t)nrb + 1);
+  if (rc) {
+    free_crs(c);
+    return rc;
+  }
+  HIPCHK(c, hipMemcpy(c->crs_ptr, ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+  if (nnz > 0) {
+    HIPCHK(c, hipMemcpy(c->crs_index, index, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->crs_value, value, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpy(c->crs_rb, rb.data(), sizeof(int) * ((size_t)nrb + 1), hipMemcpyHostToDevice));
+  c->crs_nnz = nnz;
+  c->crs_nrb = nrb;
+  c->has_crs = true;
+  if (!c->format) c->format = CGHIP_FORMAT_CRS;
+  return CGHIP_OK;
+}
+
+int cghip_upload_ell(cghip_ctx *c, int n, int width, const int32_t *length, const int32_t *index,
+                     const double *data) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (n <= 0 || width < 0 || !length || (width > 0 && (!index || !data)))
+    return fail(c, CGHIP_ERR_ARG, "invalid ELL arguments (n=%d, width=%d)", n, width);
+  const int64_t slots = (int64_t)n * width;
+  for (int i = 0; i < n; i++) {
+    if (length[i] < 0 || length[i] > width)
+      return fail(c, CGHIP_ERR_ARG, "ELL length[%d] = %d outside [0, %d]", i, length[i], width);
+    for (int j = 0; j < length[i]; j++) {
+      const int col = index[(int64_t)j * n + i];
+      if (col < 0 || col >= n)
+        return fail(c, CGHIP_ERR_ARG, "ELL column index %d out of range (row %d, slot %d)", col, i, j);
+    }
+  }
+  if (c->n != n)
+    if (int rc = cghip_set_size(c, n)) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  free_ell(c);
+  int rc = dalloc(c, &c->ell_length, (size_t)n);
+  if (!rc) rc = dalloc(c, &c->ell_index, (size_t)slots);
+  if (!rc) rc = dalloc(c, &c->ell_data, (size_t)slots);
+  if (rc) {
+    free_ell(c);
+    return rc;
+  }
+  HIPCHK(c, hipMemcpy(c->ell_length, length, sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+  if (slots > 0) {
+    HIPCHK(c, hipMemcpy(c->ell_index, index, sizeof(int) * (size_t)slots, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->ell_data, data, sizeof(double) * (size_t)slots, hipMemcpyHostToDevice));
+  }
+  c->ell_width = width;
+  c->has_ell = true;
+  if (!c->format) c->format = CGHIP_FORMAT_ELL;
+  return CGHIP_OK;
+}
+
+int cghip_upload_jacobi(cghip_ctx *c, const double *C) {
+  if (!c || !C) return c ? fail(c, CGHIP_ERR_ARG, "C is NULL") : CGHIP_ERR_ARG;
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size: upload a matrix first");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  if (!c->jac)
+    if (int rc = dalloc(c, &c->jac, (size_t)c->n)) return rc;
+  HIPCHK(c, hipMemcpy(c->jac, C, sizeof(double) * (size_t)c->n, hipMemcpyHostToDevice));
+  c->has_jac = true;
+  return CGHIP_OK;
+}
+
+int cghip_set_format(cghip_ctx *c, int format) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (format == CGHIP_FORMAT_CRS && !c->has_crs)
+    return fail(c, CGHIP_ERR_STATE, "no CRS matrix uploaded");
+  if (format == CGHIP_FORMAT_ELL && !c->has_ell)
+    return fail(c, CGHIP_ERR_STATE, "no ELL matrix uploaded");
+  if (format != CGHIP_FORMAT_CRS && format != CGHIP_FORMAT_ELL)
+    return fail(c, CGHIP_ERR_ARG, "unsupported matrix format %d", format);
+  if (c->format != format) drop_graph(c);
+  c->format = format;
+  return CGHIP_OK;
+}
+
+int cghip_set_dot_mode(cghip_ctx *c, int mode) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (mode != CGHIP_DOT_TREE && mode != CGHIP_DOT_SERIAL)
+    return fail(c, CGHIP_ERR_ARG, "unknown dot mode %d", mode);
+  c->dot_mode = mode;
+  return CGHIP_OK;
+}
+
+int cghip_upload_vector(cghip_ctx *c, int id, const double *host) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(id) || !host) return fail(c, CGHIP_ERR_ARG, "invalid vector id %d or NULL", id);
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(c->vec[id], host, sizeof(double) * (size_t)c->n, hipMemcpyHostToDevice,
+                           c->stream));
+  return sync(c);
+}
+
+int cghip_download_vector(cghip_ctx *c, int id, double *host) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(id) || !host) return fail(c, CGHIP_ERR_ARG, "invalid vector id %d or NULL", id);
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(host, c->vec[id], sizeof(double) * (size_t)c->n, hipMemcpyDeviceToHost,
+                           c->stream));
+  return sync(c);
+}
+
+// ---- hooks ---------------------------------------------------------------
+
+int cghip_cpy(cghip_ctx *c, int dst, int src) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(dst) || !valid_vec(src)) return fail(c, CGHIP_ERR_ARG, "invalid vector id");
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  HIPCHK(c, hipMemcpyAsync(c->vec[dst], c->vec[src], sizeof(double) * (size_t)c->n,
+                           hipMemcpyDeviceToDevice, c->stream));
+  return sync(c);
+}
+
+int cghip_matvec(cghip_ctx *c, int x, int y) {
+  if (int rc = check_ready(c)) return rc;
+  if (!valid_vec(x) || !valid_vec(y) || x == y)
+    return fail(c, CGHIP_ERR_ARG, "invalid vectors for matvec (%d -> %d)", x, y);
+  HIPCHK(c, launch_matvec(c, c->vec[x], c->vec[y]));
+  return sync(c);
+}
+
+int cghip_axpy(cghip_ctx *c, double a, int x, int y) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(x) || !valid_vec(y)) return fail(c, CGHIP_ERR_ARG, "invalid vector id");
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  HIPCHK(c, axpy(c->n, a, c->vec[x], c->vec[y], c->stream));
+  return sync(c);
+}
+
+int cghip_xpay(cghip_ctx *c, int x, double a, int y) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(x) || !valid_vec(y)) return fail(c, CGHIP_ERR_ARG, "invalid vector id");
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  HIPCHK(c, xpay(c->n, c->vec[x], a, c->vec[y], c->stream));
+  return sync(c);
+}
+
+int cghip_vector_dot(cghip_ctx *c, int a, int b, double *out) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(a) || !valid_vec(b) || !out) return fail(c, CGHIP_ERR_ARG, "invalid arguments");
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  if (c->dot_mode == CGHIP_DOT_SERIAL)
+    HIPCHK(c, dot_serial(c->n, c->vec[a], c->vec[b], c->dot_out, c->stream));
+  else
+    HIPCHK(c, dot_tree(c->n, c->vec[a], c->vec[b], reduce_of(c), c->dot_out, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_scalar, c->dot_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = sync(c)) return rc;
+  *out = c->h_scalar[0];
+  return CGHIP_OK;
+}
+
+int cghip_apply_preconditioner(cghip_ctx *c, int x, int y) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(x) || !valid_vec(y)) return fail(c, CGHIP_ERR_ARG, "invalid vector id");
+  if (!c->has_jac) return fail(c, CGHIP_ERR_STATE, "no Jacobi preconditioner uploaded");
+  HIPCHK(c, jacobi(c->n, c->jac, c->vec[x], c->vec[y], c->stream));
+  return sync(c);
+}
+
+// ---- fused solver --------------------------------------------------------
+
+namespace {
+
+int enqueue_iteration(cghip_ctx *c, const double *C, History h, hipEvent_t *ev) {
+  double *p = c->vec[CGHIP_VEC_P], *q = c->vec[CGHIP_VEC_Q], *r = c->vec[CGHIP_VEC_R],
+         *x = c->vec[CGHIP_VEC_X];
+  Reduce red = reduce_of(c);
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  if (c->format == CGHIP_FORMAT_CRS)
+    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, red, c->d_state, h, c->stream));
+  else
+    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, red, c->d_state, h, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  HIPCHK(c, pcg_update(c->n, x, r, p, q, C, red, c->d_state, h, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  HIPCHK(c, pcg_xpay(c->n, p, r, C, c->d_state, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+  return CGHIP_OK;
+}
+
+int ensure_events(cghip_ctx *c, size_t count) {
+  while (c->events.size() < count) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->events.push_back(e);
+  }
+  return CGHIP_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+}  // namespace
+
+int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info) {
+  if (int rc = check_ready(c)) return rc;
+  if (!o || !info) return fail(c, CGHIP_ERR_ARG, "opts/info is NULL");
+  if (o->max_iter <= 0) return fail(c, CGHIP_ERR_ARG, "max_iter must be positive");
+  if (!(o->tol > 0)) return fail(c, CGHIP_ERR_ARG, "tol must be positive");
+  if (o->precond != CGHIP_PRECOND_NONE && o->precond != CGHIP_PRECOND_JACOBI)
+    return fail(c, CGHIP_ERR_ARG, "unsupported preconditioner %d", o->precond);
+  if (o->precond == CGHIP_PRECOND_JACOBI && !c->has_jac)
+    return fail(c, CGHIP_ERR_STATE, "Jacobi requested but no diagonal uploaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::memset(info, 0, sizeof(*info));
+
+  if (c->hist_cap < o->max_iter + 1) {
+    drop_graph(c);
+    dfree(c->d_hist);
+    c->hist_cap = 0;
+    if (int rc = dalloc(c, &c->d_hist, 5 * ((size_t)o->max_iter + 1))) return rc;
+    c->hist_cap = o->max_iter + 1;
+  }
+  const size_t hc = (size_t)c->hist_cap;
+  History h{c->d_hist, c->d_hist + hc, c->d_hist + 2 * hc, c->d_hist + 3 * hc, c->d_hist + 4 * hc};
+  const double *C = o->precond == CGHIP_PRECOND_JACOBI ? c->jac : nullptr;
+
+  SolveState init{};
+  init.tol = o->tol;
+  init.max_iter = o->max_iter;
+  *c->h_state = init;
+  HIPCHK(c, hipMemcpyAsync(c->d_state, c->h_state, sizeof(SolveState), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+
+  const bool profile = o->profile != 0;
+  int batch = o->batch > 0 ? o->batch : (c->n >= (1 << 20) ? 8 : 32);
+  if (profile) batch = 1;
+  if (profile && ensure_events(c, 6)) return CGHIP_ERR_HIP;
+
+  const auto t0 = std::chrono::steady_clock::now();
+  // r = A x; r = k - r; p = C r; rho; r2
+  if (profile) HIPCHK(c, hipEventRecord(c->events[4], c->stream));
+  HIPCHK(c, launch_matvec(c, c->vec[CGHIP_VEC_X], c->vec[CGHIP_VEC_R]));
+  HIPCHK(c, pcg_init(c->n, c->vec[CGHIP_VEC_K], c->vec[CGHIP_VEC_R], c->vec[CGHIP_VEC_P], C,
+                     reduce_of(c), c->d_state, h, c->stream));
+  if (profile) {
+    HIPCHK(c, hipEventRecord(c->events[5], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    info->ms_init = elapsed(c->events[4], c->events[5]);
+    info->n_init = 1;
+  }
+
+  if (profile) {
+    for (long nb = 0;; nb++) {
+      if (nb > (long)o->max_iter + 2)
+        return fail(c, CGHIP_ERR_STATE, "solver did not stop after %ld iterations", nb);
+      if (int rc = enqueue_iteration(c, C, h, c->events.data())) return rc;
+      HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(SolveState), hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      const SolveState &s = *c->h_state;
+      // Kernels that ran past the end return at once; count the real ones.
+      const bool ran = s.iteration > info->n_update;
+      if (ran) {
+        info->ms_matvec += elapsed(c->events[0], c->events[1]);
+        info->ms_update += elapsed(c->events[1], c->events[2]);
+        info->n_matvec++;
+        info->n_update++;
+        if (!s.done) {
+          info->ms_xpay += elapsed(c->events[2], c->events[3]);
+          info->n_xpay++;
+        }
+      }
+      if (s.done) break;
+    }
+  } else {
+    if (!c->graph || c->graph_batch != batch || c->graph_format != c->format ||
+        c->graph_precond != o->precond) {
+      drop_graph(c);
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = CGHIP_OK;
+      for (int i = 0; i < batch && !rc; i++) rc = enqueue_iteration(c, C, h, nullptr);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess)
+        return fail(c, CGHIP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->graph = nullptr;
+        return fail(c, CGHIP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      }
+      c->graph_batch = batch;
+      c->graph_format = c->format;
+      c->graph_precond = o->precond;
+    }
+    // Every batch advances the device counter by `batch` iterations until
+    // CG_MAX_ITER stops it, so this bound is never reached by a correct run.
+    const long max_batches = (long)o->max_iter / batch + 3;
+    for (long nb = 0;; nb++) {
+      if (nb >= max_batches)
+        return fail(c, CGHIP_ERR_STATE, "solver did not stop after %ld batches", nb);
+      HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(SolveState), hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (c->h_state->done) break;
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  const SolveState &s = *c->h_state;
+  c->last_iterations = s.iteration;
+  info->iterations = s.iteration;
+  info->residual = s.residual;
+  info->r2 = s.r2;
+  info->wall_s = std::chrono::duration<double>(t1 - t0).count();
+  return CGHIP_OK;
+}
+
+int cghip_history(cghip_ctx *c, int which, double *out, int cap, int *written) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (which < 0 || which > 4 || !out || cap < 0)
+    return fail(c, CGHIP_ERR_ARG, "invalid history request");
+  if (!c->d_hist) return fail(c, CGHIP_ERR_STATE, "no solve has run");
+  const int have = (which <= 1) ? c->last_iterations + 1 : c->last_iterations;
+  const int cnt = std::min(std::min(have, cap), c->hist_cap);
+  if (cnt > 0)
+    HIPCHK(c, hipMemcpy(out, c->d_hist + (size_t)which * c->hist_cap, sizeof(double) * (size_t)cnt,
+                        hipMemcpyDeviceToHost));
+  if (written) *written = cnt;
+  return CGHIP_OK;
+}
+
+int cghip_time_matvec(cghip_ctx *c, int x, int y, int reps, double *avg_ms) {
+  if (int rc = check_ready(c)) return rc;
+  if (!valid_vec(x) || !valid_vec(y) || x == y || reps <= 0 || !avg_ms)
+    return fail(c, CGHIP_ERR_ARG, "invalid arguments");
+  if (int rc = ensure_events(c, 2)) return rc;
+  HIPCHK(c, launch_matvec(c, c->vec[x], c->vec[y]));  // warm-up
+  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
+  for (int i = 0; i < reps; i++) HIPCHK(c, launch_matvec(c, c->vec[x], c->vec[y]));
+  HIPCHK(c, hipEventRecord(c->events[1], c->stream));
+  HIPCHK(c, hipEventSynchronize(c->events[1]));
+  *avg_ms = elapsed(c->events[0], c->events[1]) / reps;
+  return CGHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+/*
+ * cghip.h -- C ABI of the MI355X (gfx950) conjugate-gradient backend.
+ *
+ * This is the drop-in boundary for the hot path of CGxx: the kernel hooks of
+ * the abstract CG class (CG.h:224-237) and the device data set-up of a GPU
+ * backend (cuda/CGCUDA.cu:38-112).  The C++ backend class CGHIP
+ * (cgxx_amd/host/CGHIP.cpp) maps every hook onto one call below; the Python
+ * package binds the same symbols with ctypes.
+ *
+ * Conventions
+ *   - Every function returns an int status: CGHIP_OK (0) or a CGHIP_ERR_*.
+ *     cghip_last_error(ctx) returns a message for the last failure.  Nothing
+ *     throws across the ABI and nothing exits the process.
+ *   - Vectors are named by id (CG::Vector, CG.h:22-35), never by pointer.
+ *     The context owns every device buffer; the caller owns host buffers.
+ *   - One HIP stream per context.  Every call returns after its device work
+ *     is complete (the reference's hooks are synchronous: CGCUDA.cu:139-140).
+ *   - A context is not thread-safe; one host thread drives it, as one host
+ *     thread drives CG::solve (CG.cpp:284-385).
+ *   - Sizes are int (int32 indices, as Matrix.h:16-18, 50-54, 77-86);
+ *     ELL slot offsets are computed in 64 bits.
+ */
+#ifndef CGHIP_H
+#define CGHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGHIP_OK 0
+#define CGHIP_ERR_ARG 1     /* invalid argument or malformed matrix */
+#define CGHIP_ERR_HIP 2     /* a HIP runtime call failed */
+#define CGHIP_ERR_STATE 3   /* call out of order (e.g. no matrix uploaded) */
+#define CGHIP_ERR_MEMORY 4  /* device allocation failed */
+
+/* CG::Vector (CG.h:22-35). */
+enum {
+  CGHIP_VEC_K = 0,
+  CGHIP_VEC_X = 1,
+  CGHIP_VEC_P = 2,
+  CGHIP_VEC_Q = 3,
+  CGHIP_VEC_R = 4,
+  CGHIP_VEC_Z = 5
+};
+
+/* CG::MatrixFormat (CG.h:38-47).  COO is a host-only format (the reference
+ * runs it only in SerialCG, serial/SerialCG.cpp:79-85). */
+enum { CGHIP_FORMAT_CRS = 1, CGHIP_FORMAT_ELL = 2 };
+
+/* CG::Preconditioner (CG.h:50-55). */
+enum { CGHIP_PRECOND_NONE = 0, CGHIP_PRECOND_JACOBI = 1 };
+
+/* Summation order of the dot-product hook.  TREE: deterministic single-pass
+ * wave64 tree (fixed order for a given length).  SERIAL: left to right, the
+ * order of SerialCG::vectorDotKernel (serial/SerialCG.cpp:145-155); slow,
+ * for bit-exact parity runs. */
+enum { CGHIP_DOT_TREE = 0, CGHIP_DOT_SERIAL = 1 };
+
+typedef struct cghip_ctx cghip_ctx;
+
+/* --- context ------------------------------------------------------------ */
+
+/* Replaces CGCUDA::init's device selection (cuda/CGCUDA.cu:29-36). */
+int cghip_create(int device, cghip_ctx **out);
+int cghip_destroy(cghip_ctx *ctx);
+/* Message of the last failed call on ctx (ctx may be NULL: last failure of
+ * cghip_create). Never NULL. */
+const char *cghip_last_error(const cghip_ctx *ctx);
+/* Reduction geometry of the kernels (the summation order is a function of
+ * these and of the vector length / CRS row pointer only). */
+int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap,
+                   int *spmv_gmax);
+
+/* --- data (CGCUDA::doTransferTo / doTransferFrom, cuda/CGCUDA.cu:38-112) - */
+
+/* Allocates the six N-vectors K, X, P, Q, R, Z (zeroed). */
+int cghip_set_size(cghip_ctx *ctx, int n);
+/* CRS (MatrixDataCRS, Matrix.h:48-71): ptr[n+1], index[nnz], value[nnz].
+ * Validated on the host (monotone ptr, ptr[0] = 0, indices in [0, n)). */
+int cghip_upload_crs(cghip_ctx *ctx, int n, int nnz, const int32_t *ptr,
+                     const int32_t *index, const double *value);
+/* ELL (MatrixDataELL, Matrix.h:74-103), column-major: slot (j, i) at
+ * j * n + i for j < width; length[i] <= width live slots per row. */
+int cghip_upload_ell(cghip_ctx *ctx, int n, int width, const int32_t *length,
+                     const int32_t *index, const double *data);
+/* Jacobi::C (Preconditioner.h:12-25): C[i] = 1 / A_ii. */
+int cghip_upload_jacobi(cghip_ctx *ctx, const double *C);
+/* Chooses which uploaded matrix the hooks and the solver use. */
+int cghip_set_format(cghip_ctx *ctx, int format);
+int cghip_upload_vector(cghip_ctx *ctx, int id, const double *host);
+int cghip_download_vector(cghip_ctx *ctx, int id, double *host);
+int cghip_set_dot_mode(cghip_ctx *ctx, int mode);
+
+/* --- hooks (CG.h:224-237) ------------------------------------------------- */
+
+int cghip_cpy(cghip_ctx *ctx, int dst, int src);               /* CG.h:224 */
+int cghip_matvec(cghip_ctx *ctx, int x, int y);                /* CG.h:226 y = A x */
+int cghip_axpy(cghip_ctx *ctx, double a, int x, int y);        /* CG.h:228 y = a x + y */
+int cghip_xpay(cghip_ctx *ctx, int x, double a, int y);        /* CG.h:230 y = x + a y */
+int cghip_vector_dot(cghip_ctx *ctx, int a, int b, double *out); /* CG.h:232 */
+int cghip_apply_preconditioner(cghip_ctx *ctx, int x, int y);  /* CG.h:235 y = C o x */
+
+/* --- fused solver ----------------------------------------------------------
+ * The whole loop of CG::solve (CG.cpp:284-385) on the device: three kernels
+ * per iteration (SpMV + <p,q>; x/r update + <r,r> + <r,z>; p = C r + b p),
+ * alpha, beta and the stop test computed by the last block of each
+ * reduction, iterations replayed from a HIP graph, no host round trip per
+ * iteration.  Uses X (x0 on entry, solution on return) and K (RHS). */
+typedef struct {
+  int precond;        /* CGHIP_PRECOND_* */
+  int max_iter;       /* CG_MAX_ITER, > 0 */
+  double tol;         /* CG_TOLERANCE: stop when sqrt(r2)/sqrt(r2_0) <= tol */
+  int batch;          /* iterations per graph launch; 0 = automatic */
+  int profile;        /* 1: launch kernel by kernel and time each with events */
+} cghip_solve_opts;
+
+typedef struct {
+  int iterations;     /* loop bodies run, counting the converging one */
+  double residual;    /* last sqrt(r2)/sqrt(r2_0) */
+  double r2;          /* last <r,r> */
+  double wall_s;      /* host wall time of the solve, device synchronised */
+  /* profile = 1 only: summed kernel time (ms) and launch counts */
+  double ms_matvec, ms_update, ms_xpay, ms_init;
+  int n_matvec, n_update, n_xpay, n_init;
+} cghip_solve_info;
+
+int cghip_solve(cghip_ctx *ctx, const cghip_solve_opts *opts,
+                cghip_solve_info *info);
+
+/* Per-iteration scalars of the last cghip_solve: which = 0 r2, 1 rho,
+ * 2 dot_pq, 3 alpha, 4 beta.  r2/rho have iterations + 1 entries (index 0 is
+ * before the loop), the others iterations.  Returns the count written. */
+int cghip_history(cghip_ctx *ctx, int which, double *out, int cap,
+                  int *written);
+
+/* Average duration (ms) of `reps` back-to-back launches of the matvec kernel
+ * x -> y, timed with HIP events on the context's stream. */
+int cghip_time_matvec(cghip_ctx *ctx, int x, int y, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,547 @@
+/*
+ * CPU oracle for the CGxx conjugate-gradient hot path.
+ * TEST INFRASTRUCTURE ONLY -- the header cg_oracle.h says what it restates
+ * and how each part is pinned.  Nothing in the product links or calls it.
+ */
+#include "cg_oracle.h"
+
+#include <ctype.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Matrix Market reader: Matrix.cpp:13-111.                                  */
+
+static void lower(char *s) {
+  for (; *s; s++) *s = (char)tolower((unsigned char)*s);
+}
+
+/* getline(): strips the trailing newline, keeps everything else. */
+static int read_line(FILE *f, char **buf, size_t *cap) {
+  ssize_t n = getline(buf, cap, f);
+  if (n < 0) return 0;
+  if (n > 0 && (*buf)[n - 1] == '\n') (*buf)[n - 1] = 0;
+  return 1;
+}
+
+int orc_mm_read(const char *path, orc_coo *out) {
+  memset(out, 0, sizeof(*out));
+  FILE *f = fopen(path, "r");
+  if (!f) return 1;
+  char *line = NULL;
+  size_t cap = 0;
+  char banner[256] = "", mtx[256] = "", crd[256] = "", type[256] = "",
+       storage[256] = "";
+  int rc = 0;
+  if (!read_line(f, &line, &cap)) line = strdup("");
+  sscanf(line, "%255s %255s %255s %255s %255s", banner, mtx, crd, type,
+         storage);
+  lower(mtx);
+  lower(crd);
+  lower(type);
+  lower(storage);
+  /* Matrix.cpp:35-40 */
+  if (strcmp(banner, "%%MatrixMarket") || strcmp(mtx, "matrix") ||
+      strcmp(crd, "coordinate") || strcmp(type, "real")) {
+    rc = 2;
+    goto done;
+  }
+  /* Matrix.cpp:42-49 */
+  int symmetric = 0;
+  if (!strcmp(storage, "symmetric"))
+    symmetric = 1;
+  else if (strcmp(storage, "general")) {
+    rc = 3;
+    goto done;
+  }
+  /* Matrix.cpp:51-54: skip empty and comment lines */
+  do {
+    if (!read_line(f, &line, &cap)) {
+      rc = 5;
+      goto done;
+    }
+  } while (line[0] == 0 || line[0] == '%');
+  int M = 0, N = 0, nz = 0;
+  if (sscanf(line, "%d %d %d", &M, &N, &nz) != 3) {
+    rc = 5;
+    goto done;
+  }
+  if (N != M) { /* Matrix.cpp:62-65 */
+    rc = 4;
+    goto done;
+  }
+  if (symmetric) nz = 2 * nz - N; /* Matrix.cpp:67-70 */
+  out->N = N;
+  out->nz = nz;
+  out->I = (int *)malloc(sizeof(int) * (size_t)(nz > 0 ? nz : 1));
+  out->J = (int *)malloc(sizeof(int) * (size_t)(nz > 0 ? nz : 1));
+  out->V = (double *)malloc(sizeof(double) * (size_t)(nz > 0 ? nz : 1));
+  out->nzPerRow = (int *)calloc((size_t)(N > 0 ? N : 1), sizeof(int));
+  /* Matrix.cpp:80-104: 1-based -> 0-based, mirror off-diagonals in place */
+  for (int i = 0; i < nz; i++) {
+    if (!read_line(f, &line, &cap)) {
+      rc = 5;
+      goto done;
+    }
+    int a, b;
+    double v;
+    if (sscanf(line, "%d %d %lf", &a, &b, &v) != 3) {
+      rc = 5;
+      goto done;
+    }
+    a--;
+    b--;
+    if (a < 0 || a >= N || b < 0 || b >= N) {
+      rc = 5;
+      goto done;
+    }
+    out->I[i] = a;
+    out->J[i] = b;
+    out->V[i] = v;
+    out->nzPerRow[a]++;
+    if (symmetric && a != b) {
+      i++;
+      if (i >= nz) {
+        rc = 5;
+        goto done;
+      }
+      out->I[i] = b;
+      out->J[i] = a;
+      out->V[i] = v;
+      out->nzPerRow[b]++;
+    }
+  }
+done:
+  free(line);
+  fclose(f);
+  if (rc) orc_coo_free(out);
+  return rc;
+}
+
+void orc_coo_free(orc_coo *c) {
+  free(c->I);
+  free(c->J);
+  free(c->V);
+  free(c->nzPerRow);
+  memset(c, 0, sizeof(*c));
+}
+
+/* Matrix.cpp:113-121 */
+int orc_max_nz(const orc_coo *c) {
+  int m = 0;
+  for (int i = 0; i < c->N; i++)
+    if (c->nzPerRow[i] > m) m = c->nzPerRow[i];
+  return m;
+}
+
+/* Matrix.cpp:203-228: exclusive scan of nzPerRow, then a stable scatter. */
+void orc_coo_to_crs(const orc_coo *c, int *ptr, int *index, double *value) {
+  int N = c->N;
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(N > 0 ? N : 1));
+  ptr[0] = 0;
+  for (int i = 0; i < N; i++) {
+    fill[i] = ptr[i];
+    ptr[i + 1] = ptr[i] + c->nzPerRow[i];
+  }
+  for (int e = 0; e < c->nz; e++) {
+    int r = c->I[e];
+    index[fill[r]] = c->J[e];
+    value[fill[r]] = c->V[e];
+    fill[r]++;
+  }
+  free(fill);
+}
+
+/* Matrix.cpp:339-361, with 64-bit slot offsets (slot = j * N + i). */
+void orc_coo_to_ell(const orc_coo *c, int maxNz, int *length, int *index,
+                    double *data) {
+  int N = c->N;
+  int64_t slots = (int64_t)N * maxNz;
+  memset(index, 0, sizeof(int) * (size_t)slots);
+  memset(data, 0, sizeof(double) * (size_t)slots);
+  memcpy(length, c->nzPerRow, sizeof(int) * (size_t)N);
+  int *fill = (int *)calloc((size_t)(N > 0 ? N : 1), sizeof(int));
+  for (int e = 0; e < c->nz; e++) {
+    int r = c->I[e];
+    int64_t s = (int64_t)fill[r] * N + r;
+    index[s] = c->J[e];
+    data[s] = c->V[e];
+    fill[r]++;
+  }
+  free(fill);
+}
+
+/* Preconditioner.cpp:13-23: C[i] = 1 / A_ii, the last diagonal entry wins. */
+int orc_jacobi_init(const orc_coo *c, double *C) {
+  int N = c->N;
+  for (int i = 0; i < N; i++) C[i] = NAN;
+  for (int e = 0; e < c->nz; e++)
+    if (c->I[e] == c->J[e]) C[c->I[e]] = 1 / c->V[e];
+  int missing = 0;
+  for (int i = 0; i < N; i++)
+    if (isnan(C[i])) missing++;
+  return missing;
+}
+
+/* CG.cpp:256-261: k = A * 1 accumulated in COO order. */
+void orc_rhs_ones(const orc_coo *c, double *k) {
+  memset(k, 0, sizeof(double) * (size_t)c->N);
+  for (int e = 0; e < c->nz; e++) k[c->I[e]] += c->V[e];
+}
+
+/* CG.cpp:387-408 */
+int orc_check(int N, const double *x, double checkTolerance) {
+  int errors = 0;
+  for (int i = 0; i < N; i++)
+    if (fabs(x[i] - 1.0) > checkTolerance) errors++;
+  return errors;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Serial kernels: serial/SerialCG.cpp:79-161.                               */
+
+void orc_spmv_coo(const orc_coo *c, const double *x, double *y) {
+  memset(y, 0, sizeof(double) * (size_t)c->N);
+  for (int e = 0; e < c->nz; e++) y[c->I[e]] += c->V[e] * x[c->J[e]];
+}
+
+void orc_spmv_crs(int N, const int *ptr, const int *index, const double *value,
+                  const double *x, double *y) {
+  for (int i = 0; i < N; i++) {
+    double t = 0;
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) t += value[j] * x[index[j]];
+    y[i] = t;
+  }
+}
+
+void orc_spmv_ell(int N, const int *length, const int *index,
+                  const double *data, const double *x, double *y) {
+  for (int i = 0; i < N; i++) {
+    double t = 0;
+    for (int j = 0; j < length[i]; j++) {
+      int64_t s = (int64_t)j * N + i;
+      t += data[s] * x[index[s]];
+    }
+    y[i] = t;
+  }
+}
+
+void orc_axpy(int N, double a, const double *x, double *y) {
+  for (int i = 0; i < N; i++) y[i] += a * x[i];
+}
+
+void orc_xpay(int N, const double *x, double a, double *y) {
+  for (int i = 0; i < N; i++) y[i] = x[i] + a * y[i];
+}
+
+void orc_jacobi_apply(int N, const double *C, const double *x, double *y) {
+  for (int i = 0; i < N; i++) y[i] = C[i] * x[i];
+}
+
+double orc_dot_serial(int N, const double *a, const double *b) {
+  double r = 0;
+  for (int i = 0; i < N; i++) r += a[i] * b[i];
+  return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* OpenMP kernels: openmp/CGOpenMP.cpp:188-265 (order depends on threads).   */
+
+static void omp_spmv_crs(int N, const int *ptr, const int *index,
+                         const double *value, const double *x, double *y,
+                         int nt) {
+#pragma omp parallel for num_threads(nt)
+  for (int i = 0; i < N; i++) {
+    double t = 0;
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) t += value[j] * x[index[j]];
+    y[i] = t;
+  }
+}
+
+static void omp_spmv_ell(int N, const int *length, const int *index,
+                         const double *data, const double *x, double *y,
+                         int nt) {
+#pragma omp parallel for num_threads(nt)
+  for (int i = 0; i < N; i++) {
+    double t = 0;
+    for (int j = 0; j < length[i]; j++) {
+      int64_t s = (int64_t)j * N + i;
+      t += data[s] * x[index[s]];
+    }
+    y[i] = t;
+  }
+}
+
+static void omp_axpy(int N, double a, const double *x, double *y, int nt) {
+#pragma omp parallel for num_threads(nt)
+  for (int i = 0; i < N; i++) y[i] += a * x[i];
+}
+
+static void omp_xpay(int N, const double *x, double a, double *y, int nt) {
+#pragma omp parallel for num_threads(nt)
+  for (int i = 0; i < N; i++) y[i] = x[i] + a * y[i];
+}
+
+static void omp_jacobi(int N, const double *C, const double *x, double *y,
+                       int nt) {
+#pragma omp parallel for num_threads(nt)
+  for (int i = 0; i < N; i++) y[i] = C[i] * x[i];
+}
+
+static double omp_dot(int N, const double *a, const double *b, int nt) {
+  double r = 0;
+#pragma omp parallel for reduction(+ : r) num_threads(nt)
+  for (int i = 0; i < N; i++) r += a[i] * b[i];
+  return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tree orders of the HIP kernels.                                           */
+/* A block of 256 threads = 4 wave64s.  Each wave folds its 64 values with   */
+/* shift-down steps 32,16,8,4,2,1 (lane l += lane l+off); lane 0 of each     */
+/* wave lands in shared memory and thread 0 adds the four wave sums left to  */
+/* right.  Block partials are reduced by the last block to finish: thread t  */
+/* adds partials t, t+256, ... left to right, then the same block fold.      */
+
+static double wave_fold(double *t) {
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int l = 0; l < off; l++) t[l] = t[l] + t[l + off];
+  return t[0];
+}
+
+static double block_fold(const double *v) {
+  double t[64], w[4];
+  for (int k = 0; k < 4; k++) {
+    memcpy(t, v + 64 * k, sizeof(t));
+    w[k] = wave_fold(t);
+  }
+  return ((w[0] + w[1]) + w[2]) + w[3];
+}
+
+static double final_fold(const double *partial, int G) {
+  double v[ORC_BLOCK];
+  for (int t = 0; t < ORC_BLOCK; t++) {
+    double s = 0.0;
+    for (int i = t; i < G; i += ORC_BLOCK) s += partial[i];
+    v[t] = s;
+  }
+  return block_fold(v);
+}
+
+int orc_vec_grid(int N) {
+  int64_t g = ((int64_t)N + ORC_BLOCK - 1) / ORC_BLOCK;
+  if (g > ORC_VEC_GMAX) g = ORC_VEC_GMAX;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+/* Grid-stride kernels: thread g*256+t visits i = g*256+t, +T, +2T, ... with
+ * T = 256 * G, adding its products left to right. */
+static double vec_tree(int N, const double *a, const double *b,
+                       const double *c /* nullable: a*(c*b) */) {
+  int G = orc_vec_grid(N);
+  int64_t T = (int64_t)G * ORC_BLOCK;
+  double *partial = (double *)malloc(sizeof(double) * (size_t)G);
+  double v[ORC_BLOCK];
+  for (int g = 0; g < G; g++) {
+    for (int t = 0; t < ORC_BLOCK; t++) {
+      double s = 0.0;
+      for (int64_t i = (int64_t)g * ORC_BLOCK + t; i < N; i += T) {
+        double bi = c ? c[i] * b[i] : b[i];
+        s += a[i] * bi;
+      }
+      v[t] = s;
+    }
+    partial[g] = block_fold(v);
+  }
+  double r = final_fold(partial, G);
+  free(partial);
+  return r;
+}
+
+double orc_dot_tree(int N, const double *a, const double *b) {
+  return vec_tree(N, a, b, NULL);
+}
+
+/* Row blocks of the CRS SpMV: greedy, at most ORC_ROWS_MAX rows and at most
+ * ORC_NNZ_CAP nonzeros, except that a block always takes its first row. */
+int orc_crs_rowblocks(int N, const int *ptr, int *starts) {
+  int nb = 0, r = 0;
+  while (r < N) {
+    int r0 = r;
+    r++;
+    while (r < N && r - r0 < ORC_ROWS_MAX && ptr[r + 1] - ptr[r0] <= ORC_NNZ_CAP)
+      r++;
+    starts[nb++] = r0;
+  }
+  starts[nb] = N;
+  return nb;
+}
+
+/* SpMV blocks: G = min(nrb, 2048) blocks; block g owns row blocks
+ * [g*nrb/G, (g+1)*nrb/G); thread t owns row start+t of each of them. */
+double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
+                             const double *q) {
+  if (nrb <= 0) return 0.0;
+  int G = nrb < ORC_SPMV_GMAX ? nrb : ORC_SPMV_GMAX;
+  double *partial = (double *)malloc(sizeof(double) * (size_t)G);
+  double v[ORC_BLOCK];
+  for (int g = 0; g < G; g++) {
+    int lo = (int)((int64_t)g * nrb / G), hi = (int)((int64_t)(g + 1) * nrb / G);
+    for (int t = 0; t < ORC_BLOCK; t++) v[t] = 0.0;
+    for (int rb = lo; rb < hi; rb++)
+      for (int t = 0; t < ORC_BLOCK; t++) {
+        int r = starts[rb] + t;
+        if (r < starts[rb + 1]) v[t] += p[r] * q[r];
+      }
+    partial[g] = block_fold(v);
+  }
+  double r = final_fold(partial, G);
+  free(partial);
+  return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PCG: CG.cpp:284-385.                                                      */
+
+typedef struct {
+  const orc_problem *p;
+  int *starts; /* row blocks of the SpMV (orders 1) */
+  int nrb;
+} ctx_t;
+
+static void spmv(const ctx_t *c, const double *x, double *y) {
+  const orc_problem *p = c->p;
+  int nt = p->nthreads > 0 ? p->nthreads : 1;
+  switch (p->format) {
+  case 0:
+    orc_spmv_coo(p->coo, x, y);
+    break;
+  case 1:
+    if (p->order == 3)
+      omp_spmv_crs(p->N, p->ptr, p->index, p->value, x, y, nt);
+    else
+      orc_spmv_crs(p->N, p->ptr, p->index, p->value, x, y);
+    break;
+  default:
+    if (p->order == 3)
+      omp_spmv_ell(p->N, p->length, p->ell_index, p->ell_data, x, y, nt);
+    else
+      orc_spmv_ell(p->N, p->length, p->ell_index, p->ell_data, x, y);
+    break;
+  }
+}
+
+static double dot(const ctx_t *c, const double *a, const double *b) {
+  const orc_problem *p = c->p;
+  if (p->order == 0) return orc_dot_serial(p->N, a, b);
+  if (p->order == 3)
+    return omp_dot(p->N, a, b, p->nthreads > 0 ? p->nthreads : 1);
+  return orc_dot_tree(p->N, a, b);
+}
+
+static void axpy(const ctx_t *c, double a, const double *x, double *y) {
+  if (c->p->order == 3)
+    omp_axpy(c->p->N, a, x, y, c->p->nthreads > 0 ? c->p->nthreads : 1);
+  else
+    orc_axpy(c->p->N, a, x, y);
+}
+
+static void xpay(const ctx_t *c, const double *x, double a, double *y) {
+  if (c->p->order == 3)
+    omp_xpay(c->p->N, x, a, y, c->p->nthreads > 0 ? c->p->nthreads : 1);
+  else
+    orc_xpay(c->p->N, x, a, y);
+}
+
+static void jacobi(const ctx_t *c, const double *x, double *y) {
+  if (c->p->order == 3)
+    omp_jacobi(c->p->N, c->p->C, x, y, c->p->nthreads > 0 ? c->p->nthreads : 1);
+  else
+    orc_jacobi_apply(c->p->N, c->p->C, x, y);
+}
+
+int orc_pcg(const orc_problem *p, const double *k, double *x, orc_result *res,
+            double *h_r2, double *h_rho, double *h_pq, double *h_alpha,
+            double *h_beta) {
+  int N = p->N;
+  size_t bytes = sizeof(double) * (size_t)(N > 0 ? N : 1);
+  double *P = (double *)malloc(bytes), *Q = (double *)malloc(bytes),
+         *R = (double *)malloc(bytes), *Z = (double *)malloc(bytes);
+  ctx_t c = {p, NULL, 0};
+  int fused = p->order == 1;
+  if (fused) {
+    if (p->format == 1) {
+      c.starts = (int *)malloc(sizeof(int) * (size_t)(N + 1));
+      c.nrb = orc_crs_rowblocks(N, p->ptr, c.starts);
+    } else if (p->format == 2) {
+      c.nrb = (N + ORC_ROWS_MAX - 1) / ORC_ROWS_MAX;
+      c.starts = (int *)malloc(sizeof(int) * (size_t)(c.nrb + 1));
+      for (int b = 0; b <= c.nrb; b++)
+        c.starts[b] = b * ORC_ROWS_MAX < N ? b * ORC_ROWS_MAX : N;
+    } else {
+      free(P), free(Q), free(R), free(Z);
+      return -1;
+    }
+  }
+  int jac = p->precond == 1;
+  double rho, rho_old, r2, nrm2_0, dot_pq, a, b;
+  double residual = 0;
+  int it;
+
+  /* r(0) = k - A x(0) */
+  spmv(&c, x, R);
+  xpay(&c, k, -1.0, R);
+  if (!jac)
+    memcpy(P, R, bytes);
+  else
+    jacobi(&c, R, P);
+  if (fused) {
+    /* one pass: <r,r> and <p,r> in the vector-kernel tree */
+    r2 = orc_dot_tree(N, R, R);
+    nrm2_0 = sqrt(r2);
+    rho = jac ? orc_dot_tree(N, P, R) : r2;
+  } else {
+    r2 = dot(&c, R, R);
+    nrm2_0 = sqrt(r2);
+    rho = jac ? dot(&c, P, R) : r2;
+  }
+  if (h_r2) h_r2[0] = r2;
+  if (h_rho) h_rho[0] = rho;
+
+  for (it = 0; it < p->max_iter; it++) {
+    spmv(&c, P, Q);
+    dot_pq = fused ? orc_rowblock_dot_tree(c.nrb, c.starts, P, Q)
+                   : dot(&c, P, Q);
+    a = rho / dot_pq;
+    if (h_pq) h_pq[it] = dot_pq;
+    if (h_alpha) h_alpha[it] = a;
+    axpy(&c, a, P, x);
+    axpy(&c, -a, Q, R);
+    r2 = fused ? orc_dot_tree(N, R, R) : dot(&c, R, R);
+    if (h_r2) h_r2[it + 1] = r2;
+    residual = sqrt(r2) / nrm2_0;
+    if (residual <= p->tol) {
+      it++;
+      break;
+    }
+    rho_old = rho;
+    if (!jac) {
+      rho = r2;
+    } else {
+      jacobi(&c, R, Z);
+      rho = fused ? vec_tree(N, R, R, p->C) : dot(&c, R, Z);
+    }
+    if (h_rho) h_rho[it + 1] = rho;
+    b = rho / rho_old;
+    if (h_beta) h_beta[it] = b;
+    xpay(&c, jac ? Z : R, b, P);
+  }
+  res->iterations = it;
+  res->residual = residual;
+  free(P), free(Q), free(R), free(Z);
+  free(c.starts);
+  return 0;
+}

@@ -1,0 +1,133 @@
+"""Regenerates the golden fixtures in this directory from the reference itself.
+
+Run in the development container, where the reference sources exist:
+
+    make -C oracle ref            # builds oracle/_ref/cg_serial_trace
+    python tests/golden/make_golden.py
+
+``cg_serial_trace`` is the reference's serial binary (serial/SerialCG.cpp +
+CG.cpp + Matrix.cpp + Preconditioner.cpp + WorkDistribution.cpp) compiled
+with its own DEBUG_SOLVE switch and std::cout raised to 17 significant
+digits, so every printed r2, rho, dot_pq, a and b is exact.  For each case
+this script writes the matrix with cgxx_amd.problems.write_mm, runs the
+binary under the given CG_* environment, and stores the full per-iteration
+history, the iteration count, the printed residual, the first ten entries of
+x and the check line as JSON (floats as C99 hex strings).
+
+The fixtures are data: inputs are named by generator and parameters (plus the
+sha256 of the .mtx text), outputs are what the reference printed.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from cgxx_amd import problems  # noqa: E402
+
+BIN = os.path.join(REPO, "oracle", "_ref", "cg_serial_trace")
+
+# (case name, generator name, generator kwargs, symmetric file, runs)
+# runs: (CG_MATRIX_FORMAT, CG_PRECONDITIONER, extra env)
+CASES = [
+    ("lap2d_16", "laplace2d", {"side": 16}, False,
+     [("CRS", "jacobi", {}), ("CRS", "none", {}), ("ELL", "jacobi", {}),
+      ("ELL", "none", {}), ("COO", "jacobi", {}), ("COO", "none", {})]),
+    ("lap2d_64", "laplace2d", {"side": 64}, False,
+     [("CRS", "jacobi", {}), ("ELL", "none", {})]),
+    ("lap2d_256", "laplace2d", {"side": 256}, False,
+     [("CRS", "jacobi", {}), ("ELL", "jacobi", {}), ("CRS", "none", {})]),
+    ("lap3d_16", "laplace3d", {"side": 16}, False,
+     [("CRS", "jacobi", {}), ("ELL", "jacobi", {})]),
+    ("rand_sym_40", "random_spd_dense", {"n": 40, "density": 0.3, "seed": 7}, True,
+     [("CRS", "jacobi", {}), ("ELL", "jacobi", {}), ("COO", "jacobi", {}),
+      ("CRS", "none", {})]),
+    ("rand_gen_300", "random_spd_dense", {"n": 300, "density": 0.05, "seed": 11}, False,
+     [("CRS", "jacobi", {}), ("ELL", "none", {})]),
+    ("diff3d_24", "diffusion3d", {"side": 24, "sigma": 1.0, "corr": 4.0, "seed": 5}, False,
+     [("CRS", "jacobi", {"CG_MAX_ITER": "5000"}), ("ELL", "jacobi", {"CG_MAX_ITER": "5000"})]),
+    ("zipf_3000", "zipf_spd", {"n": 3000, "alpha": 2.0, "cap": 200, "seed": 3}, False,
+     [("CRS", "jacobi", {}), ("ELL", "jacobi", {})]),
+    ("lap2d_64_cap", "laplace2d", {"side": 64}, False,
+     [("CRS", "jacobi", {"CG_MAX_ITER": "37"}), ("CRS", "jacobi", {"CG_TOLERANCE": "1e-4"})]),
+    ("lap3d_128", "laplace3d", {"side": 128}, False,
+     [("CRS", "jacobi", {}), ("ELL", "jacobi", {})]),
+]
+
+
+def parse(stdout: str) -> dict:
+    """Split the traced run's stdout into the loop's scalar sequences."""
+    out = {"r2": [], "rho0": None, "dot_pq": [], "a": [], "b": [], "x_head": []}
+    lines = stdout.splitlines()
+    solving = False
+    for ln in lines:
+        if ln.startswith("Solving..."):
+            solving = True
+            continue
+        if not solving:
+            continue
+        for key in ("r2", "rho", "dot_pq", "a", "b"):
+            pre = key + " = "
+            if ln.startswith(pre):
+                v = float(ln[len(pre):])
+                if key == "rho":
+                    out["rho0"] = v.hex()
+                else:
+                    out[key].append(v.hex())
+        if ln.startswith("x = [ "):
+            out["x_head"] = [float(t).hex() for t in ln[6:].split() if t != "]"]
+        if ln.startswith("Iterations:"):
+            out["iterations"] = int(ln.split()[-1])
+        if ln.startswith("Residual:"):
+            out["residual_printed"] = ln.split()[-1]
+        if ln.endswith("Solution is correct!") or ln.endswith("elements are incorrect!"):
+            out["check"] = ln.strip()
+        if ln.startswith("Checking solution..."):
+            solving = True
+    return out
+
+
+def main() -> None:
+    if not os.path.exists(BIN):
+        sys.exit(f"missing {BIN}: run `make -C oracle ref` first")
+    only = set(sys.argv[1:])
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, gen, kw, sym, runs in CASES:
+            if only and name not in only:
+                continue
+            m = getattr(problems, gen)(**kw)
+            path = os.path.join(tmp, name + ".mtx")
+            sha = problems.write_mm(path, m, symmetric=sym)
+            fixture = {
+                "case": name,
+                "generator": gen,
+                "kwargs": kw,
+                "symmetric": sym,
+                "mtx_sha256": sha,
+                "N": m.n,
+                "nz_file": int(m.nnz),
+                "runs": [],
+            }
+            for fmt, pre, extra in runs:
+                env = dict(os.environ)
+                env.update({"CG_MATRIX_FORMAT": fmt, "CG_PRECONDITIONER": pre})
+                env.update(extra)
+                r = subprocess.run([BIN, path], env=env, capture_output=True, text=True, check=True)
+                rec = parse(r.stdout)
+                rec["env"] = {"CG_MATRIX_FORMAT": fmt, "CG_PRECONDITIONER": pre, **extra}
+                fixture["runs"].append(rec)
+                print(f"{name} {fmt} {pre} {extra}: {rec['iterations']} iterations, {rec.get('check')}",
+                      flush=True)
+            with open(os.path.join(HERE, name + ".json"), "w") as f:
+                json.dump(fixture, f, indent=0)
+            os.remove(path)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,243 @@
+"""Pins the CPU oracle before anything is checked against it.
+
+1. Bit-exact against the reference's own serial binary (golden fixtures).
+2. Brute force, closed forms and CG invariants for what the fixtures leave
+   open -- chosen so that a dropped term, a wrong sign or index or a swapped
+   operand fails one of them.
+3. The tree orders restated from the HIP kernels: exact where every partial
+   sum is exact, bounded by gamma_N * sum|a_i b_i| otherwise.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cgxx_amd import problems
+from oracle import oracle
+from tests.cases import Case, assert_bitwise, golden_sequences, max_rel, run_env
+from tests.conftest import golden_runs, run_id
+
+SMALL = golden_runs(max_n=300_000)
+LARGE = [p for p in golden_runs() if p[0]["N"] > 300_000]
+
+
+@pytest.mark.parametrize("pair", SMALL, ids=[run_id(p) for p in SMALL])
+def test_serial_order_matches_reference_bitwise(pair, tmp_path):
+    fx, run = pair
+    case = Case(fx, str(tmp_path))
+    fmt, pre, maxit, tol = run_env(run)
+    g = golden_sequences(run)
+    x, it, res, h = oracle.pcg(n=case.n, fmt=fmt, precond=pre, order=0, k=case.k,
+                               tol=tol, max_iter=maxit, crs=case.crs, ell=case.ell,
+                               mm=case.mm, C_jac=case.C)
+    assert it == run["iterations"]
+    assert_bitwise(h["r2"], g["r2"], "r2")
+    assert_bitwise(h["rho"][0], g["rho0"], "rho0")
+    assert_bitwise(h["pq"], g["pq"], "dot_pq")
+    assert_bitwise(h["alpha"], g["alpha"], "a")
+    nb = g["beta"].size
+    assert_bitwise(h["beta"][:nb], g["beta"], "b")
+    assert_bitwise(x[: g["x_head"].size], g["x_head"], "x[:10]")
+    assert f"{res:e}" == run["residual_printed"]
+    errors = oracle.lib().orc_check(case.n, x, 1e-5)
+    expect = "Solution is correct!" if errors == 0 else f"{errors} elements are incorrect!"
+    assert run["check"] == expect
+
+
+@pytest.mark.parametrize("pair", LARGE, ids=[run_id(p) for p in LARGE])
+def test_serial_order_matches_reference_large(pair, tmp_path):
+    fx, run = pair
+    case = Case(fx, str(tmp_path))
+    fmt, pre, maxit, tol = run_env(run)
+    g = golden_sequences(run)
+    x, it, res, h = oracle.pcg(n=case.n, fmt=fmt, precond=pre, order=0, k=case.k,
+                               tol=tol, max_iter=maxit, crs=case.crs, ell=case.ell,
+                               C_jac=case.C)
+    assert it == run["iterations"]
+    assert_bitwise(h["r2"], g["r2"], "r2")
+    assert_bitwise(h["pq"], g["pq"], "dot_pq")
+    assert_bitwise(x[: g["x_head"].size], g["x_head"], "x[:10]")
+
+
+def _dense(ptr, idx, val, n):
+    a = np.zeros((n, n))
+    for i in range(n):
+        for j in range(ptr[i], ptr[i + 1]):
+            a[i, idx[j]] += val[j]
+    return a
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_reader_and_converters_brute_force(tmp_path, sym):
+    m = problems.random_spd_dense(25, 0.4, seed=1)
+    path = str(tmp_path / "a.mtx")
+    problems.write_mm(path, m, symmetric=sym)
+    mm = oracle.MMFile(path)
+    assert mm.N == 25 and mm.nz == m.nnz
+    I, J, V = mm.coo
+    dense = np.zeros((25, 25))
+    dense[I, J] = V
+    ref = _dense(m.ptr, m.index, m.value, 25)
+    assert np.array_equal(dense, ref)
+    ptr, idx, val = mm.crs()
+    assert np.array_equal(_dense(ptr, idx, val, 25), ref)
+    # stable scatter: within a row, entries keep file order
+    for r in range(25):
+        order = [J[e] for e in range(mm.nz) if I[e] == r]
+        assert list(idx[ptr[r]:ptr[r + 1]]) == order
+    w, length, eidx, edat = mm.ell()
+    assert w == int(np.diff(ptr).max())
+    for r in range(25):
+        for j in range(length[r]):
+            assert eidx[j * 25 + r] == idx[ptr[r] + j]
+            assert edat[j * 25 + r] == val[ptr[r] + j]
+    c, missing = mm.jacobi()
+    assert missing == 0
+    assert np.array_equal(c, 1 / np.diag(ref))
+
+
+def test_reader_error_modes(tmp_path):
+    cases = {
+        "missing.mtx": (None, 1),
+        "pattern.mtx": ("%%MatrixMarket matrix coordinate pattern general\n2 2 1\n1 1\n", 2),
+        "array.mtx": ("%%MatrixMarket matrix array real general\n2 2\n1\n0\n0\n1\n", 2),
+        "hermitian.mtx": ("%%MatrixMarket matrix coordinate real hermitian\n2 2 1\n1 1 1\n", 3),
+        "rect.mtx": ("%%MatrixMarket matrix coordinate real general\n2 3 1\n1 1 1\n", 4),
+        "short.mtx": ("%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1\n", 5),
+    }
+    for name, (text, code) in cases.items():
+        path = tmp_path / name
+        if text is not None:
+            path.write_text(text)
+        with pytest.raises(ValueError, match=f"code {code}"):
+            oracle.MMFile(str(path))
+
+
+def test_reader_comments_and_case(tmp_path):
+    path = tmp_path / "c.mtx"
+    path.write_text("%%MatrixMarket MATRIX Coordinate REAL General\n% comment\n\n%x\n"
+                    "2 2 3\n1 1 2.5\n2 1 -1\n2 2 4\n")
+    mm = oracle.MMFile(str(path))
+    ptr, idx, val = mm.crs()
+    assert list(ptr) == [0, 1, 3] and list(idx) == [0, 0, 1] and list(val) == [2.5, -1.0, 4.0]
+
+
+def test_spmv_closed_forms():
+    m = problems.laplace3d(12)
+    ones = np.ones(m.n)
+    y = np.zeros(m.n)
+    oracle.lib().orc_spmv_crs(m.n, m.ptr, m.index, m.value, ones, y)
+    # interior rows sum to 0; a boundary row loses one -1 per missing face
+    side = 12
+    i = np.arange(m.n)
+    z, yy, x = i // side**2, (i // side) % side, i % side
+    faces = sum(((c == 0) | (c == side - 1)).astype(int) for c in (z, yy, x))
+    assert np.array_equal(y, faces.astype(float))
+    w, length, idx, dat = __import__("tests.cases", fromlist=["to_ell"]).to_ell(m.ptr, m.index, m.value)
+    y2 = np.zeros(m.n)
+    oracle.lib().orc_spmv_ell(m.n, length, idx, dat, ones, y2)
+    assert np.array_equal(y, y2)
+    # symmetry <Ax, y> = <x, Ay> with integer data (exact)
+    rng = np.random.default_rng(0)
+    a = rng.integers(-5, 5, m.n).astype(float)
+    b = rng.integers(-5, 5, m.n).astype(float)
+    Aa, Ab = np.zeros(m.n), np.zeros(m.n)
+    oracle.lib().orc_spmv_crs(m.n, m.ptr, m.index, m.value, a, Aa)
+    oracle.lib().orc_spmv_crs(m.n, m.ptr, m.index, m.value, b, Ab)
+    assert float(Aa @ b) == float(a @ Ab)
+
+
+def test_vector_kernels_closed_forms():
+    L = oracle.lib()
+    n = 1000
+    x = np.arange(n, dtype=float)
+    y = np.full(n, 2.0)
+    L.orc_axpy(n, 0.5, x, y)
+    assert np.array_equal(y, 2.0 + 0.5 * x)
+    y = np.full(n, 2.0)
+    L.orc_xpay(n, x, -0.25, y)  # y = x + a*y (the code, not CG.h:229's comment)
+    assert np.array_equal(y, x - 0.5)
+    c = np.full(n, 0.25)
+    z = np.zeros(n)
+    L.orc_jacobi_apply(n, c, x, z)
+    assert np.array_equal(z, x / 4)
+
+
+@pytest.mark.parametrize("n", [1, 7, 255, 256, 257, 65536, 524288, 524289, 3_000_001])
+def test_tree_dot_exact_on_integers(n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(-64, 64, n).astype(float)
+    b = rng.integers(-64, 64, n).astype(float)
+    exact = float(np.dot(a.astype(np.int64), b.astype(np.int64)))
+    L = oracle.lib()
+    assert L.orc_dot_tree(n, a, b) == exact
+    assert L.orc_dot_serial(n, a, b) == exact
+
+
+@pytest.mark.parametrize("n", [1000, 65536, 2_000_003])
+def test_tree_dot_error_bound(n):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(n)
+    b = rng.standard_normal(n)
+    ref = math.fsum(a * b)  # products are exact in the fsum sense up to 1 ulp each
+    bound = (n * 2**-53 / (1 - n * 2**-53)) * float(np.abs(a * b).sum()) + n * 2**-53 * float(np.abs(a * b).sum())
+    L = oracle.lib()
+    assert abs(L.orc_dot_tree(n, a, b) - ref) <= bound
+    assert abs(L.orc_dot_serial(n, a, b) - ref) <= bound
+
+
+def test_tree_order_is_not_serial_order():
+    # Exact-integer tests cannot tell fold patterns apart; this input can.
+    n = 512
+    a = np.zeros(n)
+    a[0], a[1], a[2] = 1e16, -1e16, 1.0
+    b = np.ones(n)
+    L = oracle.lib()
+    # serial: (1e16 - 1e16) + 1 = 1
+    assert L.orc_dot_serial(n, a, b) == 1.0
+    # shift-down fold: lane 0 meets lane 2 at offset 2 (1e16 + 1 rounds to
+    # 1e16), then lane 1 at offset 1 -> 0.  Pairing neighbours first would
+    # give 1, pairing through shared memory in another order would differ.
+    assert L.orc_dot_tree(n, a, b) == 0.0
+    assert L.orc_vec_grid(n) == 2
+
+
+def test_rowblocks_rule():
+    L = oracle.lib()
+    ptr = np.array([0, 3, 3, 2054, 2056, 2060], np.int32)  # row 2 has 2051 nnz
+    starts = np.zeros(6, np.int32)
+    nb = L.orc_crs_rowblocks(5, ptr, starts)
+    assert list(starts[: nb + 1]) == [0, 2, 3, 5]
+    m = problems.laplace2d(64)
+    starts = np.zeros(m.n + 1, np.int32)
+    nb = L.orc_crs_rowblocks(m.n, m.ptr, starts)
+    assert nb == m.n // 256 and list(starts[:3]) == [0, 256, 512]
+
+
+@pytest.mark.parametrize("n,density,seed", [(12, 0.5, 0), (16, 0.3, 2)])
+def test_cg_invariants_tiny(n, density, seed):
+    """Finite termination (<= n steps up to rounding), x -> A^-1 k."""
+    m = problems.random_spd_dense(n, density, seed)
+    a = _dense(m.ptr, m.index, m.value, n)
+    k = m.ones_rhs()
+    c = 1 / np.diag(a)
+    for order in (0, 1, 2):
+        x, it, res, h = oracle.pcg(n=n, fmt="CRS", precond="jacobi", order=order, k=k,
+                                   tol=1e-12, max_iter=n + 2, crs=(m.ptr, m.index, m.value), C_jac=c)
+        assert it <= n + 1
+        assert np.allclose(x, np.linalg.solve(a, k), rtol=0, atol=1e-10)
+
+
+def test_fused_orders_agree_closely_with_serial(tmp_path):
+    """Orders 1 and 2 reorder only the dot sums: histories stay close on C1."""
+    fx = [f for f in __import__("tests.conftest", fromlist=["load_golden"]).load_golden()
+          if f["case"] == "lap2d_256"][0]
+    case = Case(fx, str(tmp_path))
+    out = {}
+    for order in (0, 1, 2):
+        x, it, res, h = oracle.pcg(n=case.n, fmt="CRS", precond="jacobi", order=order, k=case.k,
+                                   crs=case.crs, C_jac=case.C)
+        out[order] = (it, h)
+    for order in (1, 2):
+        assert abs(out[order][0] - out[0][0]) <= 1
+        assert max_rel(out[order][1]["r2"], out[0][1]["r2"]) < 1e-6
