@@ -26,6 +26,7 @@ SYMBOLS = [
     "cghip_set_format", "cghip_upload_vector", "cghip_download_vector", "cghip_set_dot_mode",
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
+    "cghip_fill_vector", "cghip_true_residual",
 ]
 
 
@@ -84,6 +85,8 @@ def load() -> C.CDLL:
     L.cghip_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]
     L.cghip_history.argtypes = [vp, C.c_int, _f64p, C.c_int, C.POINTER(C.c_int)]
     L.cghip_time_matvec.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.cghip_fill_vector.argtypes = [vp, C.c_int, C.c_double]
+    L.cghip_true_residual.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -204,6 +207,16 @@ class Context:
         w = C.c_int()
         self._chk(self.L.cghip_history(self._h, idx, out, count, C.byref(w)), "cghip_history")
         return out[: w.value]
+
+    def fill(self, vec: str, value: float):
+        self._chk(self.L.cghip_fill_vector(self._h, VEC[vec], value), "cghip_fill_vector")
+
+    def true_residual(self, x="X", b="K") -> tuple[float, float]:
+        """(||b - A x||, ||b||) in FP64 on the device; overwrites Q."""
+        nr, nb = C.c_double(), C.c_double()
+        self._chk(self.L.cghip_true_residual(self._h, VEC[x], VEC[b], C.byref(nr), C.byref(nb)),
+                  "cghip_true_residual")
+        return nr.value, nb.value
 
     def time_matvec(self, x="P", y="Q", reps=20) -> float:
         ms = C.c_double()

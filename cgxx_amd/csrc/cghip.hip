@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -47,8 +48,7 @@ struct cghip_ctx {
   int dot_mode = CGHIP_DOT_TREE;
 
   // reduction scratch
-  double *partial = nullptr;  // 2 x kVecGMax (>= kSpmvGMax)
-  unsigned *ticket = nullptr;
+  double *partial = nullptr;  // 3 x kVecGMax: <p,q> | <r,r> | <r,z> (kSpmvGMax <= kVecGMax)
   double *dot_out = nullptr;
 
   // solver
@@ -133,7 +133,15 @@ void free_ell(cghip_ctx *c) {
 
 bool valid_vec(int id) { return id >= CGHIP_VEC_K && id <= CGHIP_VEC_Z; }
 
-Reduce reduce_of(cghip_ctx *c) { return Reduce{c->partial, c->partial + kVecGMax, c->ticket}; }
+// Partials of the SpMV's <p,q>, and of the vector kernels' two sums.
+Partials pq_of(cghip_ctx *c) { return Partials{c->partial, nullptr}; }
+Partials pr_of(cghip_ctx *c) {
+  return Partials{c->partial + kVecGMax, c->partial + 2 * (size_t)kVecGMax};
+}
+
+int spmv_blocks(cghip_ctx *c) {
+  return spmv_grid(c->format == CGHIP_FORMAT_CRS ? c->crs_nrb : (c->n + kBlock - 1) / kBlock);
+}
 
 Crs<double> crs_of(cghip_ctx *c) {
   return Crs<double>{c->crs_ptr, c->crs_index, c->crs_value, c->crs_rb, c->n, c->crs_nrb};
@@ -198,12 +206,9 @@ int cghip_create(int device, cghip_ctx **out) {
   int rc = CGHIP_OK;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     rc = fail(nullptr, CGHIP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
-  if (!rc) rc = dalloc(c, &c->partial, 2 * (size_t)kVecGMax);
-  if (!rc) rc = dalloc(c, &c->ticket, 1);
-  if (!rc) rc = dalloc(c, &c->dot_out, 1);
+  if (!rc) rc = dalloc(c, &c->partial, 3 * (size_t)kVecGMax);
+  if (!rc) rc = dalloc(c, &c->dot_out, 8);
   if (!rc) rc = dalloc(c, &c->d_state, 1);
-  if (!rc && (e = hipMemset(c->ticket, 0, sizeof(unsigned))) != hipSuccess)
-    rc = fail(nullptr, CGHIP_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
   if (!rc && (e = hipHostMalloc((void **)&c->h_state, sizeof(SolveState), 0)) != hipSuccess)
     rc = fail(nullptr, CGHIP_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
   if (!rc && (e = hipHostMalloc((void **)&c->h_scalar, sizeof(double) * 8, 0)) != hipSuccess)
@@ -230,7 +235,6 @@ int cghip_destroy(cghip_ctx *c) {
   free_ell(c);
   dfree(c->jac);
   dfree(c->partial);
-  dfree(c->ticket);
   dfree(c->dot_out);
   dfree(c->d_state);
   dfree(c->d_hist);
@@ -436,7 +440,7 @@ int cghip_vector_dot(cghip_ctx *c, int a, int b, double *out) {
   if (c->dot_mode == CGHIP_DOT_SERIAL)
     HIPCHK(c, dot_serial(c->n, c->vec[a], c->vec[b], c->dot_out, c->stream));
   else
-    HIPCHK(c, dot_tree(c->n, c->vec[a], c->vec[b], reduce_of(c), c->dot_out, c->stream));
+    HIPCHK(c, dot_tree(c->n, c->vec[a], c->vec[b], pr_of(c), c->dot_out, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->h_scalar, c->dot_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (int rc = sync(c)) return rc;
   *out = c->h_scalar[0];
@@ -458,16 +462,16 @@ namespace {
 int enqueue_iteration(cghip_ctx *c, const double *C, History h, hipEvent_t *ev) {
   double *p = c->vec[CGHIP_VEC_P], *q = c->vec[CGHIP_VEC_Q], *r = c->vec[CGHIP_VEC_R],
          *x = c->vec[CGHIP_VEC_X];
-  Reduce red = reduce_of(c);
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   if (c->format == CGHIP_FORMAT_CRS)
-    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, red, c->d_state, h, c->stream));
+    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, pq_of(c), c->d_state, c->stream));
   else
-    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, red, c->d_state, h, c->stream));
+    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, pq_of(c), c->d_state, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  HIPCHK(c, pcg_update(c->n, x, r, p, q, C, red, c->d_state, h, c->stream));
+  HIPCHK(c, pcg_update(c->n, spmv_blocks(c), x, r, p, q, C, pq_of(c), pr_of(c), c->d_state, h,
+                       c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  HIPCHK(c, pcg_xpay(c->n, p, r, C, c->d_state, c->stream));
+  HIPCHK(c, pcg_xpay(c->n, p, r, C, pr_of(c), c->d_state, h, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   return CGHIP_OK;
 }
@@ -522,44 +526,40 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
 
   const bool profile = o->profile != 0;
   int batch = o->batch > 0 ? o->batch : (c->n >= (1 << 20) ? 8 : 32);
-  if (profile) batch = 1;
   if (profile && ensure_events(c, 6)) return CGHIP_ERR_HIP;
 
+  auto read_state = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(SolveState), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CGHIP_OK;
+  };
+
   const auto t0 = std::chrono::steady_clock::now();
-  // r = A x; r = k - r; p = C r; rho; r2
+  // r = A x; r = k - r; p = C r; r2 = <r,r>; rho = <p,r>
   if (profile) HIPCHK(c, hipEventRecord(c->events[4], c->stream));
   HIPCHK(c, launch_matvec(c, c->vec[CGHIP_VEC_X], c->vec[CGHIP_VEC_R]));
   HIPCHK(c, pcg_init(c->n, c->vec[CGHIP_VEC_K], c->vec[CGHIP_VEC_R], c->vec[CGHIP_VEC_P], C,
-                     reduce_of(c), c->d_state, h, c->stream));
+                     pr_of(c), c->d_state, h, c->stream));
   if (profile) {
     HIPCHK(c, hipEventRecord(c->events[5], c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (int rc = read_state()) return rc;
     info->ms_init = elapsed(c->events[4], c->events[5]);
     info->n_init = 1;
-  }
-
-  if (profile) {
-    for (long nb = 0;; nb++) {
-      if (nb > (long)o->max_iter + 2)
-        return fail(c, CGHIP_ERR_STATE, "solver did not stop after %ld iterations", nb);
+    // One iteration at a time, each kernel between two events.
+    for (;;) {
+      const Scalars &nx = c->h_state->next;
+      if (nx.done || nx.iteration >= o->max_iter) break;
       if (int rc = enqueue_iteration(c, C, h, c->events.data())) return rc;
-      HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(SolveState), hipMemcpyDeviceToHost,
-                               c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      const SolveState &s = *c->h_state;
-      // Kernels that ran past the end return at once; count the real ones.
-      const bool ran = s.iteration > info->n_update;
-      if (ran) {
-        info->ms_matvec += elapsed(c->events[0], c->events[1]);
-        info->ms_update += elapsed(c->events[1], c->events[2]);
-        info->n_matvec++;
-        info->n_update++;
-        if (!s.done) {
-          info->ms_xpay += elapsed(c->events[2], c->events[3]);
-          info->n_xpay++;
-        }
+      if (int rc = read_state()) return rc;
+      info->ms_matvec += elapsed(c->events[0], c->events[1]);
+      info->ms_update += elapsed(c->events[1], c->events[2]);
+      info->n_matvec++;
+      info->n_update++;
+      if (!c->h_state->next.done) {  // the p update ran
+        info->ms_xpay += elapsed(c->events[2], c->events[3]);
+        info->n_xpay++;
       }
-      if (s.done) break;
     }
   } else {
     if (!c->graph || c->graph_batch != batch || c->graph_format != c->format ||
@@ -593,14 +593,12 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
       if (nb >= max_batches)
         return fail(c, CGHIP_ERR_STATE, "solver did not stop after %ld batches", nb);
       HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(SolveState), hipMemcpyDeviceToHost,
-                               c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      if (c->h_state->done) break;
+      if (int rc = read_state()) return rc;
+      if (c->h_state->cur.done || c->h_state->next.done) break;
     }
   }
   const auto t1 = std::chrono::steady_clock::now();
-  const SolveState &s = *c->h_state;
+  const Scalars &s = c->h_state->next;
   c->last_iterations = s.iteration;
   info->iterations = s.iteration;
   info->residual = s.residual;
@@ -620,6 +618,29 @@ int cghip_history(cghip_ctx *c, int which, double *out, int cap, int *written) {
     HIPCHK(c, hipMemcpy(out, c->d_hist + (size_t)which * c->hist_cap, sizeof(double) * (size_t)cnt,
                         hipMemcpyDeviceToHost));
   if (written) *written = cnt;
+  return CGHIP_OK;
+}
+
+int cghip_fill_vector(cghip_ctx *c, int id, double value) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(id)) return fail(c, CGHIP_ERR_ARG, "invalid vector id %d", id);
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  HIPCHK(c, fill(c->n, value, c->vec[id], c->stream));
+  return sync(c);
+}
+
+int cghip_true_residual(cghip_ctx *c, int x, int b, double *norm_r, double *norm_b) {
+  if (int rc = check_ready(c)) return rc;
+  if (!valid_vec(x) || !valid_vec(b) || x == CGHIP_VEC_Q || b == CGHIP_VEC_Q || !norm_r || !norm_b)
+    return fail(c, CGHIP_ERR_ARG, "invalid arguments (x and b must not be Q)");
+  HIPCHK(c, launch_matvec(c, c->vec[x], c->vec[CGHIP_VEC_Q]));
+  HIPCHK(c, residual_norms(c->n, c->vec[b], c->vec[CGHIP_VEC_Q], pr_of(c), c->dot_out,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_scalar, c->dot_out, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  if (int rc = sync(c)) return rc;
+  *norm_r = std::sqrt(c->h_scalar[0]);
+  *norm_b = std::sqrt(c->h_scalar[1]);
   return CGHIP_OK;
 }
 

@@ -14,24 +14,41 @@ namespace cghip {
 // the CPU oracle.
 constexpr int kBlock = 256;      // threads per block (4 wave64s)
 constexpr int kVecGMax = 2048;   // max blocks of grid-stride vector kernels
-constexpr int kRowsMax = 256;    // max rows in one CRS row block
+constexpr int kRowsMax = 256;    // max rows in one CRS row block (one per thread)
 constexpr int kNnzCap = 2048;    // max nonzeros staged in LDS per row block
 constexpr int kSpmvGMax = 2048;  // max blocks of the SpMV kernels
 
+// Vector kernels work on element pairs: G = ceil(pairs / 256), at most 2048.
 inline int vec_grid(int n) {
-  int64_t g = ((int64_t)n + kBlock - 1) / kBlock;
+  const int64_t pairs = ((int64_t)n + 1) / 2;
+  int64_t g = (pairs + kBlock - 1) / kBlock;
   if (g > kVecGMax) g = kVecGMax;
   if (g < 1) g = 1;
   return (int)g;
 }
 
-inline int spmv_grid(int nrb) { return nrb < kSpmvGMax ? (nrb > 0 ? nrb : 1) : kSpmvGMax; }
+// SpMV kernels: block g handles row blocks g, g + G, g + 2G, ...
+inline int spmv_grid(int nrb) {
+  int64_t g = nrb;
+  if (g > kSpmvGMax) g = kSpmvGMax;
+  if (g < 1) g = 1;
+  return (int)g;
+}
 
-// Device-resident scalars of the fused PCG loop (CG.cpp:288-291 plus the
-// loop counter and the stop flag).
+// Scalars of the fused PCG loop (CG.cpp:288-291 plus the loop counter and
+// the stop flag).  No kernel reads a field that the same kernel writes:
+// the p-update kernel (last of an iteration) writes `next`, the SpMV kernel
+// (first of an iteration) copies `next` to `cur`, and the x/r-update and
+// p-update kernels read `cur`.
+struct Scalars {
+  double rho, r2, residual, beta;
+  int iteration, done;
+};
+
 struct SolveState {
-  double rho, rho_old, r2, nrm2_0, dot_pq, alpha, beta, residual, tol;
-  int iteration, max_iter, done, pad;
+  Scalars cur, next;
+  double nrm2_0, tol;  // written once, before the loop
+  int max_iter, pad;
 };
 
 // Per-iteration history (nullable arrays of max_iter + 1 entries).
@@ -39,12 +56,12 @@ struct History {
   double *r2, *rho, *pq, *alpha, *beta;
 };
 
-// Scratch of the single-pass reductions: per-block partials and the ticket
-// counter that elects the last block.  The last block resets the ticket, so
-// no memset is ever needed between launches.
-struct Reduce {
-  double *partial0, *partial1;
-  unsigned *ticket;
+// Per-block partial sums of the reductions.  A kernel writes its blocks'
+// partials with plain stores; the next kernel on the stream (after the
+// kernel-boundary coherence point) folds them in every block, in the fixed
+// order, so no cross-block handoff ever happens inside a kernel.
+struct Partials {
+  double *a, *b;  // kVecGMax entries each
 };
 
 template <typename V>
@@ -73,20 +90,26 @@ hipError_t matvec_ell(const Ell<double> &A, const double *x, double *y, hipStrea
 hipError_t axpy(int n, double a, const double *x, double *y, hipStream_t s);
 hipError_t xpay(int n, const double *x, double a, double *y, hipStream_t s);
 hipError_t jacobi(int n, const double *C, const double *x, double *y, hipStream_t s);
-hipError_t dot_tree(int n, const double *a, const double *b, Reduce red, double *out,
-                    hipStream_t s);
+hipError_t fill(int n, double v, double *y, hipStream_t s);
 hipError_t dot_serial(int n, const double *a, const double *b, double *out, hipStream_t s);
-
-// Fused PCG loop kernels.
-hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *C,
-                    Reduce red, SolveState *st, History h, hipStream_t s);
-hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Reduce red,
-                              SolveState *st, History h, hipStream_t s);
-hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Reduce red,
-                              SolveState *st, History h, hipStream_t s);
-hipError_t pcg_update(int n, double *x, double *r, const double *p, const double *q,
-                      const double *C, Reduce red, SolveState *st, History h, hipStream_t s);
-hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, const SolveState *st,
+// Tree-ordered dot: partials kernel + one-block fold into out[0].
+hipError_t dot_tree(int n, const double *a, const double *b, Partials pa, double *out,
                     hipStream_t s);
+// out[0] = sum (b - q)^2, out[1] = sum b^2 (vector-kernel tree order)
+hipError_t residual_norms(int n, const double *b, const double *q, Partials pa, double *out,
+                          hipStream_t s);
+
+// Fused PCG loop kernels (one iteration = matvec_dot, update, xpay).
+hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *C, Partials pa,
+                    SolveState *st, History h, hipStream_t s);
+hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Partials pq,
+                              SolveState *st, hipStream_t s);
+hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
+                              SolveState *st, hipStream_t s);
+hipError_t pcg_update(int n, int gpq, double *x, double *r, const double *p, const double *q,
+                      const double *C, Partials pq, Partials pr, SolveState *st, History h,
+                      hipStream_t s);
+hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, Partials pr,
+                    SolveState *st, History h, hipStream_t s);
 
 }  // namespace cghip

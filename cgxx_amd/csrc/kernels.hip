@@ -1,21 +1,26 @@
 // HIP kernels of the CG hot path for gfx950 (MI355X).
 //
 // Every kernel is HBM-bound (SpMV: 2 flop per 12 B in FP64), so the design
-// goals are coalesced streams, no re-reads, and reductions that do not cost
-// a second launch:
-//   * CRS SpMV: CSR-stream.  A block stages the products value[e]*x[index[e]]
-//     of a whole row block in LDS with fully coalesced loads, then one thread
-//     per row adds its products left to right -- the per-row order of
-//     SerialCG::matvecKernelCRS (serial/SerialCG.cpp:87-95).
+// goals are many independent bytes in flight per CU, coalesced streams, no
+// re-reads, and no cross-block synchronisation inside a kernel:
+//   * CRS SpMV: block-level CSR-stream.  A block of 256 threads owns a row
+//     block (<= 256 rows, <= 2048 nonzeros); it issues all index/value loads
+//     of the row block at once (8 rounds of 256 coalesced threads), then all
+//     x gathers, stages the products in LDS, and every thread adds its row's
+//     products left to right -- the per-row order of
+//     SerialCG::matvecKernelCRS (serial/SerialCG.cpp:87-95).  Blocks walk
+//     row blocks g, g + G, g + 2G, ... (G <= 2048), so the partial sums of a
+//     fused dot stay few.
 //   * ELL SpMV: one thread per row over column-major slots j*N + i
-//     (Matrix.cpp:339-361), coalesced across the wave, 64-bit slot offsets.
+//     (Matrix.cpp:339-361), 8 slots in flight per thread, 64-bit slot offsets.
+//   * Vector kernels: 16-byte loads of element pairs, 4 pairs in flight per
+//     thread, grid-stride over pairs.
 //   * Reductions: wave64 shift-down fold, four wave sums added in LDS, one
-//     partial per block, and the last block to finish (elected by a ticket
-//     counter) folds the partials and finishes the scalar work (alpha, beta,
-//     stop test).  The order is fixed by the geometry in cghip_kernels.h.
-//   * SpMV blocks walk a contiguous range of row blocks, and hardware block
-//     w (which runs on XCD w % 8) is mapped so that each XCD owns a
-//     contiguous slice of rows: x-vector neighbours stay in that XCD's L2.
+//     partial per block written with a plain store.  The next kernel on the
+//     stream folds the partials (every block, same fixed order) before it
+//     uses the sum.  A device-scope fence inside the hot kernels would write
+//     back and invalidate the XCD's L2 once per block; measured on MI355X it
+//     cost ~90 us per 940 MB update pass, so none is used.
 //
 // Floating-point contraction is off (build flag and pragma): a*x+y is a
 // rounded multiply then a rounded add, as in the reference's CPU kernels.
@@ -27,6 +32,10 @@
 namespace cghip {
 
 namespace {
+
+constexpr int kRounds = kNnzCap / kBlock;  // load rounds per row block
+constexpr int kEllChunk = 8;               // ELL slots in flight per thread
+constexpr int kUnroll = 4;                 // element pairs in flight per thread
 
 // ---------------------------------------------------------------------------
 // Reductions.
@@ -50,245 +59,309 @@ __device__ inline double block_fold(double v, double *lds) {
   return r;
 }
 
-// Stores this block's partial(s) and returns true in every thread of the
-// block that finished last.  Partials are written with agent-scope atomics
-// and a device fence before the ticket increment, so the last block (which
-// fences again) reads every partial whichever XCD's L2 wrote it.
-template <int K>
-__device__ inline bool publish(const double *bp, Reduce red, int G, int g) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&red.partial0[g], bp[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (K > 1)
-      __hip_atomic_store(&red.partial1[g], bp[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(red.ticket, 1u);
-    s_last = (t == (unsigned)(G - 1));
-  }
-  __syncthreads();
-  bool last = s_last != 0;
-  if (last) __threadfence();
-  return last;
-}
-
-// Thread t adds partials t, t+256, ... left to right; then the block fold.
-__device__ inline double final_fold(const double *partial, int G, double *lds) {
+// Fold of G partials written by an earlier kernel: thread t adds partials
+// t, t + 256, ... left to right, then the block fold.  Every block that calls
+// it gets the same value.
+__device__ inline double fold_partials(const double *partial, int G, double *lds) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < G; i += kBlock)
-    s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = threadIdx.x; i < G; i += kBlock) s += partial[i];
   return block_fold(s, lds);
-}
-
-__device__ inline void reset_ticket(Reduce red) {
-  if (threadIdx.x == 0) __hip_atomic_store(red.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Hardware block w runs on XCD w % 8.  Give XCD x the contiguous logical
-// blocks [base_x, base_x + count_x), count_x = ceil((G - x) / 8).
-__device__ inline int xcd_block(int w, int G) {
-  int xcd = w & 7, idx = w >> 3, base = 0;
-  for (int x = 0; x < xcd; x++) base += (G - x + 7) >> 3;
-  return base + idx;
 }
 
 // ---------------------------------------------------------------------------
 // SpMV.
 
-// CSR-stream over the row blocks [lo, hi) of logical block g.  Returns the
-// thread's partial of <x, y> (rows it owns, in order) when WITH_DOT.
-template <bool WITH_DOT>
-__device__ inline double crs_rows(const Crs<double> &A, const double *__restrict__ x,
-                                  double *__restrict__ y, int lo, int hi, double *prod,
-                                  int *sptr) {
-  const int tid = threadIdx.x;
-  double tpart = 0.0;
-  for (int rb = lo; rb < hi; rb++) {
-    const int r0 = A.rb_start[rb], r1 = A.rb_start[rb + 1];
-    const int e0 = A.ptr[r0], e1 = A.ptr[r1];
-    const int nrows = r1 - r0;
-    for (int t = tid; t <= nrows; t += kBlock) sptr[t] = A.ptr[r0 + t] - e0;
-    if (e1 - e0 <= kNnzCap) {
-      // Stage the products with coalesced loads, four in flight per thread.
-      int e = e0 + tid;
-      for (; e + 3 * kBlock < e1; e += 4 * kBlock) {
-        int c0 = A.index[e], c1 = A.index[e + kBlock], c2 = A.index[e + 2 * kBlock],
-            c3 = A.index[e + 3 * kBlock];
-        double v0 = A.value[e], v1 = A.value[e + kBlock], v2 = A.value[e + 2 * kBlock],
-               v3 = A.value[e + 3 * kBlock];
-        prod[e - e0] = v0 * x[c0];
-        prod[e - e0 + kBlock] = v1 * x[c1];
-        prod[e - e0 + 2 * kBlock] = v2 * x[c2];
-        prod[e - e0 + 3 * kBlock] = v3 * x[c3];
-      }
-      for (; e < e1; e += kBlock) prod[e - e0] = A.value[e] * x[A.index[e]];
-      __syncthreads();
-      if (tid < nrows) {
-        double s = 0.0;
-        const int b = sptr[tid], end = sptr[tid + 1];
-        for (int j = b; j < end; j++) s += prod[j];
-        y[r0 + tid] = s;
-        if (WITH_DOT) tpart += x[r0 + tid] * s;
-      }
-      __syncthreads();
-    } else {
-      // One row longer than kNnzCap: chunks through LDS, thread 0 adds in order.
-      double s = 0.0;
-      for (int c = e0; c < e1; c += kNnzCap) {
-        const int ce = min(c + kNnzCap, e1);
-        for (int e = c + tid; e < ce; e += kBlock) prod[e - c] = A.value[e] * x[A.index[e]];
-        __syncthreads();
-        if (tid == 0)
-          for (int j = 0; j < ce - c; j++) s += prod[j];
-        __syncthreads();
-      }
-      if (tid == 0) {
-        y[r0] = s;
-        if (WITH_DOT) tpart += x[r0] * s;
+// CSR-stream of row block rb.  Returns the thread's product x[r] * y[r] for
+// the row it owns (0 if none) when WITH_DOT.
+template <typename V, typename X, bool WITH_DOT>
+__device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, X *__restrict__ y,
+                                      int rb, X *__restrict__ prod) {
+  const int t = threadIdx.x;
+  const int r0 = A.rb_start[rb], r1 = A.rb_start[rb + 1];
+  const int e0 = A.ptr[r0], e1 = A.ptr[r1];
+  const int r = r0 + t;
+  const bool own = r < r1;
+  double dp = 0.0;
+  if (e1 - e0 <= kNnzCap) {
+    int c[kRounds];
+    V v[kRounds];
+#pragma unroll
+    for (int u = 0; u < kRounds; u++) {
+      const int e = e0 + u * kBlock + t;
+      if (e < e1) {
+        c[u] = A.index[e];
+        v[u] = A.value[e];
       }
     }
+    int rb0 = 0, re0 = 0;
+    if (own) {
+      rb0 = A.ptr[r];
+      re0 = A.ptr[r + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kRounds; u++) {
+      const int e = e0 + u * kBlock + t;
+      if (e < e1) prod[e - e0] = X(v[u]) * x[c[u]];
+    }
+    __syncthreads();
+    if (own) {
+      X s = X(0);
+      for (int j = rb0; j < re0; j++) s += prod[j - e0];
+      y[r] = s;
+      if (WITH_DOT) dp = (double)x[r] * (double)s;
+    }
+    __syncthreads();
+  } else {
+    // One row longer than kNnzCap: chunks through LDS, thread 0 adds in order.
+    X s = X(0);
+    for (int c0 = e0; c0 < e1; c0 += kNnzCap) {
+      const int ce = min(c0 + kNnzCap, e1);
+      for (int e = c0 + t; e < ce; e += kBlock) prod[e - c0] = X(A.value[e]) * x[A.index[e]];
+      __syncthreads();
+      if (t == 0)
+        for (int j = 0; j < ce - c0; j++) s += prod[j];
+      __syncthreads();
+    }
+    if (t == 0) {
+      y[r0] = s;
+      if (WITH_DOT) dp = (double)x[r0] * (double)s;
+    }
   }
-  return tpart;
+  return dp;
 }
 
-// ELL rows [rb*256, rb*256+256) for rb in [lo, hi): thread t owns row
-// rb*256 + t and adds its live slots left to right.
-template <bool WITH_DOT>
-__device__ inline double ell_rows(const Ell<double> &A, const double *__restrict__ x,
-                                  double *__restrict__ y, int lo, int hi) {
-  double tpart = 0.0;
+// ELL rows rb*256 + t: the thread adds its live slots left to right.
+template <typename V, typename X, bool WITH_DOT>
+__device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, X *__restrict__ y,
+                                      int rb) {
+  const int r = rb * kBlock + (int)threadIdx.x;
+  if (r >= A.n) return 0.0;
   const int64_t n = A.n;
-  for (int rb = lo; rb < hi; rb++) {
-    const int r = rb * kBlock + (int)threadIdx.x;
-    if (r >= A.n) continue;
-    const int len = A.length[r];
-    double s = 0.0;
-    int j = 0;
-    for (; j + 3 < len; j += 4) {
-      const int64_t k0 = (int64_t)j * n + r;
-      int c0 = A.index[k0], c1 = A.index[k0 + n], c2 = A.index[k0 + 2 * n], c3 = A.index[k0 + 3 * n];
-      double v0 = A.data[k0], v1 = A.data[k0 + n], v2 = A.data[k0 + 2 * n], v3 = A.data[k0 + 3 * n];
-      double p0 = v0 * x[c0], p1 = v1 * x[c1], p2 = v2 * x[c2], p3 = v3 * x[c3];
-      s += p0;
-      s += p1;
-      s += p2;
-      s += p3;
-    }
-    for (; j < len; j++) {
-      const int64_t k = (int64_t)j * n + r;
-      s += A.data[k] * x[A.index[k]];
-    }
-    y[r] = s;
-    if (WITH_DOT) tpart += x[r] * s;
-  }
-  return tpart;
-}
-
-__global__ __launch_bounds__(kBlock) void k_matvec_crs(Crs<double> A, const double *__restrict__ x,
-                                                       double *__restrict__ y) {
-  __shared__ double prod[kNnzCap];
-  __shared__ int sptr[kRowsMax + 1];
-  const int G = gridDim.x, g = xcd_block(blockIdx.x, G);
-  const int lo = (int)((int64_t)g * A.nrb / G), hi = (int)((int64_t)(g + 1) * A.nrb / G);
-  crs_rows<false>(A, x, y, lo, hi, prod, sptr);
-}
-
-__global__ __launch_bounds__(kBlock) void k_matvec_ell(Ell<double> A, const double *__restrict__ x,
-                                                       double *__restrict__ y) {
-  const int G = gridDim.x, g = xcd_block(blockIdx.x, G);
-  const int nrb = (A.n + kBlock - 1) / kBlock;
-  const int lo = (int)((int64_t)g * nrb / G), hi = (int)((int64_t)(g + 1) * nrb / G);
-  ell_rows<false>(A, x, y, lo, hi);
-}
-
-// Last block of the SpMV+<p,q> kernel: alpha = rho / <p,q> (CG.cpp:328-334).
-__device__ inline void finish_matvec_dot(double tpart, Reduce red, SolveState *st, History h,
-                                         int G, int g, bool skip, double *lds) {
-  double bp = block_fold(tpart, lds);
-  if (publish<1>(&bp, red, G, g)) {
-    double pq = final_fold(red.partial0, G, lds);
-    if (threadIdx.x == 0) {
-      if (skip) {
-        st->done = 1;  // CG_MAX_ITER reached (CG.cpp:323)
-      } else {
-        const int it = st->iteration;
-        const double a = st->rho / pq;
-        st->dot_pq = pq;
-        st->alpha = a;
-        if (h.pq) h.pq[it] = pq;
-        if (h.alpha) h.alpha[it] = a;
+  const int len = A.length[r];
+  X s = X(0);
+  for (int j0 = 0; j0 < len; j0 += kEllChunk) {
+    int c[kEllChunk];
+    V v[kEllChunk];
+#pragma unroll
+    for (int u = 0; u < kEllChunk; u++) {
+      if (j0 + u < len) {
+        const int64_t k = (int64_t)(j0 + u) * n + r;
+        c[u] = A.index[k];
+        v[u] = A.data[k];
       }
     }
-    reset_ticket(red);
+#pragma unroll
+    for (int u = 0; u < kEllChunk; u++)
+      if (j0 + u < len) s += X(v[u]) * x[c[u]];
   }
+  y[r] = s;
+  return WITH_DOT ? (double)x[r] * (double)s : 0.0;
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kBlock) void k_matvec_crs(Crs<V> A, const X *__restrict__ x,
+                                                       X *__restrict__ y) {
+  __shared__ X prod[kNnzCap];
+  for (int rb = blockIdx.x; rb < A.nrb; rb += gridDim.x) crs_rowblock<V, X, false>(A, x, y, rb, prod);
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(kBlock) void k_matvec_ell(Ell<V> A, const X *__restrict__ x,
+                                                       X *__restrict__ y) {
+  const int nrb = (A.n + kBlock - 1) / kBlock;
+  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) ell_rowblock<V, X, false>(A, x, y, rb);
+}
+
+// First kernel of an iteration: copies the loop scalars `next` -> `cur`
+// (block 0), stops at CG_MAX_ITER (CG.cpp:323), else q = A p with the
+// partials of <p,q> (CG.cpp:325-328).  Returns false if the iteration is off.
+__device__ inline bool begin_iteration(SolveState *st) {
+  const Scalars s = st->next;
+  // Also when already done: the later kernels of this iteration read `cur`.
+  const bool stop = s.done || s.iteration >= st->max_iter;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Scalars c = s;
+    if (stop) c.done = 1;
+    st->cur = c;
+  }
+  return !stop;
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_crs(Crs<double> A, const double *__restrict__ p,
-                                                               double *__restrict__ q, Reduce red,
-                                                               SolveState *st, History h) {
+                                                               double *__restrict__ q, Partials pq,
+                                                               SolveState *st) {
   __shared__ double prod[kNnzCap];
-  __shared__ int sptr[kRowsMax + 1];
   __shared__ double lds[4];
-  if (st->done) return;
-  const bool skip = st->iteration >= st->max_iter;
-  const int G = gridDim.x, g = xcd_block(blockIdx.x, G);
+  if (!begin_iteration(st)) return;
   double tpart = 0.0;
-  if (!skip) {
-    const int lo = (int)((int64_t)g * A.nrb / G), hi = (int)((int64_t)(g + 1) * A.nrb / G);
-    tpart = crs_rows<true>(A, p, q, lo, hi, prod, sptr);
-  }
-  finish_matvec_dot(tpart, red, st, h, G, g, skip, lds);
+  for (int rb = blockIdx.x; rb < A.nrb; rb += gridDim.x)
+    tpart += crs_rowblock<double, double, true>(A, p, q, rb, prod);
+  const double bp = block_fold(tpart, lds);
+  if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_ell(Ell<double> A, const double *__restrict__ p,
-                                                               double *__restrict__ q, Reduce red,
-                                                               SolveState *st, History h) {
+                                                               double *__restrict__ q, Partials pq,
+                                                               SolveState *st) {
   __shared__ double lds[4];
-  if (st->done) return;
-  const bool skip = st->iteration >= st->max_iter;
-  const int G = gridDim.x, g = xcd_block(blockIdx.x, G);
+  if (!begin_iteration(st)) return;
+  const int nrb = (A.n + kBlock - 1) / kBlock;
   double tpart = 0.0;
-  if (!skip) {
-    const int nrb = (A.n + kBlock - 1) / kBlock;
-    const int lo = (int)((int64_t)g * nrb / G), hi = (int)((int64_t)(g + 1) * nrb / G);
-    tpart = ell_rows<true>(A, p, q, lo, hi);
-  }
-  finish_matvec_dot(tpart, red, st, h, G, g, skip, lds);
+  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x)
+    tpart += ell_rowblock<double, double, true>(A, p, q, rb);
+  const double bp = block_fold(tpart, lds);
+  if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
 
 // ---------------------------------------------------------------------------
-// Vector kernels (grid-stride; thread gid visits gid, gid + T, ... in order).
+// Vector kernels.  Element pairs (2q, 2q+1); thread gid of T = 256*G visits
+// pairs gid, gid + T, ... in order, element 2q before 2q+1.  The last pair
+// of an odd-length vector holds one element.
+
+struct Pairs {
+  int64_t T, full, npair, q;
+  __device__ explicit Pairs(int n)
+      : T((int64_t)gridDim.x * kBlock), full(n >> 1), npair(((int64_t)n + 1) >> 1),
+        q((int64_t)blockIdx.x * kBlock + threadIdx.x) {}
+};
+
+__device__ inline const double2 *v2(const double *p) { return reinterpret_cast<const double2 *>(p); }
+__device__ inline double2 *v2(double *p) { return reinterpret_cast<double2 *>(p); }
 
 __global__ __launch_bounds__(kBlock) void k_axpy(int n, double a, const double *__restrict__ x,
                                                  double *__restrict__ y) {
-  const int64_t T = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = y[i] + a * x[i];
+  Pairs P(n);
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 xv[kUnroll], yv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      xv[u] = v2(x)[P.q + u * P.T];
+      yv[u] = v2(y)[P.q + u * P.T];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      yv[u].x = yv[u].x + a * xv[u].x;
+      yv[u].y = yv[u].y + a * xv[u].y;
+      v2(y)[P.q + u * P.T] = yv[u];
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T) {
+    const int64_t e = 2 * P.q;
+    y[e] = y[e] + a * x[e];
+    if (e + 1 < n) y[e + 1] = y[e + 1] + a * x[e + 1];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_xpay(int n, const double *__restrict__ x, double a,
                                                  double *__restrict__ y) {
-  const int64_t T = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = x[i] + a * y[i];
+  Pairs P(n);
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 xv[kUnroll], yv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      xv[u] = v2(x)[P.q + u * P.T];
+      yv[u] = v2(y)[P.q + u * P.T];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      yv[u].x = xv[u].x + a * yv[u].x;
+      yv[u].y = xv[u].y + a * yv[u].y;
+      v2(y)[P.q + u * P.T] = yv[u];
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T) {
+    const int64_t e = 2 * P.q;
+    y[e] = x[e] + a * y[e];
+    if (e + 1 < n) y[e + 1] = x[e + 1] + a * y[e + 1];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const double *__restrict__ C,
                                                    const double *__restrict__ x, double *__restrict__ y) {
-  const int64_t T = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = C[i] * x[i];
+  Pairs P(n);
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 cv[kUnroll], xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      cv[u] = v2(C)[P.q + u * P.T];
+      xv[u] = v2(x)[P.q + u * P.T];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      double2 yv;
+      yv.x = cv[u].x * xv[u].x;
+      yv.y = cv[u].y * xv[u].y;
+      v2(y)[P.q + u * P.T] = yv;
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T) {
+    const int64_t e = 2 * P.q;
+    y[e] = C[e] * x[e];
+    if (e + 1 < n) y[e + 1] = C[e + 1] * x[e + 1];
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot_tree(int n, const double *__restrict__ a,
-                                                     const double *__restrict__ b, Reduce red,
-                                                     double *out) {
+__global__ __launch_bounds__(kBlock) void k_fill(int n, double v, double *__restrict__ y) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_partials(int n, const double *__restrict__ a,
+                                                         const double *__restrict__ b, Partials pa) {
   __shared__ double lds[4];
-  const int G = gridDim.x, g = blockIdx.x;
-  const int64_t T = (int64_t)G * kBlock;
+  Pairs P(n);
   double s = 0.0;
-  for (int64_t i = (int64_t)g * kBlock + threadIdx.x; i < n; i += T) s += a[i] * b[i];
-  double bp = block_fold(s, lds);
-  if (publish<1>(&bp, red, G, g)) {
-    double r = final_fold(red.partial0, G, lds);
-    if (threadIdx.x == 0) *out = r;
-    reset_ticket(red);
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 av[kUnroll], bv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      av[u] = v2(a)[P.q + u * P.T];
+      bv[u] = v2(b)[P.q + u * P.T];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      s += av[u].x * bv[u].x;
+      s += av[u].y * bv[u].y;
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T) {
+    const int64_t e = 2 * P.q;
+    s += a[e] * b[e];
+    if (e + 1 < n) s += a[e + 1] * b[e + 1];
+  }
+  const double bp = block_fold(s, lds);
+  if (threadIdx.x == 0) pa.a[blockIdx.x] = bp;
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual_partials(int n, const double *__restrict__ b,
+                                                              const double *__restrict__ q,
+                                                              Partials pa) {
+  __shared__ double lds[4];
+  Pairs P(n);
+  double s_rr = 0.0, s_bb = 0.0;
+  for (; P.q < P.npair; P.q += P.T) {
+    for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
+      const double ri = b[e] - q[e];
+      s_rr += ri * ri;
+      s_bb += b[e] * b[e];
+    }
+  }
+  const double b0 = block_fold(s_rr, lds);
+  const double b1 = block_fold(s_bb, lds);
+  if (threadIdx.x == 0) {
+    pa.a[blockIdx.x] = b0;
+    pa.b[blockIdx.x] = b1;
+  }
+}
+
+// One block: out[0] = fold(pa.a), out[1] = fold(pa.b) (if two_sums).
+__global__ __launch_bounds__(kBlock) void k_fold_out(Partials pa, int G, int two_sums, double *out) {
+  __shared__ double lds[4];
+  const double s0 = fold_partials(pa.a, G, lds);
+  const double s1 = two_sums ? fold_partials(pa.b, G, lds) : 0.0;
+  if (threadIdx.x == 0) {
+    out[0] = s0;
+    if (two_sums) out[1] = s1;
   }
 }
 
@@ -312,106 +385,178 @@ __global__ __launch_bounds__(64) void k_dot_serial(int n, const double *__restri
 template <bool JAC>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double *__restrict__ k,
                                                      double *__restrict__ r, double *__restrict__ p,
-                                                     const double *__restrict__ C, Reduce red,
-                                                     SolveState *st, History h) {
+                                                     const double *__restrict__ C, Partials pa) {
   __shared__ double lds[4];
-  const int G = gridDim.x, g = blockIdx.x;
-  const int64_t T = (int64_t)G * kBlock;
+  Pairs P(n);
   double s_rr = 0.0, s_pr = 0.0;
-  for (int64_t i = (int64_t)g * kBlock + threadIdx.x; i < n; i += T) {
-    const double ri = k[i] + (-1.0) * r[i];
-    r[i] = ri;
-    const double pi = JAC ? C[i] * ri : ri;
-    p[i] = pi;
-    s_rr += ri * ri;
-    s_pr += pi * ri;
-  }
-  double bp[2];
-  bp[0] = block_fold(s_rr, lds);
-  bp[1] = block_fold(s_pr, lds);
-  if (publish<2>(bp, red, G, g)) {
-    const double r2 = final_fold(red.partial0, G, lds);
-    const double pr = final_fold(red.partial1, G, lds);
-    if (threadIdx.x == 0) {
-      const double rho = JAC ? pr : r2;
-      st->r2 = r2;
-      st->nrm2_0 = sqrt(r2);
-      st->rho = rho;
-      st->iteration = 0;
-      st->done = 0;
-      if (h.r2) h.r2[0] = r2;
-      if (h.rho) h.rho[0] = rho;
+  for (; P.q < P.npair; P.q += P.T) {
+    for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
+      const double ri = k[e] + (-1.0) * r[e];
+      r[e] = ri;
+      const double pi = JAC ? C[e] * ri : ri;
+      p[e] = pi;
+      s_rr += ri * ri;
+      s_pr += pi * ri;
     }
-    reset_ticket(red);
+  }
+  const double b0 = block_fold(s_rr, lds);
+  const double b1 = block_fold(s_pr, lds);
+  if (threadIdx.x == 0) {
+    pa.a[blockIdx.x] = b0;
+    pa.b[blockIdx.x] = b1;
   }
 }
 
-// x += a p; r += (-a) q (CG.cpp:340-342); <r,r>, stop test (CG.cpp:344-355);
-// z = C o r, rho = <r,z>, b = rho / rho_old (CG.cpp:357-370).
+// One block: r2, rho and nrm2_0 = sqrt(r2) before the loop (CG.cpp:305-318).
 template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, double *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_pcg_init_finish(Partials pa, int G, SolveState *st,
+                                                            History h) {
+  __shared__ double lds[4];
+  const double r2 = fold_partials(pa.a, G, lds);
+  const double pr = fold_partials(pa.b, G, lds);
+  if (threadIdx.x == 0) {
+    Scalars s{};
+    s.r2 = r2;
+    s.rho = JAC ? pr : r2;
+    s.residual = 1.0;
+    s.iteration = 0;
+    s.done = 0;
+    st->next = s;
+    st->cur = s;
+    st->nrm2_0 = sqrt(r2);
+    if (h.r2) h.r2[0] = r2;
+    if (h.rho) h.rho[0] = s.rho;
+  }
+}
+
+// a = rho / <p,q> (CG.cpp:334); x += a p; r += (-a) q (CG.cpp:340-342);
+// partials of <r,r> and <r,z> with z = C o r (CG.cpp:344, 363-366).
+template <bool JAC>
+__device__ inline void update_elem(double &xv, double &rv, double pv, double qv, double cv,
+                                   double a, double na, double &s_rr, double &s_rz) {
+  xv = xv + a * pv;
+  rv = rv + na * qv;
+  const double zv = JAC ? cv * rv : rv;
+  s_rr += rv * rv;
+  s_rz += rv * zv;
+}
+
+template <bool JAC>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *__restrict__ x,
                                                        double *__restrict__ r,
                                                        const double *__restrict__ p,
                                                        const double *__restrict__ q,
-                                                       const double *__restrict__ C, Reduce red,
-                                                       SolveState *st, History h) {
+                                                       const double *__restrict__ C, Partials ppq,
+                                                       Partials prr, const SolveState *st,
+                                                       History h) {
   __shared__ double lds[4];
-  if (st->done) return;
-  const double a = st->alpha;
-  const double na = -a;
-  const int G = gridDim.x, g = blockIdx.x;
-  const int64_t T = (int64_t)G * kBlock;
-  double s_rr = 0.0, s_rz = 0.0;
-  for (int64_t i = (int64_t)g * kBlock + threadIdx.x; i < n; i += T) {
-    x[i] = x[i] + a * p[i];
-    const double ri = r[i] + na * q[i];
-    r[i] = ri;
-    const double zi = JAC ? C[i] * ri : ri;
-    s_rr += ri * ri;
-    s_rz += ri * zi;
+  const Scalars c = st->cur;
+  if (c.done) return;
+  const double pq = fold_partials(ppq.a, gpq, lds);
+  const double a = c.rho / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (h.pq) h.pq[c.iteration] = pq;
+    if (h.alpha) h.alpha[c.iteration] = a;
   }
-  double bp[2];
-  bp[0] = block_fold(s_rr, lds);
-  bp[1] = block_fold(s_rz, lds);
-  if (publish<2>(bp, red, G, g)) {
-    const double r2 = final_fold(red.partial0, G, lds);
-    const double rz = final_fold(red.partial1, G, lds);
-    if (threadIdx.x == 0) {
-      const int it = st->iteration;
-      st->r2 = r2;
-      if (h.r2) h.r2[it + 1] = r2;
-      const double res = sqrt(r2) / st->nrm2_0;
-      st->residual = res;
-      if (res <= st->tol) {
-        st->done = 1;
-      } else {
-        const double rho_old = st->rho;
-        const double rho = JAC ? rz : r2;
-        const double b = rho / rho_old;
-        st->rho_old = rho_old;
-        st->rho = rho;
-        st->beta = b;
-        if (h.rho) h.rho[it + 1] = rho;
-        if (h.beta) h.beta[it] = b;
-      }
-      st->iteration = it + 1;
+  const double na = -a;
+  Pairs P(n);
+  double s_rr = 0.0, s_rz = 0.0;
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 xv[kUnroll], rv[kUnroll], pv[kUnroll], qv[kUnroll], cv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t j = P.q + u * P.T;
+      xv[u] = v2(x)[j];
+      rv[u] = v2(r)[j];
+      pv[u] = v2(p)[j];
+      qv[u] = v2(q)[j];
+      if (JAC) cv[u] = v2(C)[j];
     }
-    reset_ticket(red);
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t j = P.q + u * P.T;
+      update_elem<JAC>(xv[u].x, rv[u].x, pv[u].x, qv[u].x, JAC ? cv[u].x : 0.0, a, na, s_rr, s_rz);
+      update_elem<JAC>(xv[u].y, rv[u].y, pv[u].y, qv[u].y, JAC ? cv[u].y : 0.0, a, na, s_rr, s_rz);
+      v2(x)[j] = xv[u];
+      v2(r)[j] = rv[u];
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T) {
+    for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
+      double xe = x[e], re = r[e];
+      update_elem<JAC>(xe, re, p[e], q[e], JAC ? C[e] : 0.0, a, na, s_rr, s_rz);
+      x[e] = xe;
+      r[e] = re;
+    }
+  }
+  const double b0 = block_fold(s_rr, lds);
+  const double b1 = block_fold(s_rz, lds);
+  if (threadIdx.x == 0) {
+    prr.a[blockIdx.x] = b0;
+    prr.b[blockIdx.x] = b1;
   }
 }
 
-// p = z + b p with z = C o r recomputed (CG.cpp:375-381).
+// Last kernel of an iteration: r2 and the stop test (CG.cpp:344-355); if not
+// converged rho = <r,z>, b = rho / rho_old (CG.cpp:357-370) and
+// p = z + b p with z = C o r recomputed (CG.cpp:375-381).  Writes `next`.
 template <bool JAC>
 __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__ p,
                                                      const double *__restrict__ r,
-                                                     const double *__restrict__ C,
-                                                     const SolveState *st) {
-  if (st->done) return;
-  const double b = st->beta;
-  const int64_t T = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
-    const double zi = JAC ? C[i] * r[i] : r[i];
-    p[i] = zi + b * p[i];
+                                                     const double *__restrict__ C, Partials prr,
+                                                     SolveState *st, History h) {
+  __shared__ double lds[4];
+  const Scalars c = st->cur;
+  if (c.done) return;
+  const int G = gridDim.x;  // the update kernel ran on the same grid
+  const double r2 = fold_partials(prr.a, G, lds);
+  const double rz = fold_partials(prr.b, G, lds);
+  const double res = sqrt(r2) / st->nrm2_0;
+  const bool converged = res <= st->tol;
+  const double rho = JAC ? rz : r2;
+  const double b = rho / c.rho;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int it = c.iteration;
+    Scalars s = c;
+    s.r2 = r2;
+    s.residual = res;
+    s.iteration = it + 1;
+    if (h.r2) h.r2[it + 1] = r2;
+    if (converged) {
+      s.done = 1;
+    } else {
+      s.rho = rho;
+      s.beta = b;
+      if (h.rho) h.rho[it + 1] = rho;
+      if (h.beta) h.beta[it] = b;
+    }
+    st->next = s;
+  }
+  if (converged) return;
+  Pairs P(n);
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 pv[kUnroll], rv[kUnroll], cv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t j = P.q + u * P.T;
+      pv[u] = v2(p)[j];
+      rv[u] = v2(r)[j];
+      if (JAC) cv[u] = v2(C)[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const double zx = JAC ? cv[u].x * rv[u].x : rv[u].x;
+      const double zy = JAC ? cv[u].y * rv[u].y : rv[u].y;
+      pv[u].x = zx + b * pv[u].x;
+      pv[u].y = zy + b * pv[u].y;
+      v2(p)[P.q + u * P.T] = pv[u];
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T) {
+    for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
+      const double ze = JAC ? C[e] * r[e] : r[e];
+      p[e] = ze + b * p[e];
+    }
   }
 }
 
@@ -434,17 +579,27 @@ int crs_rowblocks(int n, const int *ptr, int *starts) {
 
 static inline dim3 vgrid(int n) { return dim3(vec_grid(n)); }
 
-hipError_t matvec_crs(const Crs<double> &A, const double *x, double *y, hipStream_t s) {
+template <typename V, typename X>
+hipError_t matvec_crs_t(const Crs<V> &A, const X *x, X *y, hipStream_t s) {
   if (A.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_matvec_crs, dim3(spmv_grid(A.nrb)), dim3(kBlock), 0, s, A, x, y);
+  hipLaunchKernelGGL((k_matvec_crs<V, X>), dim3(spmv_grid(A.nrb)), dim3(kBlock), 0, s, A, x, y);
   return hipGetLastError();
 }
 
-hipError_t matvec_ell(const Ell<double> &A, const double *x, double *y, hipStream_t s) {
+template <typename V, typename X>
+hipError_t matvec_ell_t(const Ell<V> &A, const X *x, X *y, hipStream_t s) {
   if (A.n == 0) return hipSuccess;
   const int nrb = (A.n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_matvec_ell, dim3(spmv_grid(nrb)), dim3(kBlock), 0, s, A, x, y);
+  hipLaunchKernelGGL((k_matvec_ell<V, X>), dim3(spmv_grid(nrb)), dim3(kBlock), 0, s, A, x, y);
   return hipGetLastError();
+}
+
+hipError_t matvec_crs(const Crs<double> &A, const double *x, double *y, hipStream_t s) {
+  return matvec_crs_t<double, double>(A, x, y, s);
+}
+
+hipError_t matvec_ell(const Ell<double> &A, const double *x, double *y, hipStream_t s) {
+  return matvec_ell_t<double, double>(A, x, y, s);
 }
 
 hipError_t axpy(int n, double a, const double *x, double *y, hipStream_t s) {
@@ -462,9 +617,22 @@ hipError_t jacobi(int n, const double *C, const double *x, double *y, hipStream_
   return hipGetLastError();
 }
 
-hipError_t dot_tree(int n, const double *a, const double *b, Reduce red, double *out,
+hipError_t fill(int n, double v, double *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, vgrid(n), dim3(kBlock), 0, s, n, v, y);
+  return hipGetLastError();
+}
+
+hipError_t dot_tree(int n, const double *a, const double *b, Partials pa, double *out,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_dot_tree, vgrid(n), dim3(kBlock), 0, s, n, a, b, red, out);
+  hipLaunchKernelGGL(k_dot_partials, vgrid(n), dim3(kBlock), 0, s, n, a, b, pa);
+  hipLaunchKernelGGL(k_fold_out, dim3(1), dim3(kBlock), 0, s, pa, vec_grid(n), 0, out);
+  return hipGetLastError();
+}
+
+hipError_t residual_norms(int n, const double *b, const double *q, Partials pa, double *out,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_residual_partials, vgrid(n), dim3(kBlock), 0, s, n, b, q, pa);
+  hipLaunchKernelGGL(k_fold_out, dim3(1), dim3(kBlock), 0, s, pa, vec_grid(n), 1, out);
   return hipGetLastError();
 }
 
@@ -473,47 +641,51 @@ hipError_t dot_serial(int n, const double *a, const double *b, double *out, hipS
   return hipGetLastError();
 }
 
-hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *C, Reduce red,
+hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *C, Partials pa,
                     SolveState *st, History h, hipStream_t s) {
-  if (C)
-    hipLaunchKernelGGL(k_pcg_init<true>, vgrid(n), dim3(kBlock), 0, s, n, k, r, p, C, red, st, h);
-  else
-    hipLaunchKernelGGL(k_pcg_init<false>, vgrid(n), dim3(kBlock), 0, s, n, k, r, p, C, red, st, h);
+  if (C) {
+    hipLaunchKernelGGL(k_pcg_init<true>, vgrid(n), dim3(kBlock), 0, s, n, k, r, p, C, pa);
+    hipLaunchKernelGGL(k_pcg_init_finish<true>, dim3(1), dim3(kBlock), 0, s, pa, vec_grid(n), st, h);
+  } else {
+    hipLaunchKernelGGL(k_pcg_init<false>, vgrid(n), dim3(kBlock), 0, s, n, k, r, p, C, pa);
+    hipLaunchKernelGGL(k_pcg_init_finish<false>, dim3(1), dim3(kBlock), 0, s, pa, vec_grid(n), st, h);
+  }
   return hipGetLastError();
 }
 
-hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Reduce red,
-                              SolveState *st, History h, hipStream_t s) {
+hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Partials pq,
+                              SolveState *st, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_matvec_dot_crs, dim3(spmv_grid(A.nrb)), dim3(kBlock), 0, s, A, p, q,
-                     red, st, h);
+                     pq, st);
   return hipGetLastError();
 }
 
-hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Reduce red,
-                              SolveState *st, History h, hipStream_t s) {
+hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
+                              SolveState *st, hipStream_t s) {
   const int nrb = (A.n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_pcg_matvec_dot_ell, dim3(spmv_grid(nrb)), dim3(kBlock), 0, s, A, p, q,
-                     red, st, h);
+                     pq, st);
   return hipGetLastError();
 }
 
-hipError_t pcg_update(int n, double *x, double *r, const double *p, const double *q,
-                      const double *C, Reduce red, SolveState *st, History h, hipStream_t s) {
+hipError_t pcg_update(int n, int gpq, double *x, double *r, const double *p, const double *q,
+                      const double *C, Partials pq, Partials pr, SolveState *st, History h,
+                      hipStream_t s) {
   if (C)
-    hipLaunchKernelGGL(k_pcg_update<true>, vgrid(n), dim3(kBlock), 0, s, n, x, r, p, q, C, red,
-                       st, h);
+    hipLaunchKernelGGL(k_pcg_update<true>, vgrid(n), dim3(kBlock), 0, s, n, gpq, x, r, p, q, C,
+                       pq, pr, st, h);
   else
-    hipLaunchKernelGGL(k_pcg_update<false>, vgrid(n), dim3(kBlock), 0, s, n, x, r, p, q, C, red,
-                       st, h);
+    hipLaunchKernelGGL(k_pcg_update<false>, vgrid(n), dim3(kBlock), 0, s, n, gpq, x, r, p, q, C,
+                       pq, pr, st, h);
   return hipGetLastError();
 }
 
-hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, const SolveState *st,
-                    hipStream_t s) {
+hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, Partials pr,
+                    SolveState *st, History h, hipStream_t s) {
   if (C)
-    hipLaunchKernelGGL(k_pcg_xpay<true>, vgrid(n), dim3(kBlock), 0, s, n, p, r, C, st);
+    hipLaunchKernelGGL(k_pcg_xpay<true>, vgrid(n), dim3(kBlock), 0, s, n, p, r, C, pr, st, h);
   else
-    hipLaunchKernelGGL(k_pcg_xpay<false>, vgrid(n), dim3(kBlock), 0, s, n, p, r, C, st);
+    hipLaunchKernelGGL(k_pcg_xpay<false>, vgrid(n), dim3(kBlock), 0, s, n, p, r, C, pr, st, h);
   return hipGetLastError();
 }
 

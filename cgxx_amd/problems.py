@@ -103,17 +103,51 @@ def _stencil3d(side: int, diag: np.ndarray, faces: list[np.ndarray]) -> CRS:
     numbers; stored negated), each of shape (n,), ignored on the boundary.
     """
     n = side**3
-    i = np.arange(n, dtype=np.int64)
-    z, rem = i // (side * side), i % (side * side)
-    y, x = rem // side, rem % side
     s2 = side * side
-    cols = np.stack([i - s2, i - side, i - 1, i, i + 1, i + side, i + s2], axis=1)
-    live = np.stack(
-        [z > 0, y > 0, x > 0, np.ones(n, bool), x < side - 1, y < side - 1, z < side - 1],
-        axis=1,
-    )
-    vals = np.stack([-faces[0], -faces[1], -faces[2], diag, -faces[3], -faces[4], -faces[5]], axis=1)
-    return _assemble(n, cols, vals, live)
+    shape = (side, side, side)
+    # Entries per row: 7 minus one per boundary face.
+    cnt = np.full(shape, 7, dtype=np.int64)
+    cnt[0] -= 1
+    cnt[-1] -= 1
+    cnt[:, 0] -= 1
+    cnt[:, -1] -= 1
+    cnt[:, :, 0] -= 1
+    cnt[:, :, -1] -= 1
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(cnt.ravel(), out=ptr[1:])
+    del cnt
+    nnz = int(ptr[-1])
+    if nnz >= 2**31:
+        raise ValueError("nnz exceeds int32 indexing")
+    index = np.empty(nnz, dtype=np.int32)
+    value = np.empty(nnz, dtype=np.float64)
+    pos = ptr[:-1].copy()  # next free slot of each row
+    rows = np.arange(n, dtype=np.int32)
+
+    def live_mask(axis: int, lower: bool) -> np.ndarray:
+        m = np.ones(shape, dtype=bool)
+        sl = [slice(None)] * 3
+        sl[axis] = 0 if lower else side - 1
+        m[tuple(sl)] = False
+        return m.ravel()
+
+    # Ascending column order: -z, -y, -x, diagonal, +x, +y, +z.
+    dirs = [(-s2, live_mask(0, True), faces[0], -1.0), (-side, live_mask(1, True), faces[1], -1.0),
+            (-1, live_mask(2, True), faces[2], -1.0), (0, None, diag, 1.0),
+            (1, live_mask(2, False), faces[3], -1.0), (side, live_mask(1, False), faces[4], -1.0),
+            (s2, live_mask(0, False), faces[5], -1.0)]
+    for delta, live, vals, sign in dirs:
+        if live is None:
+            p = pos
+            index[p] = rows
+            value[p] = vals
+            pos += 1
+        else:
+            p = pos[live]
+            index[p] = rows[live] + np.int32(delta)
+            value[p] = sign * vals[live]
+            pos[live] += 1
+    return CRS(n=n, ptr=ptr.astype(np.int32), index=index, value=value)
 
 
 def laplace3d(side: int) -> CRS:

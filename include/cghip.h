@@ -135,6 +135,16 @@ int cghip_solve(cghip_ctx *ctx, const cghip_solve_opts *opts,
 int cghip_history(cghip_ctx *ctx, int which, double *out, int cap,
                   int *written);
 
+/* Sets every entry of vector `id` to `value` on the device (resetX,
+ * CG.cpp:272-275, without a host transfer). */
+int cghip_fill_vector(cghip_ctx *ctx, int id, double value);
+
+/* FP64 true residual of vector x against right-hand side b:
+ * *norm_r = ||b - A x||_2 and *norm_b = ||b||_2, with tree-ordered sums.
+ * Overwrites vector Q with A x. */
+int cghip_true_residual(cghip_ctx *ctx, int x, int b, double *norm_r,
+                        double *norm_b);
+
 /* Average duration (ms) of `reps` back-to-back launches of the matvec kernel
  * x -> y, timed with HIP events on the context's stream. */
 int cghip_time_matvec(cghip_ctx *ctx, int x, int y, int reps, double *avg_ms);

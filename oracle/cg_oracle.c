@@ -330,28 +330,32 @@ static double final_fold(const double *partial, int G) {
   return block_fold(v);
 }
 
+/* Vector kernels work on element pairs (2q, 2q+1): G = ceil(pairs / 256),
+ * at most 2048, T = 256 * G threads; thread g*256+t visits pairs
+ * q = g*256+t, +T, +2T, ... and adds element 2q before 2q+1. */
 int orc_vec_grid(int N) {
-  int64_t g = ((int64_t)N + ORC_BLOCK - 1) / ORC_BLOCK;
+  int64_t pairs = ((int64_t)N + 1) / 2;
+  int64_t g = (pairs + ORC_BLOCK - 1) / ORC_BLOCK;
   if (g > ORC_VEC_GMAX) g = ORC_VEC_GMAX;
   if (g < 1) g = 1;
   return (int)g;
 }
 
-/* Grid-stride kernels: thread g*256+t visits i = g*256+t, +T, +2T, ... with
- * T = 256 * G, adding its products left to right. */
 static double vec_tree(int N, const double *a, const double *b,
                        const double *c /* nullable: a*(c*b) */) {
   int G = orc_vec_grid(N);
   int64_t T = (int64_t)G * ORC_BLOCK;
+  int64_t pairs = ((int64_t)N + 1) / 2;
   double *partial = (double *)malloc(sizeof(double) * (size_t)G);
   double v[ORC_BLOCK];
   for (int g = 0; g < G; g++) {
     for (int t = 0; t < ORC_BLOCK; t++) {
       double s = 0.0;
-      for (int64_t i = (int64_t)g * ORC_BLOCK + t; i < N; i += T) {
-        double bi = c ? c[i] * b[i] : b[i];
-        s += a[i] * bi;
-      }
+      for (int64_t q = (int64_t)g * ORC_BLOCK + t; q < pairs; q += T)
+        for (int64_t i = 2 * q; i < 2 * q + 2 && i < N; i++) {
+          double bi = c ? c[i] * b[i] : b[i];
+          s += a[i] * bi;
+        }
       v[t] = s;
     }
     partial[g] = block_fold(v);
@@ -380,8 +384,10 @@ int orc_crs_rowblocks(int N, const int *ptr, int *starts) {
   return nb;
 }
 
-/* SpMV blocks: G = min(nrb, 2048) blocks; block g owns row blocks
- * [g*nrb/G, (g+1)*nrb/G); thread t owns row start+t of each of them. */
+/* SpMV geometry: G = min(nrb, 2048) blocks; block g owns row blocks
+ * g, g + G, g + 2G, ... and its thread t owns row start + t of each.  The
+ * thread adds p[r]*q[r] over its rows in order (each product added to the
+ * running sum from 0.0); then the block and final folds. */
 double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
                              const double *q) {
   if (nrb <= 0) return 0.0;
@@ -389,12 +395,11 @@ double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
   double *partial = (double *)malloc(sizeof(double) * (size_t)G);
   double v[ORC_BLOCK];
   for (int g = 0; g < G; g++) {
-    int lo = (int)((int64_t)g * nrb / G), hi = (int)((int64_t)(g + 1) * nrb / G);
     for (int t = 0; t < ORC_BLOCK; t++) v[t] = 0.0;
-    for (int rb = lo; rb < hi; rb++)
+    for (int rb = g; rb < nrb; rb += G)
       for (int t = 0; t < ORC_BLOCK; t++) {
         int r = starts[rb] + t;
-        if (r < starts[rb + 1]) v[t] += p[r] * q[r];
+        v[t] += r < starts[rb + 1] ? p[r] * q[r] : 0.0;
       }
     partial[g] = block_fold(v);
   }

@@ -43,7 +43,7 @@ extern "C" {
 /* Geometry of the HIP kernels' deterministic reductions (restated). */
 #define ORC_BLOCK 256     /* threads per block of every reducing kernel */
 #define ORC_VEC_GMAX 2048 /* max blocks of the grid-stride vector kernels */
-#define ORC_ROWS_MAX 256  /* max rows of one CRS row block */
+#define ORC_ROWS_MAX 256  /* max rows of one SpMV row block (one per thread) */
 #define ORC_NNZ_CAP 2048  /* max nonzeros staged per CRS row block */
 #define ORC_SPMV_GMAX 2048 /* max blocks of the SpMV kernels */
 
@@ -88,7 +88,8 @@ double orc_dot_serial(int N, const double *a, const double *b);
 int orc_vec_grid(int N);
 int orc_crs_rowblocks(int N, const int *ptr, int *starts /* >= N+1 */);
 double orc_dot_tree(int N, const double *a, const double *b);
-/* <p,q> in the order of the fused SpMV+dot kernel over the given row blocks. */
+/* <p,q> in the order of the fused SpMV+dot kernel over the given row blocks
+ * (CRS: orc_crs_rowblocks; ELL: 256-row chunks). */
 double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
                              const double *q);
 

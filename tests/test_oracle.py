@@ -190,16 +190,17 @@ def test_tree_order_is_not_serial_order():
     # Exact-integer tests cannot tell fold patterns apart; this input can.
     n = 512
     a = np.zeros(n)
-    a[0], a[1], a[2] = 1e16, -1e16, 1.0
+    a[0], a[2], a[4] = 1e16, -1e16, 1.0
     b = np.ones(n)
     L = oracle.lib()
     # serial: (1e16 - 1e16) + 1 = 1
     assert L.orc_dot_serial(n, a, b) == 1.0
-    # shift-down fold: lane 0 meets lane 2 at offset 2 (1e16 + 1 rounds to
-    # 1e16), then lane 1 at offset 1 -> 0.  Pairing neighbours first would
-    # give 1, pairing through shared memory in another order would differ.
+    # pairs: thread 0 holds a[0] (+ a[1]), thread 1 a[2], thread 2 a[4]; the
+    # shift-down fold meets lanes 0 and 2 at offset 2 (1e16 + 1 rounds to
+    # 1e16), then lane 1 at offset 1 -> 0.  Pairing neighbouring lanes first,
+    # or visiting elements one per thread, would give 1.
     assert L.orc_dot_tree(n, a, b) == 0.0
-    assert L.orc_vec_grid(n) == 2
+    assert L.orc_vec_grid(n) == 1
 
 
 def test_rowblocks_rule():
@@ -212,6 +213,11 @@ def test_rowblocks_rule():
     starts = np.zeros(m.n + 1, np.int32)
     nb = L.orc_crs_rowblocks(m.n, m.ptr, starts)
     assert nb == m.n // 256 and list(starts[:3]) == [0, 256, 512]
+    m = problems.random_spd_dense(300, 0.05, seed=11)  # ~16 per row: 128-row blocks
+    starts = np.zeros(m.n + 1, np.int32)
+    nb = L.orc_crs_rowblocks(m.n, m.ptr, starts)
+    assert all(m.ptr[starts[i + 1]] - m.ptr[starts[i]] <= 2048 for i in range(nb))
+    assert all(starts[i + 1] - starts[i] <= 256 for i in range(nb))
 
 
 @pytest.mark.parametrize("n,density,seed", [(12, 0.5, 0), (16, 0.3, 2)])
