@@ -459,6 +459,7 @@ int cghip_apply_preconditioner(cghip_ctx *c, int x, int y) {
 
 namespace {
 
+// One fused iteration.  With `ev`, events ev[0..3] bracket the three kernels.
 int enqueue_iteration(cghip_ctx *c, const double *C, History h, hipEvent_t *ev) {
   double *p = c->vec[CGHIP_VEC_P], *q = c->vec[CGHIP_VEC_Q], *r = c->vec[CGHIP_VEC_R],
          *x = c->vec[CGHIP_VEC_X];
@@ -485,9 +486,14 @@ int ensure_events(cghip_ctx *c, size_t count) {
   return CGHIP_OK;
 }
 
+// Milliseconds between two completed events (0 if the runtime cannot tell;
+// the error is cleared so that it is not reported by a later launch check).
 float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, a, b);
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    ms = 0.f;
+  }
   return ms;
 }
 
@@ -524,9 +530,14 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
 
-  const bool profile = o->profile != 0;
+  if (o->profile != CGHIP_PROFILE_NONE && o->profile != CGHIP_PROFILE_KERNELS)
+    return fail(c, CGHIP_ERR_ARG, "unknown profile mode %d", o->profile);
+  const int mode = o->profile;
   int batch = o->batch > 0 ? o->batch : (c->n >= (1 << 20) ? 8 : 32);
-  if (profile && ensure_events(c, 6)) return CGHIP_ERR_HIP;
+  if (mode == CGHIP_PROFILE_KERNELS) batch = 1;
+  // events[0..1] bracket the set-up; events[2..5] the kernels of an iteration.
+  if (mode != CGHIP_PROFILE_NONE && ensure_events(c, 6)) return CGHIP_ERR_HIP;
+  hipEvent_t *ev = mode != CGHIP_PROFILE_NONE ? c->events.data() : nullptr;
 
   auto read_state = [&]() -> int {
     HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state, sizeof(SolveState), hipMemcpyDeviceToHost,
@@ -534,32 +545,40 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CGHIP_OK;
   };
+  // Charges the kernels of one iteration timed by ev[2..5]; its p update was
+  // skipped if the solve converged there.
+  auto charge = [&](bool converged) {
+    hipEvent_t *e = ev + 2;
+    info->ms_matvec += elapsed(e[0], e[1]);
+    info->ms_update += elapsed(e[1], e[2]);
+    info->n_matvec++;
+    info->n_update++;
+    if (!converged) {
+      info->ms_xpay += elapsed(e[2], e[3]);
+      info->n_xpay++;
+    }
+  };
 
   const auto t0 = std::chrono::steady_clock::now();
   // r = A x; r = k - r; p = C r; r2 = <r,r>; rho = <p,r>
-  if (profile) HIPCHK(c, hipEventRecord(c->events[4], c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   HIPCHK(c, launch_matvec(c, c->vec[CGHIP_VEC_X], c->vec[CGHIP_VEC_R]));
   HIPCHK(c, pcg_init(c->n, c->vec[CGHIP_VEC_K], c->vec[CGHIP_VEC_R], c->vec[CGHIP_VEC_P], C,
                      pr_of(c), c->d_state, h, c->stream));
-  if (profile) {
-    HIPCHK(c, hipEventRecord(c->events[5], c->stream));
-    if (int rc = read_state()) return rc;
-    info->ms_init = elapsed(c->events[4], c->events[5]);
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  if (int rc = read_state()) return rc;
+  if (ev) {
+    info->ms_init = elapsed(ev[0], ev[1]);
     info->n_init = 1;
-    // One iteration at a time, each kernel between two events.
+  }
+
+  if (mode == CGHIP_PROFILE_KERNELS) {
     for (;;) {
       const Scalars &nx = c->h_state->next;
       if (nx.done || nx.iteration >= o->max_iter) break;
-      if (int rc = enqueue_iteration(c, C, h, c->events.data())) return rc;
+      if (int rc = enqueue_iteration(c, C, h, ev + 2)) return rc;
       if (int rc = read_state()) return rc;
-      info->ms_matvec += elapsed(c->events[0], c->events[1]);
-      info->ms_update += elapsed(c->events[1], c->events[2]);
-      info->n_matvec++;
-      info->n_update++;
-      if (!c->h_state->next.done) {  // the p update ran
-        info->ms_xpay += elapsed(c->events[2], c->events[3]);
-        info->n_xpay++;
-      }
+      charge(c->h_state->next.done != 0);
     }
   } else {
     if (!c->graph || c->graph_batch != batch || c->graph_format != c->format ||

@@ -1,0 +1,147 @@
+// The conjugate-gradient driver: the abstract class a backend plugs into.
+//
+// Interface of the reference's CG class (CG.h:19-293): the same enums, the
+// same protected kernel hooks and allocation hooks, the same environment
+// variables, command line and report.  A backend is a subclass that
+// implements the hooks and defines CG::getInstance() (one per executable,
+// chosen at link time, as in the reference).
+//
+// One addition: solveFused().  The reference's solve() is not virtual and
+// drives the hooks one by one, each synchronous; a device backend that runs
+// the whole loop on the device overrides solveFused() and returns true, and
+// solve() then takes its iteration count, residual and per-category kernel
+// times.  The default returns false and solve() runs the reference loop.
+#ifndef CGHIP_HOST_CG_H
+#define CGHIP_HOST_CG_H
+
+#include <cassert>
+#include <chrono>
+#include <memory>
+#include <string>
+
+#include "Matrix.h"
+#include "Preconditioner.h"
+#include "def.h"
+
+class CG {
+public:
+  enum Vector { VectorK, VectorX, VectorP, VectorQ, VectorR, VectorZ };
+  enum MatrixFormat { MatrixFormatCOO, MatrixFormatCRS, MatrixFormatELL };
+  enum Preconditioner { PreconditionerNone, PreconditionerJacobi };
+  enum WorkDistributionCalc { WorkDistributionByRow, WorkDistributionByNz };
+
+private:
+  int iteration = 0;
+  int maxIterations = 1000;
+  floatType residual = 0;
+  floatType tolerance = 1e-9;
+  floatType checkTolerance = 1e-5;
+
+  struct Timing {
+    using clock = std::chrono::steady_clock;
+    using duration = std::chrono::duration<double>;
+    duration io{0}, converting{0}, transferTo{0}, transferFrom{0}, check{0};
+    duration solve{0}, matvec{0}, axpy{0}, xpay{0}, vectorDot{0}, preconditioner{0};
+  };
+  Timing timing;
+
+  using time_point = Timing::clock::time_point;
+  time_point now() const { return Timing::clock::now(); }
+
+  // Wall-clock wrappers of the hooks (the hooks are synchronous).
+  void matvec(Vector in, Vector out);
+  void axpy(floatType a, Vector x, Vector y);
+  void xpay(Vector x, floatType a, Vector y);
+  floatType vectorDot(Vector a, Vector b);
+  void applyPreconditioner(Vector x, Vector y);
+
+protected:
+  int N = 0;
+  int nz = 0;
+
+  WorkDistributionCalc workDistributionCalc = WorkDistributionByRow;
+  bool overlappedGather = false;
+
+  MatrixFormat matrixFormat;
+  std::unique_ptr<MatrixCOO> matrixCOO;
+  std::unique_ptr<MatrixCRS> matrixCRS;
+  std::unique_ptr<MatrixELL> matrixELL;
+
+  Preconditioner preconditioner;
+  std::unique_ptr<Jacobi> jacobi;
+
+  floatType *k = nullptr;  // VectorK, the right-hand side
+  floatType *x = nullptr;  // VectorX, the solution
+
+  CG(MatrixFormat defaultMatrixFormat, Preconditioner defaultPreconditioner = PreconditionerNone,
+     bool overlappedGather = false)
+      : overlappedGather(overlappedGather), matrixFormat(defaultMatrixFormat),
+        preconditioner(defaultPreconditioner) {}
+
+  virtual bool supportsMatrixFormat(MatrixFormat format) = 0;
+  virtual bool supportsPreconditioner(Preconditioner) { return false; }
+  // Multi-device work distribution: -1 = none (the only mode here).
+  virtual int getNumberOfChunks() { return -1; }
+  virtual bool supportsOverlappedGather() { return false; }
+
+  virtual void allocateMatrixCRS();
+  virtual void allocateMatrixELL();
+  virtual void allocateJacobi();
+  virtual void allocateK();
+  virtual void deallocateK();
+  virtual void allocateX();
+  virtual void deallocateX();
+
+  virtual void doTransferTo() {}
+  virtual void doTransferFrom() {}
+
+  // Kernel hooks (CG.h:224-237 of the reference).
+  virtual void cpy(Vector dst, Vector src) = 0;                        // dst = src
+  virtual void matvecKernel(Vector x, Vector y) = 0;                   // y = A x
+  virtual void axpyKernel(floatType a, Vector x, Vector y) = 0;        // y = a x + y
+  virtual void xpayKernel(Vector x, floatType a, Vector y) = 0;        // y = x + a y
+  virtual floatType vectorDotKernel(Vector a, Vector b) = 0;           // <a, b>
+  virtual void applyPreconditionerKernel(Vector, Vector) {             // y = B x
+    assert(0 && "Preconditioner not implemented!");
+  }
+
+  // Device-resident loop (see the file comment).  Times are seconds.
+  struct FusedResult {
+    int iterations = 0;
+    floatType residual = 0;
+    double matvec = 0, axpy = 0, xpay = 0, vectorDot = 0, preconditioner = 0;
+  };
+  virtual bool solveFused(int maxIterations, floatType tolerance, FusedResult &result) {
+    (void)maxIterations;
+    (void)tolerance;
+    (void)result;
+    return false;
+  }
+
+  static void printPadded(const char *label, const std::string &value);
+
+public:
+  virtual ~CG() {}
+
+  virtual void parseEnvironment();
+  virtual void init(const char *matrixFile);
+
+  void resetX();
+  void resetNonInitTimings();
+
+  virtual bool needsTransfer() { return false; }
+  void transferTo();
+  void solve();
+  void transferFrom();
+
+  virtual void printSummary();
+  bool check();
+  virtual void cleanup();
+
+  int getIterations() const { return iteration; }
+  floatType getResidual() const { return residual; }
+
+  static CG *getInstance();
+};
+
+#endif

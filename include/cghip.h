@@ -104,16 +104,23 @@ int cghip_apply_preconditioner(cghip_ctx *ctx, int x, int y);  /* CG.h:235 y = C
 
 /* --- fused solver ----------------------------------------------------------
  * The whole loop of CG::solve (CG.cpp:284-385) on the device: three kernels
- * per iteration (SpMV + <p,q>; x/r update + <r,r> + <r,z>; p = C r + b p),
- * alpha, beta and the stop test computed by the last block of each
- * reduction, iterations replayed from a HIP graph, no host round trip per
- * iteration.  Uses X (x0 on entry, solution on return) and K (RHS). */
+ * per iteration (SpMV + <p,q>; x/r update + <r,r> + <r,z>; p = C r + b p).
+ * Each kernel writes per-block partial sums; the next kernel folds them (in
+ * a fixed order) and computes alpha, beta and the stop test itself, so no
+ * host round trip is needed per iteration.  Iterations are replayed from a
+ * HIP graph.  Uses X (x0 on entry, solution on return) and K (RHS). */
+/* Kernel timing of cghip_solve.  NONE: iterations replayed from a HIP graph,
+ * no per-kernel timing.  KERNELS: one iteration at a time, each kernel
+ * between two HIP events on the context's stream (adds one host round trip
+ * per iteration). */
+enum { CGHIP_PROFILE_NONE = 0, CGHIP_PROFILE_KERNELS = 1 };
+
 typedef struct {
   int precond;        /* CGHIP_PRECOND_* */
   int max_iter;       /* CG_MAX_ITER, > 0 */
   double tol;         /* CG_TOLERANCE: stop when sqrt(r2)/sqrt(r2_0) <= tol */
   int batch;          /* iterations per graph launch; 0 = automatic */
-  int profile;        /* 1: launch kernel by kernel and time each with events */
+  int profile;        /* CGHIP_PROFILE_* */
 } cghip_solve_opts;
 
 typedef struct {
@@ -121,7 +128,8 @@ typedef struct {
   double residual;    /* last sqrt(r2)/sqrt(r2_0) */
   double r2;          /* last <r,r> */
   double wall_s;      /* host wall time of the solve, device synchronised */
-  /* profile = 1 only: summed kernel time (ms) and launch counts */
+  /* profile != NONE: summed kernel time (ms) and counts of the kernels that
+   * did work (the SpMV + <p,q>, the x/r update, the p update, the set-up) */
   double ms_matvec, ms_update, ms_xpay, ms_init;
   int n_matvec, n_update, n_xpay, n_init;
 } cghip_solve_info;

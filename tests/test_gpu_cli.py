@@ -1,0 +1,97 @@
+"""The cg_hip executable end to end, next to the reference's own serial binary.
+
+Both binaries read the same .mtx file.  The reference binary
+(oracle/_ref/cg_serial, built from the reference's sources by oracle/Makefile
+and shipped prebuilt) is the checker here; cg_hip is the product.  With
+CG_HIP_FUSED=0 CG_HIP_DOT=serial, cg_hip runs the reference's loop hook by
+hook in the reference's summation order, so every printed line except the
+timings must be identical.  The default fused device loop must reach the
+same iteration count and pass the same check.
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+from cgxx_amd import problems
+from tests.conftest import REPO
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(REPO, "cgxx_amd", "bin", "cg_hip")
+REF = os.path.join(REPO, "oracle", "_ref", "cg_serial")
+TIMED = re.compile(r"^(IO time|Converting time|Transfer to time|Solve time|Transfer from time|"
+                   r"Check time|Total time \(excl\. IO\)|MatVec time|MatVec GFLOP/s|axpy time|"
+                   r"xpay time|vectorDot time|Preconditioner time):")
+
+
+def run(exe, path, **env):
+    reps = env.pop("_reps", None)
+    e = dict(os.environ)
+    e.update({k: str(v) for k, v in env.items()})
+    print(f"run {os.path.basename(exe)} {os.path.basename(path)} {env}", flush=True)
+    args = [exe, path] + ([str(reps)] if reps is not None else [])
+    return subprocess.run(args, capture_output=True, text=True, env=e, timeout=300)
+
+
+def untimed(stdout: str):
+    out = []
+    for ln in stdout.splitlines():
+        if ln.startswith("Transfer to time:") or ln.startswith("Transfer from time:"):
+            continue  # only device backends transfer
+        out.append(TIMED.sub(lambda m: m.group(1) + ": <t>", ln) if TIMED.match(ln) else ln)
+    return [re.sub(r"<t>.*", "<t>", ln) for ln in out]
+
+
+@pytest.fixture(scope="module")
+def mtx(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli")
+    files = {}
+    for name, m, sym in [("lap2d_256", problems.laplace2d(256), False),
+                         ("rand_sym_40", problems.random_spd_dense(40, 0.3, 7), True),
+                         ("diff3d_24", problems.diffusion3d(24, 1.0, 4.0, 5), False)]:
+        p = str(d / f"{name}.mtx")
+        problems.write_mm(p, m, symmetric=sym)
+        files[name] = p
+    return files
+
+
+@pytest.mark.parametrize("name", ["lap2d_256", "rand_sym_40", "diff3d_24"])
+@pytest.mark.parametrize("fmt", ["CRS", "ELL"])
+def test_reference_order_output_identical(mtx, name, fmt):
+    env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=5000)
+    ref = run(REF, mtx[name], **env)
+    ours = run(EXE, mtx[name], CG_HIP_FUSED=0, CG_HIP_DOT="serial", **env)
+    assert ref.returncode == 0 and ours.returncode == 0, ours.stderr
+    assert untimed(ours.stdout) == untimed(ref.stdout)
+
+
+@pytest.mark.parametrize("name", ["lap2d_256", "diff3d_24"])
+@pytest.mark.parametrize("fmt", ["CRS", "ELL"])
+def test_fused_matches_reference_run(mtx, name, fmt):
+    env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=5000)
+    ref = run(REF, mtx[name], **env)
+    ours = run(EXE, mtx[name], **env)
+    assert ours.returncode == 0, ours.stderr
+
+    def field(out, label):
+        return [ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith(label)][0]
+
+    assert abs(int(field(ours.stdout, "Iterations")) - int(field(ref.stdout, "Iterations"))) <= 1
+    assert "Solution is correct!" in ours.stdout
+    # every timing category is reported as a number
+    for label in ("Solve time", "MatVec time", "axpy time", "xpay time", "MatVec GFLOP/s"):
+        float(field(ours.stdout, label))
+    assert float(field(ours.stdout, "MatVec time")) > 0
+
+
+def test_repetitions_and_strict_check(mtx):
+    ours = run(EXE, mtx["lap2d_256"], _reps=2, CG_MATRIX_FORMAT="CRS")
+    assert ours.returncode == 0
+    assert ours.stdout.count("Solution is correct!") == 2
+    assert ours.stdout.count("Iterations:") == 2
+    bad = run(EXE, mtx["lap2d_256"], CG_MAX_ITER=5, CG_STRICT_CHECK=1)
+    assert bad.returncode == 2 and "elements are incorrect!" in bad.stdout
+    lenient = run(EXE, mtx["lap2d_256"], CG_MAX_ITER=5)
+    assert lenient.returncode == 0 and "elements are incorrect!" in lenient.stdout

@@ -255,7 +255,7 @@ def test_fused_profile_and_batches_agree(ctx, tmp_path):
     fx = [f for f in load_golden() if f["case"] == "lap2d_64"][0]
     case = Case(fx, str(tmp_path))
     ref = None
-    for batch, profile in [(0, False), (1, False), (7, False), (0, True)]:
+    for batch, profile in [(0, 0), (1, 0), (7, 0), (0, 1)]:
         _fresh(ctx, case, "CRS", "jacobi")
         info = ctx.solve(precond="jacobi", batch=batch, profile=profile)
         x = ctx.download("X")
@@ -267,7 +267,8 @@ def test_fused_profile_and_batches_agree(ctx, tmp_path):
             assert info["n_matvec"] == info["iterations"]
             assert info["n_update"] == info["iterations"]
             assert info["n_xpay"] == info["iterations"] - 1
-            assert info["ms_matvec"] > 0
+            assert info["ms_matvec"] > 0 and info["ms_update"] > 0 and info["ms_xpay"] > 0
+            assert info["n_init"] == 1 and info["ms_init"] > 0
 
 
 def test_device_sqrt_and_division_are_correctly_rounded(ctx, tmp_path):
