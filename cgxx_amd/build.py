@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
 ]
 
-LIB_SOURCES = ["cghip.hip", "kernels.hip"]
+LIB_SOURCES = ["cghip.hip", "kernels.hip", "kernels_mixed.hip", "cghip_mlp.hip", "kernels_mlp.hip",
+               "kernels_features.hip"]
 HOST_SOURCES = ["CG.cpp", "Matrix.cpp", "Preconditioner.cpp", "CGHIP.cpp", "main.cpp"]
 
 

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcghip.so")
 # CG::Vector (CG.h:22-35)
 VEC = {"K": 0, "X": 1, "P": 2, "Q": 3, "R": 4, "Z": 5}
 FORMAT = {"CRS": 1, "ELL": 2}
-PRECOND = {"none": 0, "jacobi": 1}
+PRECOND = {"none": 0, "jacobi": 1, "poly": 2}
 DOT = {"tree": 0, "serial": 1}
 
 SYMBOLS = [
@@ -26,12 +26,20 @@ SYMBOLS = [
     "cghip_set_format", "cghip_upload_vector", "cghip_download_vector", "cghip_set_dot_mode",
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
-    "cghip_fill_vector", "cghip_true_residual",
+    "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
+    "cghip_set_poly", "cghip_matrix_stats",
+    "cghip_mlp_param_count", "cghip_mlp_create", "cghip_mlp_destroy", "cghip_mlp_last_error",
+    "cghip_mlp_set_params", "cghip_mlp_get_params", "cghip_mlp_forward", "cghip_mlp_gradient",
+    "cghip_mlp_train",
 ]
 
 
 class CGHIPError(RuntimeError):
     pass
+
+
+class AdamOpts(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
 
 
 class SolveOpts(C.Structure):
@@ -44,6 +52,22 @@ class SolveInfo(C.Structure):
                 ("wall_s", C.c_double), ("ms_matvec", C.c_double), ("ms_update", C.c_double),
                 ("ms_xpay", C.c_double), ("ms_init", C.c_double), ("n_matvec", C.c_int),
                 ("n_update", C.c_int), ("n_xpay", C.c_int), ("n_init", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class MixedOpts(C.Structure):
+    _fields_ = [("precond", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
+                ("delta", C.c_double), ("batch", C.c_int), ("profile", C.c_int)]
+
+
+class MixedInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("updates", C.c_int), ("true_residual", C.c_double),
+                ("inner_residual", C.c_double), ("wall_s", C.c_double),
+                ("ms_matvec", C.c_double), ("ms_update", C.c_double), ("ms_xpay", C.c_double),
+                ("ms_refresh", C.c_double), ("n_matvec", C.c_int), ("n_update", C.c_int),
+                ("n_xpay", C.c_int), ("n_refresh", C.c_int)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -87,6 +111,21 @@ def load() -> C.CDLL:
     L.cghip_time_matvec.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.cghip_fill_vector.argtypes = [vp, C.c_int, C.c_double]
     L.cghip_true_residual.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.cghip_prepare_fp32.argtypes = [vp]
+    L.cghip_set_poly.argtypes = [vp, C.c_int, _f64p, C.c_double]
+    L.cghip_matrix_stats.argtypes = [vp, _f64p, C.c_int]
+    _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.cghip_mlp_param_count.argtypes = [C.c_int] * 4
+    L.cghip_mlp_create.argtypes = [C.c_int] * 6 + [C.POINTER(vp)]
+    L.cghip_mlp_destroy.argtypes = [vp]
+    L.cghip_mlp_last_error.argtypes = [vp]
+    L.cghip_mlp_last_error.restype = C.c_char_p
+    L.cghip_mlp_set_params.argtypes = [vp, _f32p]
+    L.cghip_mlp_get_params.argtypes = [vp, _f32p]
+    L.cghip_mlp_forward.argtypes = [vp, _f32p, C.c_int, _f32p]
+    L.cghip_mlp_gradient.argtypes = [vp, _f32p, _f32p, C.c_int, _f32p, C.POINTER(C.c_double)]
+    L.cghip_mlp_train.argtypes = [vp, _f32p, _f32p, C.c_int, C.c_int, C.POINTER(AdamOpts), vp]
+    L.cghip_solve_mixed.argtypes = [vp, C.POINTER(MixedOpts), C.POINTER(MixedInfo)]
     _lib = L
     return L
 
@@ -201,6 +240,27 @@ class Context:
         self._chk(self.L.cghip_solve(self._h, C.byref(o), C.byref(info)), "cghip_solve")
         return info.as_dict()
 
+    def prepare_fp32(self):
+        self._chk(self.L.cghip_prepare_fp32(self._h), "cghip_prepare_fp32")
+
+    def set_poly(self, coeffs, b: float):
+        """Polynomial preconditioner p(D^-1 A) D^-1, p = sum_j coeffs[j] T_j(2 l / b - 1)."""
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        self._chk(self.L.cghip_set_poly(self._h, int(c.size - 1), c, float(b)), "cghip_set_poly")
+
+    def matrix_stats(self) -> np.ndarray:
+        """36 statistics of the CRS matrix (include/cghip.h, cghip_matrix_stats)."""
+        out = np.zeros(36)
+        self._chk(self.L.cghip_matrix_stats(self._h, out, out.size), "cghip_matrix_stats")
+        return out
+
+    def solve_mixed(self, precond="jacobi", max_iter=100000, tol=1e-9, delta=0.1, batch=0,
+                    profile=False) -> dict:
+        o = MixedOpts(PRECOND[precond], max_iter, tol, delta, batch, int(profile))
+        info = MixedInfo()
+        self._chk(self.L.cghip_solve_mixed(self._h, C.byref(o), C.byref(info)), "cghip_solve_mixed")
+        return info.as_dict()
+
     def history(self, which: str, count: int) -> np.ndarray:
         idx = {"r2": 0, "rho": 1, "pq": 2, "alpha": 3, "beta": 4}[which]
         out = np.full(max(count, 1), np.nan)
@@ -272,3 +332,67 @@ def reference_loop(ctx: Context, precond: str, max_iter=1000, tol=1e-9):
         ctx.xpay("R" if precond == "none" else "Z", b, "P")
         it += 1
     return it, residual, {k: np.array(v) for k, v in hist.items()}
+
+
+class Mlp:
+    """The FP32 network of include/cghip_mlp.h: in -> h1 -> h2 -> out, ReLU."""
+
+    def __init__(self, dims, max_rows: int, device: int = 0):
+        self.L = load()
+        self.dims = tuple(int(v) for v in dims)
+        self.total = self.L.cghip_mlp_param_count(*self.dims)
+        self._h = C.c_void_p()
+        rc = self.L.cghip_mlp_create(device, *self.dims, int(max_rows), C.byref(self._h))
+        if rc:
+            raise CGHIPError(f"cghip_mlp_create failed ({rc}): "
+                             f"{self.L.cghip_mlp_last_error(None).decode()}")
+
+    def close(self):
+        if self._h:
+            self.L.cghip_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise CGHIPError(f"{what} failed ({rc}): {self.L.cghip_mlp_last_error(self._h).decode()}")
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != self.total:
+            raise ValueError(f"expected {self.total} parameters, got {p.size}")
+        self._chk(self.L.cghip_mlp_set_params(self._h, p), "cghip_mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.total, np.float32)
+        self._chk(self.L.cghip_mlp_get_params(self._h, p), "cghip_mlp_get_params")
+        return p
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.dims[3]), np.float32)
+        self._chk(self.L.cghip_mlp_forward(self._h, X, X.shape[0], Y), "cghip_mlp_forward")
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.total, np.float32)
+        loss = C.c_double()
+        self._chk(self.L.cghip_mlp_gradient(self._h, X, T, X.shape[0], g, C.byref(loss)),
+                  "cghip_mlp_gradient")
+        return g, loss.value
+
+    def train(self, X, T, steps: int, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.zeros(max(steps, 1))
+        o = AdamOpts(lr, beta1, beta2, eps)
+        self._chk(self.L.cghip_mlp_train(self._h, X, T, X.shape[0], int(steps), C.byref(o),
+                                         hist.ctypes.data), "cghip_mlp_train")
+        return hist[:steps]

@@ -9,6 +9,7 @@
 
 #include "../../include/cghip.h"
 #include "cghip_kernels.h"
+#include "features.h"
 
 #include <algorithm>
 #include <chrono>
@@ -48,7 +49,8 @@ struct cghip_ctx {
   int dot_mode = CGHIP_DOT_TREE;
 
   // reduction scratch
-  double *partial = nullptr;  // 3 x kVecGMax: <p,q> | <r,r> | <r,z> (kSpmvGMax <= kVecGMax)
+  // 4 x kVecGMax: <p,q> | <r,r> | <r,z> | ||k - A x||^2 (kSpmvGMax <= kVecGMax)
+  double *partial = nullptr;
   double *dot_out = nullptr;
 
   // solver
@@ -62,6 +64,24 @@ struct cghip_ctx {
   // graph of `graph_batch` fused iterations
   hipGraphExec_t graph = nullptr;
   int graph_batch = 0, graph_format = 0, graph_precond = -1;
+
+  // mixed precision: FP32 copies of the CRS values and the Jacobi diagonal,
+  // FP32 work vectors x32, r32, p32, q32, the inner loop's state and graph
+  bool has_fp32 = false;
+  float *value32 = nullptr, *jac32 = nullptr;
+  float *v32[4] = {};
+  MixedState *d_mstate = nullptr;
+  MixedState *h_mstate = nullptr;  // pinned
+  hipGraphExec_t graph_mp = nullptr;
+  int graph_mp_batch = 0, graph_mp_precond = -1, graph_mp_degree = 0;
+
+  // polynomial preconditioner: coefficients on the device, the three
+  // Clenshaw work vectors (allocated by cghip_set_poly, freed with the FP32
+  // data)
+  bool has_poly = false;
+  int poly_degree = 0;
+  PolyParams *d_poly = nullptr;
+  float *w32[3] = {};
 
   std::vector<hipEvent_t> events;
 };
@@ -114,6 +134,26 @@ void drop_graph(cghip_ctx *c) {
   c->graph = nullptr;
   c->graph_batch = 0;
   c->graph_precond = -1;
+  if (c->graph_mp) (void)hipGraphExecDestroy(c->graph_mp);
+  c->graph_mp = nullptr;
+  c->graph_mp_batch = 0;
+  c->graph_mp_precond = -1;
+  c->graph_mp_degree = 0;
+}
+
+void free_fp32(cghip_ctx *c) {
+  if (c->value32) (void)hipFree(c->value32);
+  if (c->jac32) (void)hipFree(c->jac32);
+  for (auto &v : c->v32)
+    if (v) (void)hipFree(v);
+  for (auto &v : c->w32)
+    if (v) (void)hipFree(v);
+  c->value32 = c->jac32 = nullptr;
+  for (auto &v : c->v32) v = nullptr;
+  for (auto &v : c->w32) v = nullptr;
+  c->has_fp32 = false;
+  c->has_poly = false;
+  c->poly_degree = 0;
 }
 
 void free_crs(cghip_ctx *c) {
@@ -133,11 +173,13 @@ void free_ell(cghip_ctx *c) {
 
 bool valid_vec(int id) { return id >= CGHIP_VEC_K && id <= CGHIP_VEC_Z; }
 
-// Partials of the SpMV's <p,q>, and of the vector kernels' two sums.
+// Partial-sum slots: the SpMV's <p,q>; the vector kernels' two sums; the
+// true residual of a reliable update.
 Partials pq_of(cghip_ctx *c) { return Partials{c->partial, nullptr}; }
 Partials pr_of(cghip_ctx *c) {
   return Partials{c->partial + kVecGMax, c->partial + 2 * (size_t)kVecGMax};
 }
+Partials pt_of(cghip_ctx *c) { return Partials{c->partial + 3 * (size_t)kVecGMax, nullptr}; }
 
 int spmv_blocks(cghip_ctx *c) {
   return spmv_grid(c->format == CGHIP_FORMAT_CRS ? c->crs_nrb : (c->n + kBlock - 1) / kBlock);
@@ -206,7 +248,8 @@ int cghip_create(int device, cghip_ctx **out) {
   int rc = CGHIP_OK;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     rc = fail(nullptr, CGHIP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
-  if (!rc) rc = dalloc(c, &c->partial, 3 * (size_t)kVecGMax);
+  if (!rc) rc = dalloc(c, &c->partial, 4 * (size_t)kVecGMax);
+  if (!rc) rc = dalloc(c, &c->d_poly, 1);
   if (!rc) rc = dalloc(c, &c->dot_out, 8);
   if (!rc) rc = dalloc(c, &c->d_state, 1);
   if (!rc && (e = hipHostMalloc((void **)&c->h_state, sizeof(SolveState), 0)) != hipSuccess)
@@ -232,13 +275,17 @@ int cghip_destroy(cghip_ctx *c) {
   for (auto &ev : c->events) (void)hipEventDestroy(ev);
   for (auto &v : c->vec) dfree(v);
   free_crs(c);
+  free_fp32(c);
   free_ell(c);
   dfree(c->jac);
   dfree(c->partial);
+  dfree(c->d_poly);
   dfree(c->dot_out);
   dfree(c->d_state);
+  dfree(c->d_mstate);
   dfree(c->d_hist);
   if (c->h_state) (void)hipHostFree(c->h_state);
+  if (c->h_mstate) (void)hipHostFree(c->h_mstate);
   if (c->h_scalar) (void)hipHostFree(c->h_scalar);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -254,6 +301,7 @@ int cghip_set_size(cghip_ctx *c, int n) {
   if (c->n != n) {
     for (auto &v : c->vec) dfree(v);
     free_crs(c);
+    free_fp32(c);
     free_ell(c);
     dfree(c->jac);
     c->has_jac = false;
@@ -284,6 +332,7 @@ int cghip_upload_crs(cghip_ctx *c, int n, int nnz, const int32_t *ptr, const int
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drop_graph(c);
   free_crs(c);
+  free_fp32(c);
   std::vector<int> rb((size_t)n + 1);
   const int nrb = crs_rowblocks(n, ptr, rb.data());
   int rc = dalloc(c, &c->crs_ptr, (size_t)n + 1);
@@ -353,6 +402,7 @@ int cghip_upload_jacobi(cghip_ctx *c, const double *C) {
   if (!c->jac)
     if (int rc = dalloc(c, &c->jac, (size_t)c->n)) return rc;
   HIPCHK(c, hipMemcpy(c->jac, C, sizeof(double) * (size_t)c->n, hipMemcpyHostToDevice));
+  free_fp32(c);  // the FP32 diagonal is stale
   c->has_jac = true;
   return CGHIP_OK;
 }
@@ -626,6 +676,272 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
   return CGHIP_OK;
 }
 
+// ---- mixed precision -----------------------------------------------------
+
+int cghip_prepare_fp32(cghip_ctx *c) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!c->has_crs) return fail(c, CGHIP_ERR_STATE, "mixed precision needs a CRS matrix");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  free_fp32(c);
+  int rc = dalloc(c, &c->value32, (size_t)c->crs_nnz);
+  for (auto &v : c->v32)
+    if (!rc) rc = dalloc(c, &v, (size_t)c->n);
+  if (!rc && c->has_jac) rc = dalloc(c, &c->jac32, (size_t)c->n);
+  if (!rc && !c->d_mstate) rc = dalloc(c, &c->d_mstate, 1);
+  if (!rc && !c->h_mstate) {
+    hipError_t e = hipHostMalloc((void **)&c->h_mstate, sizeof(MixedState), 0);
+    if (e != hipSuccess) rc = fail(c, CGHIP_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    free_fp32(c);
+    return rc;
+  }
+  HIPCHK(c, convert_f64_f32(c->crs_nnz, c->crs_value, c->value32, c->stream));
+  if (c->has_jac) HIPCHK(c, convert_f64_f32(c->n, c->jac, c->jac32, c->stream));
+  for (auto &v : c->v32) HIPCHK(c, hipMemsetAsync(v, 0, sizeof(float) * (size_t)c->n, c->stream));
+  if (int r = sync(c)) return r;
+  c->has_fp32 = true;
+  return CGHIP_OK;
+}
+
+int cghip_set_poly(cghip_ctx *c, int degree, const double *coeffs, double b) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!c->has_fp32) return fail(c, CGHIP_ERR_STATE, "call cghip_prepare_fp32 first");
+  if (!c->jac32)
+    return fail(c, CGHIP_ERR_STATE, "the polynomial preconditioner needs the Jacobi diagonal");
+  if (degree < 1 || degree > kPolyMaxDegree)
+    return fail(c, CGHIP_ERR_ARG, "degree %d outside [1, %d]", degree, kPolyMaxDegree);
+  if (!coeffs) return fail(c, CGHIP_ERR_ARG, "coeffs is NULL");
+  if (!(b > 0) || !std::isfinite(b) || !std::isfinite((float)(2.0 / b)))
+    return fail(c, CGHIP_ERR_ARG, "spectrum bound b = %g must be positive and finite", b);
+  PolyParams pp{};
+  for (int j = 0; j <= degree; j++) {
+    pp.c[j] = (float)coeffs[j];
+    if (!std::isfinite(coeffs[j]) || !std::isfinite(pp.c[j]))
+      return fail(c, CGHIP_ERR_ARG, "coefficient %d is not a finite float", j);
+  }
+  pp.two_over_b = (float)(2.0 / b);
+  pp.degree = degree;
+  // p > 0 on [0, b] (the float-rounded coefficients), evaluated in FP64 by
+  // Clenshaw's recurrence at y = 2 l / b - 1 on a uniform grid.
+  constexpr int kGrid = 4096;
+  for (int i = 0; i < kGrid; i++) {
+    const double y = -1.0 + 2.0 * i / (kGrid - 1);
+    double b1 = 0.0, b2 = 0.0;
+    for (int j = degree; j >= 1; j--) {
+      const double t = (double)pp.c[j] + 2.0 * y * b1 - b2;
+      b2 = b1;
+      b1 = t;
+    }
+    const double pv = (double)pp.c[0] + y * b1 - b2;
+    if (!(pv > 0))
+      return fail(c, CGHIP_ERR_ARG, "p(%g) = %g is not positive: the preconditioner would not be SPD",
+                  (y + 1.0) * 0.5 * b, pv);
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &v : c->w32)
+    if (!v)
+      if (int rc = dalloc(c, &v, (size_t)c->n)) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_poly, &pp, sizeof(PolyParams), hipMemcpyHostToDevice, c->stream));
+  if (int r = sync(c)) return r;
+  c->poly_degree = degree;
+  c->has_poly = true;
+  return CGHIP_OK;
+}
+
+namespace {
+
+int ensure_history(cghip_ctx *c, int max_iter, History *h) {
+  if (c->hist_cap < max_iter + 1) {
+    drop_graph(c);
+    dfree(c->d_hist);
+    c->hist_cap = 0;
+    if (int rc = dalloc(c, &c->d_hist, 5 * ((size_t)max_iter + 1))) return rc;
+    c->hist_cap = max_iter + 1;
+  }
+  const size_t hc = (size_t)c->hist_cap;
+  *h = History{c->d_hist, c->d_hist + hc, c->d_hist + 2 * hc, c->d_hist + 3 * hc,
+               c->d_hist + 4 * hc};
+  return CGHIP_OK;
+}
+
+}  // namespace
+
+int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info *info) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!o || !info) return fail(c, CGHIP_ERR_ARG, "opts/info is NULL");
+  if (!c->has_crs || c->format != CGHIP_FORMAT_CRS)
+    return fail(c, CGHIP_ERR_STATE, "mixed precision needs the CRS format selected");
+  if (!c->has_fp32) return fail(c, CGHIP_ERR_STATE, "call cghip_prepare_fp32 first");
+  if (o->max_iter <= 0) return fail(c, CGHIP_ERR_ARG, "max_iter must be positive");
+  if (!(o->tol > 0)) return fail(c, CGHIP_ERR_ARG, "tol must be positive");
+  if (!(o->delta >= 0 && o->delta < 1)) return fail(c, CGHIP_ERR_ARG, "delta must be in [0, 1)");
+  if (o->precond != CGHIP_PRECOND_NONE && o->precond != CGHIP_PRECOND_JACOBI &&
+      o->precond != CGHIP_PRECOND_POLY)
+    return fail(c, CGHIP_ERR_ARG, "unsupported preconditioner %d", o->precond);
+  if (o->precond == CGHIP_PRECOND_JACOBI && !c->jac32)
+    return fail(c, CGHIP_ERR_STATE, "Jacobi requested but no diagonal uploaded");
+  if (o->precond == CGHIP_PRECOND_POLY && !c->has_poly)
+    return fail(c, CGHIP_ERR_STATE, "polynomial requested: call cghip_set_poly first");
+  if (o->profile != CGHIP_PROFILE_NONE && o->profile != CGHIP_PROFILE_KERNELS)
+    return fail(c, CGHIP_ERR_ARG, "unknown profile mode %d", o->profile);
+  HIPCHK(c, hipSetDevice(c->device));
+  std::memset(info, 0, sizeof(*info));
+  History h;
+  if (int rc = ensure_history(c, o->max_iter, &h)) return rc;
+
+  const int n = c->n;
+  const bool poly = o->precond == CGHIP_PRECOND_POLY;
+  const int degree = poly ? c->poly_degree : 0;
+  const float *C32 = o->precond != CGHIP_PRECOND_NONE ? c->jac32 : nullptr;
+  float *const w[3] = {c->w32[0], c->w32[1], c->w32[2]};
+  float *x32 = c->v32[0], *r32 = c->v32[1], *p32 = c->v32[2], *q32 = c->v32[3];
+  double *x64 = c->vec[CGHIP_VEC_X], *k64 = c->vec[CGHIP_VEC_K], *r64 = c->vec[CGHIP_VEC_R],
+         *q64 = c->vec[CGHIP_VEC_Q];
+  const Crs<float> A32{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb};
+  // slots: <p,q> (SpMV grid), <r,r> and <r,z> (quads grid; <r,z> on the
+  // SpMV grid with the polynomial), ||k - A x||^2 (pairs grid)
+  const Partials pa = pq_of(c), pb = pr_of(c), pt = pt_of(c);
+  const Partials prr{pb.a, nullptr}, prz{pb.b, nullptr};
+  const int gz = poly ? spmv_grid(c->crs_nrb) : vec_grid4(n);
+  const double delta = o->delta > 0 ? o->delta : 0.1;
+
+  MixedState init{};
+  init.tol = o->tol;
+  init.delta2 = delta * delta;
+  init.max_iter = o->max_iter;
+  *c->h_mstate = init;
+  HIPCHK(c, hipMemcpyAsync(c->d_mstate, c->h_mstate, sizeof(MixedState), hipMemcpyHostToDevice,
+                           c->stream));
+  if (int rc = sync(c)) return rc;
+
+  const bool profile = o->profile == CGHIP_PROFILE_KERNELS;
+  const int batch = profile ? 1 : (o->batch > 0 ? o->batch : (n >= (1 << 20) ? 8 : 32));
+  if (profile && ensure_events(c, 6)) return CGHIP_ERR_HIP;
+  hipEvent_t *ev = profile ? c->events.data() : nullptr;
+
+  auto read_state = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(c->h_mstate, c->d_mstate, sizeof(MixedState), hipMemcpyDeviceToHost,
+                             c->stream));
+    return sync(c);
+  };
+  // r = k - A x64 in FP64 (and x64 += x32 first unless this is the start);
+  // r32 restarts from it.
+  auto refresh = [&](bool first) -> int {
+    if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    if (!first) HIPCHK(c, mp_accumulate(n, x64, x32, c->stream));
+    HIPCHK(c, matvec_crs(crs_of(c), x64, q64, c->stream));
+    HIPCHK(c, mp_residual(n, k64, q64, r64, pt, c->stream));
+    if (poly) {
+      HIPCHK(c, mp_restart_poly(n, r64, r32, x32, C32, c->d_poly, w[degree % 3], prr, c->stream));
+      HIPCHK(c, mp_poly_apply(A32, r32, C32, c->d_poly, degree, w, first ? p32 : nullptr, prz,
+                              nullptr, c->stream));
+    } else {
+      HIPCHK(c, mp_restart(n, r64, r32, x32, p32, C32, first, pb, c->stream));
+    }
+    HIPCHK(c, mp_restart_fold(n, pt, prr, prz.a, gz, first, c->d_mstate, h, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    if (int rc = read_state()) return rc;
+    if (ev) {
+      info->ms_refresh += elapsed(ev[0], ev[1]);
+      info->n_refresh++;
+    }
+    return CGHIP_OK;
+  };
+  auto enqueue = [&](hipEvent_t *e) -> int {
+    if (e) HIPCHK(c, hipEventRecord(e[0], c->stream));
+    HIPCHK(c, mp_matvec_dot_crs(A32, p32, q32, pa, c->d_mstate, c->stream));
+    if (e) HIPCHK(c, hipEventRecord(e[1], c->stream));
+    if (poly) {
+      // the p update times the Clenshaw steps and the p update together
+      HIPCHK(c, mp_update_poly(n, spmv_grid(c->crs_nrb), x32, r32, p32, q32, C32, c->d_poly,
+                               w[degree % 3], pa, prr, c->d_mstate, h, c->stream));
+      if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
+      HIPCHK(c, mp_poly_apply(A32, r32, C32, c->d_poly, degree, w, nullptr, prz, c->d_mstate,
+                              c->stream));
+      HIPCHK(c, mp_xpay_poly(n, p32, w[0], prr, prz, gz, c->d_mstate, h, c->stream));
+    } else {
+      HIPCHK(c, mp_update(n, spmv_grid(c->crs_nrb), x32, r32, p32, q32, C32, pa, pb, c->d_mstate,
+                          h, c->stream));
+      if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
+      HIPCHK(c, mp_xpay(n, p32, r32, C32, pb, c->d_mstate, h, c->stream));
+    }
+    if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
+    return CGHIP_OK;
+  };
+
+  if (!profile && (!c->graph_mp || c->graph_mp_batch != batch || c->graph_mp_precond != o->precond ||
+                   c->graph_mp_degree != degree)) {
+    if (c->graph_mp) (void)hipGraphExecDestroy(c->graph_mp);
+    c->graph_mp = nullptr;
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = CGHIP_OK;
+    for (int i = 0; i < batch && !rc; i++) rc = enqueue(nullptr);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, CGHIP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph_mp, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->graph_mp = nullptr;
+      return fail(c, CGHIP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    c->graph_mp_batch = batch;
+    c->graph_mp_precond = o->precond;
+    c->graph_mp_degree = degree;
+  }
+
+  const auto t0 = std::chrono::steady_clock::now();
+  if (int rc = refresh(true)) return rc;
+  // Each pass either runs a batch (advancing the iteration count until
+  // max_iter stops it) or a reliable update (at most one per batch), so the
+  // loop ends; the bound below only guards against a broken state.
+  const long max_passes = 2 * ((long)o->max_iter / batch + 3) + 64;
+  for (long pass = 0;; pass++) {
+    if (pass >= max_passes)
+      return fail(c, CGHIP_ERR_STATE, "mixed solver did not stop after %ld passes", pass);
+    const MixedState &s = *c->h_mstate;
+    if (s.next.done == 1) break;  // true residual <= tol
+    if (s.next.done == 2) {       // reliable update requested by the inner loop
+      if (int rc = refresh(false)) return rc;
+      continue;
+    }
+    if (s.cur.done == 1 || s.next.iteration >= o->max_iter) {
+      if (int rc = refresh(false)) return rc;  // fold x32 into x64, final residual
+      break;
+    }
+    if (profile) {
+      if (int rc = enqueue(ev + 2)) return rc;
+      if (int rc = read_state()) return rc;
+      info->ms_matvec += elapsed(ev[2], ev[3]);
+      info->ms_update += elapsed(ev[3], ev[4]);
+      info->ms_xpay += elapsed(ev[4], ev[5]);
+      info->n_matvec++;
+      info->n_update++;
+      info->n_xpay++;
+    } else {
+      HIPCHK(c, hipGraphLaunch(c->graph_mp, c->stream));
+      if (int rc = read_state()) return rc;
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  const MixedState &s = *c->h_mstate;
+  c->last_iterations = s.next.iteration;
+  info->iterations = s.next.iteration;
+  info->updates = s.updates;
+  info->true_residual = s.true_res;
+  info->inner_residual = s.next.residual;
+  info->wall_s = std::chrono::duration<double>(t1 - t0).count();
+  return CGHIP_OK;
+}
+
 int cghip_history(cghip_ctx *c, int which, double *out, int cap, int *written) {
   if (!c) return CGHIP_ERR_ARG;
   if (which < 0 || which > 4 || !out || cap < 0)
@@ -660,6 +976,51 @@ int cghip_true_residual(cghip_ctx *c, int x, int b, double *norm_r, double *norm
   if (int rc = sync(c)) return rc;
   *norm_r = std::sqrt(c->h_scalar[0]);
   *norm_b = std::sqrt(c->h_scalar[1]);
+  return CGHIP_OK;
+}
+
+int cghip_matrix_stats(cghip_ctx *c, double *stats, int cap) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!stats || cap < CGHIP_MATRIX_STATS) return fail(c, CGHIP_ERR_ARG, "stats needs %d entries", CGHIP_MATRIX_STATS);
+  if (!c->has_crs) return fail(c, CGHIP_ERR_STATE, "no CRS matrix uploaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int G = matrix_stats_grid(c->n);
+  double *d_part = nullptr;
+  if (int rc = dalloc(c, &d_part, (size_t)G * kStatWidth)) return rc;
+  std::vector<double> part((size_t)G * kStatWidth);
+  hipError_t e = matrix_stats(crs_of(c), d_part, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(d_part);
+  if (e != hipSuccess) return fail(c, CGHIP_ERR_HIP, "matrix statistics: %s", hipGetErrorString(e));
+  // fold the blocks in ascending order
+  double acc[kStatWidth];
+  for (int k = 0; k < kStatWidth; k++) acc[k] = part[k];
+  for (int g = 1; g < G; g++) {
+    const double *o = part.data() + (size_t)g * kStatWidth;
+    for (int q = 0; q < kStatQuantities; q++) {
+      for (int k = 0; k < 3; k++) acc[5 * q + k] += o[5 * q + k];
+      acc[5 * q + 3] = std::min(acc[5 * q + 3], o[5 * q + 3]);
+      acc[5 * q + 4] = std::max(acc[5 * q + 4], o[5 * q + 4]);
+    }
+    for (int k = 5 * kStatQuantities; k < kStatWidth; k++) acc[k] += o[k];
+  }
+  const double bad = acc[kStatWidth - 1];
+  if (bad > 0)
+    return fail(c, CGHIP_ERR_ARG, "%.0f rows have no positive diagonal entry", bad);
+  for (int q = 0; q < kStatQuantities; q++) {
+    const double n = acc[5 * q], s = acc[5 * q + 1], s2 = acc[5 * q + 2];
+    const double mean = n > 0 ? s / n : 0.0;
+    const double var = n > 0 ? std::max(s2 / n - mean * mean, 0.0) : 0.0;
+    stats[4 * q + 0] = mean;
+    stats[4 * q + 1] = std::sqrt(var);
+    stats[4 * q + 2] = n > 0 ? acc[5 * q + 3] : 0.0;
+    stats[4 * q + 3] = n > 0 ? acc[5 * q + 4] : 0.0;
+  }
+  for (int b = 0; b < kStatHistBins; b++)
+    stats[4 * kStatQuantities + b] = acc[5 * kStatQuantities + b] / (double)c->n;
   return CGHIP_OK;
 }
 

@@ -27,6 +27,15 @@ inline int vec_grid(int n) {
   return (int)g;
 }
 
+// FP32 vector kernels work on element quads: G = ceil(quads / 256), <= 2048.
+inline int vec_grid4(int n) {
+  const int64_t quads = ((int64_t)n + 3) / 4;
+  int64_t g = (quads + kBlock - 1) / kBlock;
+  if (g > kVecGMax) g = kVecGMax;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
 // SpMV kernels: block g handles row blocks g, g + G, g + 2G, ...
 inline int spmv_grid(int nrb) {
   int64_t g = nrb;
@@ -111,5 +120,78 @@ hipError_t pcg_update(int n, int gpq, double *x, double *r, const double *p, con
                       hipStream_t s);
 hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, Partials pr,
                     SolveState *st, History h, hipStream_t s);
+
+// ---- mixed precision: FP32 inner PCG with FP64 reliable updates ----------
+//
+// Inner iteration in FP32 (matrix values, x32/r32/p32/q32, the Jacobi
+// diagonal), every dot accumulated in FP64 from exact float x float
+// products.  When the recursive residual has dropped by `delta` since the
+// last reliable update (or reached the tolerance), the inner loop pauses
+// (done = 2): the host then adds x32 into the FP64 solution, recomputes the
+// true residual k - A x in FP64, restarts r32 from it and keeps p32.
+
+struct MixedScalars {
+  double rho;       // <r32, z32>
+  double r2;        // <r32, r32>
+  double residual;  // sqrt(r2) / nrm2_0, the recursive relative residual
+  double r2_ref;    // r2 right after the last reliable update
+  int iteration;    // inner iterations done
+  int done;         // 0 run, 1 stop, 2 reliable update requested
+};
+
+struct MixedState {
+  MixedScalars cur, next;
+  double nrm2_0;     // ||k - A x0|| (FP64)
+  double true_res;   // ||k - A x|| / nrm2_0 at the last reliable update
+  double tol, delta2;
+  int max_iter, updates;
+};
+
+// Polynomial preconditioner z = p(M) D^-1 r with M = D^-1 A and
+// p(l) = sum_j c[j] T_j(2 l / b - 1) (Chebyshev basis on [0, b]), applied by
+// Clenshaw's recurrence: B_j = c_j u + 2 s(M) B_{j+1} - B_{j+2} for
+// j = k-1 .. 1, z = c_0 u + s(M) B_1 - B_2, with u = D^-1 r,
+// s(M) v = (2/b) D^-1 A v - v and B_k = c_k u.  One SpMV per step.
+constexpr int kPolyMaxDegree = 16;
+struct PolyParams {
+  float c[kPolyMaxDegree + 1];
+  float two_over_b;
+  int degree;
+};
+
+// y = (float)x
+hipError_t convert_f64_f32(int64_t n, const double *x, float *y, hipStream_t s);
+// x64 += (double)x32
+hipError_t mp_accumulate(int n, double *x64, const float *x32, hipStream_t s);
+// r64 = k - q64; partials.a = ||r64||^2 (pairs order)
+hipError_t mp_residual(int n, const double *k, const double *q64, double *r64, Partials pa,
+                       hipStream_t s);
+// r32 = (float)r64; x32 = 0; z = C32 r32; p32 = z if first; partials of
+// <r32,r32> and <r32,z> (quads order) into pb.
+hipError_t mp_restart(int n, const double *r64, float *r32, float *x32, float *p32,
+                      const float *C32, bool first, Partials pb, hipStream_t s);
+// One block: folds ||r64||^2 (pairs, prt.a), <r,r> (quads, prr.a) and <r,z>
+// (rz, gz partials) into the restarted state.
+hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, int gz, bool first,
+                           MixedState *st, History h, hipStream_t s);
+// Polynomial variants.  Work buffers w[0..2] hold the Clenshaw B_j (B_j in
+// w[j % 3]); z ends in w[0].
+hipError_t mp_restart_poly(int n, const double *r64, float *r32, float *x32, const float *C32,
+                           const PolyParams *poly, float *bk, Partials prr, hipStream_t s);
+hipError_t mp_poly_apply(const Crs<float> &A, const float *r32, const float *C32,
+                         const PolyParams *poly, int degree, float *const w[3], float *pcopy,
+                         Partials prz, const MixedState *st, hipStream_t s);
+hipError_t mp_update_poly(int n, int gpq, float *x32, float *r32, const float *p32,
+                          const float *q32, const float *C32, const PolyParams *poly, float *bk,
+                          Partials pq, Partials prr, MixedState *st, History h, hipStream_t s);
+hipError_t mp_xpay_poly(int n, float *p32, const float *z32, Partials prr, Partials prz, int gz,
+                        MixedState *st, History h, hipStream_t s);
+hipError_t mp_matvec_dot_crs(const Crs<float> &A, const float *p, float *q, Partials pq,
+                             MixedState *st, hipStream_t s);
+hipError_t mp_update(int n, int gpq, float *x32, float *r32, const float *p32, const float *q32,
+                     const float *C32, Partials pq, Partials pr, MixedState *st, History h,
+                     hipStream_t s);
+hipError_t mp_xpay(int n, float *p32, const float *r32, const float *C32, Partials pr,
+                   MixedState *st, History h, hipStream_t s);
 
 }  // namespace cghip

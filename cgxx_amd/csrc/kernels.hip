@@ -25,7 +25,7 @@
 // Floating-point contraction is off (build flag and pragma): a*x+y is a
 // rounded multiply then a rounded add, as in the reference's CPU kernels.
 
-#include "cghip_kernels.h"
+#include "device_common.h"
 
 #pragma clang fp contract(off)
 
@@ -33,130 +33,7 @@ namespace cghip {
 
 namespace {
 
-constexpr int kRounds = kNnzCap / kBlock;  // load rounds per row block
-constexpr int kEllChunk = 8;               // ELL slots in flight per thread
-constexpr int kUnroll = 4;                 // element pairs in flight per thread
-
-// ---------------------------------------------------------------------------
-// Reductions.
-
-// Lane l accumulates lane l+off for off = 32, 16, ..., 1; lane 0 ends with
-// the wave's sum.
-__device__ inline double wave_fold(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Sum over the 256 threads, returned to every thread: wave sums w0..w3 are
-// added left to right.  Ends with a barrier so that `lds` can be reused.
-__device__ inline double block_fold(double v, double *lds) {
-  v = wave_fold(v);
-  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  __syncthreads();
-  return r;
-}
-
-// Fold of G partials written by an earlier kernel: thread t adds partials
-// t, t + 256, ... left to right, then the block fold.  Every block that calls
-// it gets the same value.
-__device__ inline double fold_partials(const double *partial, int G, double *lds) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < G; i += kBlock) s += partial[i];
-  return block_fold(s, lds);
-}
-
-// ---------------------------------------------------------------------------
-// SpMV.
-
-// CSR-stream of row block rb.  Returns the thread's product x[r] * y[r] for
-// the row it owns (0 if none) when WITH_DOT.
-template <typename V, typename X, bool WITH_DOT>
-__device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, X *__restrict__ y,
-                                      int rb, X *__restrict__ prod) {
-  const int t = threadIdx.x;
-  const int r0 = A.rb_start[rb], r1 = A.rb_start[rb + 1];
-  const int e0 = A.ptr[r0], e1 = A.ptr[r1];
-  const int r = r0 + t;
-  const bool own = r < r1;
-  double dp = 0.0;
-  if (e1 - e0 <= kNnzCap) {
-    int c[kRounds];
-    V v[kRounds];
-#pragma unroll
-    for (int u = 0; u < kRounds; u++) {
-      const int e = e0 + u * kBlock + t;
-      if (e < e1) {
-        c[u] = A.index[e];
-        v[u] = A.value[e];
-      }
-    }
-    int rb0 = 0, re0 = 0;
-    if (own) {
-      rb0 = A.ptr[r];
-      re0 = A.ptr[r + 1];
-    }
-#pragma unroll
-    for (int u = 0; u < kRounds; u++) {
-      const int e = e0 + u * kBlock + t;
-      if (e < e1) prod[e - e0] = X(v[u]) * x[c[u]];
-    }
-    __syncthreads();
-    if (own) {
-      X s = X(0);
-      for (int j = rb0; j < re0; j++) s += prod[j - e0];
-      y[r] = s;
-      if (WITH_DOT) dp = (double)x[r] * (double)s;
-    }
-    __syncthreads();
-  } else {
-    // One row longer than kNnzCap: chunks through LDS, thread 0 adds in order.
-    X s = X(0);
-    for (int c0 = e0; c0 < e1; c0 += kNnzCap) {
-      const int ce = min(c0 + kNnzCap, e1);
-      for (int e = c0 + t; e < ce; e += kBlock) prod[e - c0] = X(A.value[e]) * x[A.index[e]];
-      __syncthreads();
-      if (t == 0)
-        for (int j = 0; j < ce - c0; j++) s += prod[j];
-      __syncthreads();
-    }
-    if (t == 0) {
-      y[r0] = s;
-      if (WITH_DOT) dp = (double)x[r0] * (double)s;
-    }
-  }
-  return dp;
-}
-
-// ELL rows rb*256 + t: the thread adds its live slots left to right.
-template <typename V, typename X, bool WITH_DOT>
-__device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, X *__restrict__ y,
-                                      int rb) {
-  const int r = rb * kBlock + (int)threadIdx.x;
-  if (r >= A.n) return 0.0;
-  const int64_t n = A.n;
-  const int len = A.length[r];
-  X s = X(0);
-  for (int j0 = 0; j0 < len; j0 += kEllChunk) {
-    int c[kEllChunk];
-    V v[kEllChunk];
-#pragma unroll
-    for (int u = 0; u < kEllChunk; u++) {
-      if (j0 + u < len) {
-        const int64_t k = (int64_t)(j0 + u) * n + r;
-        c[u] = A.index[k];
-        v[u] = A.data[k];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kEllChunk; u++)
-      if (j0 + u < len) s += X(v[u]) * x[c[u]];
-  }
-  y[r] = s;
-  return WITH_DOT ? (double)x[r] * (double)s : 0.0;
-}
+using namespace dev;
 
 template <typename V, typename X>
 __global__ __launch_bounds__(kBlock) void k_matvec_crs(Crs<V> A, const X *__restrict__ x,
@@ -217,16 +94,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_ell(Ell<double> A, co
 // Vector kernels.  Element pairs (2q, 2q+1); thread gid of T = 256*G visits
 // pairs gid, gid + T, ... in order, element 2q before 2q+1.  The last pair
 // of an odd-length vector holds one element.
-
-struct Pairs {
-  int64_t T, full, npair, q;
-  __device__ explicit Pairs(int n)
-      : T((int64_t)gridDim.x * kBlock), full(n >> 1), npair(((int64_t)n + 1) >> 1),
-        q((int64_t)blockIdx.x * kBlock + threadIdx.x) {}
-};
-
-__device__ inline const double2 *v2(const double *p) { return reinterpret_cast<const double2 *>(p); }
-__device__ inline double2 *v2(double *p) { return reinterpret_cast<double2 *>(p); }
 
 __global__ __launch_bounds__(kBlock) void k_axpy(int n, double a, const double *__restrict__ x,
                                                  double *__restrict__ y) {

@@ -49,8 +49,9 @@ enum {
  * runs it only in SerialCG, serial/SerialCG.cpp:79-85). */
 enum { CGHIP_FORMAT_CRS = 1, CGHIP_FORMAT_ELL = 2 };
 
-/* CG::Preconditioner (CG.h:50-55). */
-enum { CGHIP_PRECOND_NONE = 0, CGHIP_PRECOND_JACOBI = 1 };
+/* CG::Preconditioner (CG.h:50-55), plus POLY: a polynomial in D^-1 A set by
+ * cghip_set_poly (mixed-precision solver only; no reference counterpart). */
+enum { CGHIP_PRECOND_NONE = 0, CGHIP_PRECOND_JACOBI = 1, CGHIP_PRECOND_POLY = 2 };
 
 /* Summation order of the dot-product hook.  TREE: deterministic single-pass
  * wave64 tree (fixed order for a given length).  SERIAL: left to right, the
@@ -137,6 +138,51 @@ typedef struct {
 int cghip_solve(cghip_ctx *ctx, const cghip_solve_opts *opts,
                 cghip_solve_info *info);
 
+/* --- mixed precision (no counterpart in the reference, whose arithmetic is
+ * FP64 throughout, def.h:6) ------------------------------------------------
+ * An FP32 inner PCG (FP32 matrix values and vectors, int32 indices, every
+ * dot accumulated in FP64) with FP64 reliable updates: whenever the inner
+ * recursive residual has dropped by `delta` (or reached `tol`), the FP32
+ * correction is added to the FP64 solution X, the true residual K - A X is
+ * recomputed in FP64 with the FP64 matrix, and the inner iteration goes on
+ * from it with its search direction kept.  The stop test is the FP64 true
+ * relative residual ||K - A X|| / ||K - A X0|| <= tol.  CRS only. */
+
+/* Builds the FP32 copy of the CRS values and of the Jacobi diagonal and the
+ * FP32 work vectors (after cghip_upload_crs / cghip_upload_jacobi). */
+int cghip_prepare_fp32(cghip_ctx *ctx);
+
+/* Polynomial preconditioner B = p(D^-1 A) D^-1 with
+ * p(l) = sum_{j=0}^{degree} coeffs[j] T_j(2 l / b - 1), the Chebyshev basis
+ * on [0, b]; b must bound the spectrum of D^-1 A from above (e.g. its
+ * Gershgorin bound).  B is SPD when p > 0 on the spectrum: the call rejects
+ * (CGHIP_ERR_ARG) coefficients with p <= 0 anywhere on a 4096-point grid of
+ * [0, b].  1 <= degree <= 16.  Applied by Clenshaw's recurrence, one FP32
+ * SpMV per degree, fused into the SpMV kernels. */
+int cghip_set_poly(cghip_ctx *ctx, int degree, const double *coeffs, double b);
+
+typedef struct {
+  int precond;       /* CGHIP_PRECOND_* */
+  int max_iter;      /* inner iterations, > 0 */
+  double tol;        /* FP64 true relative residual */
+  double delta;      /* reliable-update factor in (0, 1); 0 = 0.1 */
+  int batch;         /* inner iterations per graph launch; 0 = automatic */
+  int profile;       /* CGHIP_PROFILE_* */
+} cghip_mixed_opts;
+
+typedef struct {
+  int iterations;          /* inner FP32 iterations */
+  int updates;             /* reliable updates (FP64 residual recomputations) */
+  double true_residual;    /* final FP64 ||K - A X|| / ||K - A X0|| */
+  double inner_residual;   /* final FP32 recursive relative residual */
+  double wall_s;
+  double ms_matvec, ms_update, ms_xpay, ms_refresh;  /* profile: summed kernel ms */
+  int n_matvec, n_update, n_xpay, n_refresh;
+} cghip_mixed_info;
+
+int cghip_solve_mixed(cghip_ctx *ctx, const cghip_mixed_opts *opts,
+                      cghip_mixed_info *info);
+
 /* Per-iteration scalars of the last cghip_solve: which = 0 r2, 1 rho,
  * 2 dot_pq, 3 alpha, 4 beta.  r2/rho have iterations + 1 entries (index 0 is
  * before the loop), the others iterations.  Returns the count written. */
@@ -152,6 +198,17 @@ int cghip_fill_vector(cghip_ctx *ctx, int id, double value);
  * Overwrites vector Q with A x. */
 int cghip_true_residual(cghip_ctx *ctx, int x, int b, double *norm_r,
                         double *norm_b);
+
+/* Cheap statistics of the CRS matrix (no reference counterpart; inputs of
+ * the learned polynomial preconditioner), one pass over the matrix on the
+ * device.  Per row: q0 = log a_ii, q1 = sum_{j!=i} |a_ij| / a_ii, q2 = log of
+ * the largest and q3 of the smallest nonzero |a_ij| / a_ii (j != i),
+ * q4 = q2 - q3 (q2..q4 over rows with off-diagonal entries).  stats[0..19]:
+ * mean, std, min, max of q0..q4 (4 per quantity); stats[20..35]: fractions
+ * of rows of length 1..15 and >= 16.  CGHIP_ERR_ARG if a row has no positive
+ * diagonal. */
+#define CGHIP_MATRIX_STATS 36
+int cghip_matrix_stats(cghip_ctx *ctx, double *stats, int cap);
 
 /* Average duration (ms) of `reps` back-to-back launches of the matvec kernel
  * x -> y, timed with HIP events on the context's stream. */

@@ -550,3 +550,265 @@ int orc_pcg(const orc_problem *p, const double *k, double *x, orc_result *res,
   free(c.starts);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Mixed precision: FP32 inner PCG with FP64 reliable updates.               */
+
+int orc_vec_grid4(int N) {
+  int64_t quads = ((int64_t)N + 3) / 4;
+  int64_t g = (quads + ORC_BLOCK - 1) / ORC_BLOCK;
+  if (g > ORC_VEC_GMAX) g = ORC_VEC_GMAX;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+/* sum_i f[i] over quads order of the FP32 vector kernels, f precomputed. */
+static double quad_tree(int N, const double *f) {
+  int G = orc_vec_grid4(N);
+  int64_t T = (int64_t)G * ORC_BLOCK;
+  int64_t quads = ((int64_t)N + 3) / 4;
+  double *partial = (double *)malloc(sizeof(double) * (size_t)G);
+  double v[ORC_BLOCK];
+  for (int g = 0; g < G; g++) {
+    for (int t = 0; t < ORC_BLOCK; t++) {
+      double s = 0.0;
+      for (int64_t q = (int64_t)g * ORC_BLOCK + t; q < quads; q += T)
+        for (int64_t i = 4 * q; i < 4 * q + 4 && i < N; i++) s += f[i];
+      v[t] = s;
+    }
+    partial[g] = block_fold(v);
+  }
+  double r = final_fold(partial, G);
+  free(partial);
+  return r;
+}
+
+typedef struct {
+  double rho, r2, residual, r2_ref;
+  int iteration, done;
+} mp_scalars;
+
+/* Clenshaw's recurrence for z = p(M) u, M = D^-1 A, u = C32 r32, in FP32 with
+ * the expression order of the HIP kernels' epilogues: B_j lives in
+ * w[j % 3]; on entry w[degree % 3] holds B_degree = c_degree u; for
+ * j = degree-1 .. 0 one FP32 row-ordered SpMV t = A32 B_{j+1}, then
+ * s = (2/b) (C t) - B_{j+1}, B_j = (c_j u + 2 s) - B_{j+2} (j > 0) or
+ * z = (c_0 u + s) - B_2 (j = 0), B_{j+2} = 0 when j + 2 > degree.
+ * z ends in w[0]. */
+static void poly_clenshaw(int N, const int *ptr, const int *index,
+                          const float *v32, const float *C32,
+                          const float *r32, const orc_poly *pl, float *w[3]) {
+  const float tb = pl->two_over_b;
+  for (int j = pl->degree - 1; j >= 0; j--) {
+    const float *b1 = w[(j + 1) % 3];
+    const float *b2 = (j + 2 <= pl->degree) ? w[(j + 2) % 3] : NULL;
+    float *b0 = w[j % 3];
+    const float cj = pl->c[j];
+    for (int i = 0; i < N; i++) {
+      float t = 0.0f;
+      for (int e = ptr[i]; e < ptr[i + 1]; e++) t += v32[e] * b1[index[e]];
+      const float u = C32[i] * r32[i];
+      const float sv = tb * (C32[i] * t) - b1[i];
+      const float bb = b2 ? b2[i] : 0.0f;
+      b0[i] = j > 0 ? (cj * u + 2.0f * sv) - bb : (cj * u + sv) - bb;
+    }
+  }
+}
+
+int orc_poly_apply(int N, const int *ptr, const int *index,
+                   const double *value, const double *C, const float *r32,
+                   const orc_poly *pl, float *z) {
+  if (!pl || pl->degree < 1 || pl->degree > ORC_POLY_MAX) return -2;
+  size_t nb = (size_t)(N > 0 ? N : 1);
+  int nz = ptr[N];
+  float *v32 = (float *)malloc(sizeof(float) * (size_t)(nz > 0 ? nz : 1));
+  float *C32 = (float *)malloc(sizeof(float) * nb);
+  float *w[3];
+  for (int j = 0; j < 3; j++) w[j] = (float *)calloc(nb, sizeof(float));
+  for (int e = 0; e < nz; e++) v32[e] = (float)value[e];
+  for (int i = 0; i < N; i++) C32[i] = (float)C[i];
+  const float ck = pl->c[pl->degree];
+  for (int i = 0; i < N; i++) w[pl->degree % 3][i] = ck * (C32[i] * r32[i]);
+  poly_clenshaw(N, ptr, index, v32, C32, r32, pl, w);
+  memcpy(z, w[0], sizeof(float) * (size_t)N);
+  free(v32), free(C32);
+  for (int j = 0; j < 3; j++) free(w[j]);
+  return 0;
+}
+
+int orc_mixed_pcg(const orc_problem *p, const double *k, double *x,
+                  double delta, orc_mixed_result *res, double *h_r2,
+                  double *h_rho, double *h_pq, double *h_alpha,
+                  double *h_beta) {
+  return orc_mixed_pcg_poly(p, NULL, k, x, delta, res, h_r2, h_rho, h_pq,
+                            h_alpha, h_beta);
+}
+
+int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
+                       const double *k, double *x, double delta,
+                       orc_mixed_result *res, double *h_r2, double *h_rho,
+                       double *h_pq, double *h_alpha, double *h_beta) {
+  if (p->format != 1) return -1;
+  if (p->precond == 2 && (!pl || pl->degree < 1 || pl->degree > ORC_POLY_MAX))
+    return -2;
+  int N = p->N, nz = p->ptr[N];
+  int jac = p->precond == 1;
+  int poly = p->precond == 2;
+  size_t nb = (size_t)(N > 0 ? N : 1);
+  float *v32 = (float *)malloc(sizeof(float) * (size_t)(nz > 0 ? nz : 1));
+  float *C32 = (float *)malloc(sizeof(float) * nb);
+  float *x32 = (float *)calloc(nb, sizeof(float)), *r32 = (float *)malloc(sizeof(float) * nb),
+        *p32 = (float *)malloc(sizeof(float) * nb), *q32 = (float *)malloc(sizeof(float) * nb);
+  double *q64 = (double *)malloc(sizeof(double) * nb), *r64 = (double *)malloc(sizeof(double) * nb),
+         *f0 = (double *)malloc(sizeof(double) * nb), *f1 = (double *)malloc(sizeof(double) * nb);
+  float *w[3] = {NULL, NULL, NULL};
+  if (poly)
+    for (int j = 0; j < 3; j++) w[j] = (float *)calloc(nb, sizeof(float));
+  const int deg = poly ? pl->degree : 0;
+  const float ck = poly ? pl->c[deg] : 0.0f;
+  int nrb = 0;
+  int *starts = (int *)malloc(sizeof(int) * (nb + 1));
+  nrb = orc_crs_rowblocks(N, p->ptr, starts);
+  for (int e = 0; e < nz; e++) v32[e] = (float)p->value[e];
+  for (int i = 0; i < N; i++) C32[i] = (jac || poly) ? (float)p->C[i] : 1.0f;
+  double delta2 = (delta > 0 ? delta : 0.1) * (delta > 0 ? delta : 0.1);
+  double nrm2_0 = 0, true_res = 0;
+  int updates = 0;
+  mp_scalars s = {0, 0, 0, 0, 0, 0};
+
+  /* refresh: x64 += x32 (unless first); r64 = k - A x64; restart r32. */
+#define REFRESH(first)                                                        \
+  do {                                                                        \
+    if (!(first))                                                             \
+      for (int i = 0; i < N; i++) x[i] = x[i] + (double)x32[i];               \
+    orc_spmv_crs(N, p->ptr, p->index, p->value, x, q64);                      \
+    for (int i = 0; i < N; i++) r64[i] = k[i] - q64[i];                       \
+    double rt2 = orc_dot_tree(N, r64, r64);                                   \
+    double rr, rz;                                                            \
+    if (poly) {                                                               \
+      for (int i = 0; i < N; i++) {                                           \
+        float r = (float)r64[i];                                              \
+        r32[i] = r;                                                           \
+        x32[i] = 0.0f;                                                        \
+        w[deg % 3][i] = ck * (C32[i] * r);                                    \
+        f0[i] = (double)r * (double)r;                                        \
+      }                                                                       \
+      rr = quad_tree(N, f0);                                                  \
+      poly_clenshaw(N, p->ptr, p->index, v32, C32, r32, pl, w);               \
+      for (int i = 0; i < N; i++) {                                           \
+        if (first) p32[i] = w[0][i];                                          \
+        f0[i] = (double)r32[i];                                               \
+        f1[i] = (double)w[0][i];                                              \
+      }                                                                       \
+      rz = orc_rowblock_dot_tree(nrb, starts, f0, f1);                        \
+    } else {                                                                  \
+      for (int i = 0; i < N; i++) {                                           \
+        float r = (float)r64[i];                                              \
+        r32[i] = r;                                                           \
+        x32[i] = 0.0f;                                                        \
+        float z = jac ? C32[i] * r : r;                                       \
+        if (first) p32[i] = z;                                                \
+        f0[i] = (double)r * (double)r;                                        \
+        f1[i] = (double)r * (double)z;                                        \
+      }                                                                       \
+      rr = quad_tree(N, f0);                                                  \
+      rz = quad_tree(N, f1);                                                  \
+    }                                                                         \
+    if (first) {                                                              \
+      nrm2_0 = sqrt(rt2);                                                     \
+      updates = 0;                                                            \
+      s.iteration = 0;                                                        \
+    } else {                                                                  \
+      updates++;                                                              \
+    }                                                                         \
+    true_res = sqrt(rt2) / nrm2_0;                                            \
+    s.rho = rz;                                                               \
+    s.r2 = rr;                                                                \
+    s.r2_ref = rr;                                                            \
+    s.residual = sqrt(rr) / nrm2_0;                                           \
+    s.done = true_res <= p->tol ? 1 : 0;                                      \
+    if ((first) && h_r2) h_r2[0] = rr;                                        \
+    if ((first) && h_rho) h_rho[0] = rz;                                      \
+  } while (0)
+
+  REFRESH(1);
+  for (;;) {
+    if (s.done == 1) break;
+    if (s.done == 2) {
+      REFRESH(0);
+      continue;
+    }
+    if (s.iteration >= p->max_iter) {
+      REFRESH(0);
+      break;
+    }
+    int it = s.iteration;
+    /* q32 = A32 p32, rows in order in FP32 */
+    for (int i = 0; i < N; i++) {
+      float t = 0.0f;
+      for (int j = p->ptr[i]; j < p->ptr[i + 1]; j++) t += v32[j] * p32[p->index[j]];
+      q32[i] = t;
+    }
+    for (int i = 0; i < N; i++) f0[i] = (double)p32[i];
+    for (int i = 0; i < N; i++) f1[i] = (double)q32[i];
+    double pq = orc_rowblock_dot_tree(nrb, starts, f0, f1);
+    double alpha = s.rho / pq;
+    if (h_pq) h_pq[it] = pq;
+    if (h_alpha) h_alpha[it] = alpha;
+    float a = (float)alpha, na = (float)(-alpha);
+    double rr, rz;
+    if (poly) {
+      for (int i = 0; i < N; i++) {
+        x32[i] = x32[i] + a * p32[i];
+        float r = r32[i] + na * q32[i];
+        r32[i] = r;
+        w[deg % 3][i] = ck * (C32[i] * r);
+        f0[i] = (double)r * (double)r;
+      }
+      rr = quad_tree(N, f0);
+      poly_clenshaw(N, p->ptr, p->index, v32, C32, r32, pl, w);
+      for (int i = 0; i < N; i++) {
+        f0[i] = (double)r32[i];
+        f1[i] = (double)w[0][i];
+      }
+      rz = orc_rowblock_dot_tree(nrb, starts, f0, f1);
+    } else {
+      for (int i = 0; i < N; i++) {
+        x32[i] = x32[i] + a * p32[i];
+        float r = r32[i] + na * q32[i];
+        r32[i] = r;
+        float z = jac ? C32[i] * r : r;
+        f0[i] = (double)r * (double)r;
+        f1[i] = (double)r * (double)z;
+      }
+      rr = quad_tree(N, f0);
+      rz = quad_tree(N, f1);
+    }
+    double resid = sqrt(rr) / nrm2_0;
+    double beta = rz / s.rho;
+    mp_scalars n2 = s;
+    n2.iteration = it + 1;
+    n2.rho = rz;
+    n2.r2 = rr;
+    n2.residual = resid;
+    if (resid <= p->tol || rr <= delta2 * s.r2_ref) n2.done = 2;
+    if (h_r2) h_r2[it + 1] = rr;
+    if (h_rho) h_rho[it + 1] = rz;
+    if (h_beta) h_beta[it] = beta;
+    float b = (float)beta;
+    for (int i = 0; i < N; i++) {
+      float z = poly ? w[0][i] : jac ? C32[i] * r32[i] : r32[i];
+      p32[i] = z + b * p32[i];
+    }
+    s = n2;
+  }
+#undef REFRESH
+  res->iterations = s.iteration;
+  res->updates = updates;
+  res->true_residual = true_res;
+  res->inner_residual = s.residual;
+  free(v32), free(C32), free(x32), free(r32), free(p32), free(q32);
+  free(q64), free(r64), free(f0), free(f1), free(starts);
+  for (int j = 0; j < 3; j++) free(w[j]);
+  return 0;
+}

@@ -131,6 +131,54 @@ int orc_pcg(const orc_problem *p, const double *k, double *x, orc_result *res,
             double *h_r2, double *h_rho, double *h_pq, double *h_alpha,
             double *h_beta);
 
+/* FP32 vector kernels work on element quads: G = ceil(quads / 256) <= 2048. */
+int orc_vec_grid4(int N);
+
+/* The FP32-inner / FP64 reliable-update PCG of the HIP library (no
+ * reference counterpart), restated kernel by kernel with the same casts:
+ * FP32 matrix values (float)value, FP32 vectors, FP32 row sums in row order,
+ * FP64 dots of exact float products in the kernels' tree orders, alpha and
+ * beta in FP64 rounded to FP32 for the updates, x64 += x32 and
+ * r = k - A x64 in FP64 at every reliable update, stop on the FP64 true
+ * relative residual.  CRS only; precond 0 none, 1 Jacobi (C32 = (float)C),
+ * 2 polynomial (orc_mixed_pcg_poly).
+ * x holds x0 on entry and the solution on return.  History (nullable,
+ * max_iter + 1 entries) as for orc_pcg, of the inner recursion. */
+typedef struct {
+  int iterations;
+  int updates;
+  double true_residual;
+  double inner_residual;
+} orc_mixed_result;
+
+int orc_mixed_pcg(const orc_problem *p, const double *k, double *x,
+                  double delta, orc_mixed_result *res, double *h_r2,
+                  double *h_rho, double *h_pq, double *h_alpha,
+                  double *h_beta);
+
+/* Polynomial preconditioner of the mixed solver (precond 2): z = p(M) D^-1 r,
+ * M = D^-1 A, p(l) = sum_j c[j] T_j(2 l / b - 1), FP32 coefficients and
+ * two_over_b = (float)(2 / b) as the library rounds them.  C (the Jacobi
+ * array) holds 1 / A_ii.  The rho = <r, z> sums are in the SpMV row-block
+ * tree order (z comes out of the last Clenshaw SpMV). */
+#define ORC_POLY_MAX 16
+typedef struct {
+  int degree;
+  float c[ORC_POLY_MAX + 1];
+  float two_over_b;
+} orc_poly;
+
+/* z = p(D^-1 A) D^-1 r in FP32 with the library's expression order
+ * (values and C rounded to float as the library does). */
+int orc_poly_apply(int N, const int *ptr, const int *index,
+                   const double *value, const double *C, const float *r32,
+                   const orc_poly *pl, float *z);
+
+int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
+                       const double *k, double *x, double delta,
+                       orc_mixed_result *res, double *h_r2, double *h_rho,
+                       double *h_pq, double *h_alpha, double *h_beta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,3 +201,92 @@ def pcg(*, n, fmt, precond, order, k, x0=None, tol=1e-9, max_iter=1000, nthreads
         "beta": hist["beta"][:it],
     }
     return x, it, res.residual, h
+
+
+class MixedResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("updates", C.c_int), ("true_residual", C.c_double),
+                ("inner_residual", C.c_double)]
+
+
+class Poly(C.Structure):
+    _fields_ = [("degree", C.c_int), ("c", C.c_float * 17), ("two_over_b", C.c_float)]
+
+
+def make_poly(coeffs, b):
+    """orc_poly from Chebyshev coefficients on [0, b], rounded as the library does."""
+    pl = Poly()
+    pl.degree = len(coeffs) - 1
+    if not 1 <= pl.degree <= 16:
+        raise ValueError("degree must be in [1, 16]")
+    for j, cj in enumerate(coeffs):
+        pl.c[j] = float(np.float32(cj))
+    pl.two_over_b = float(np.float32(2.0 / b))
+    return pl
+
+
+def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max_iter=100000,
+              delta=0.1, history=True, poly=None):
+    """The oracle's FP32-inner / FP64 reliable-update PCG (CRS).
+
+    precond "poly" needs poly=(coeffs, b) (Chebyshev coefficients on [0, b])
+    and C_jac.  Returns (x, info dict, history dict)."""
+    L = lib()
+    if not hasattr(L, "_mixed_bound"):
+        L.orc_mixed_pcg_poly.argtypes = [C.POINTER(Problem), C.c_void_p, _f64p, _f64p, C.c_double,
+                                         C.POINTER(MixedResult)] + [C.c_void_p] * 5
+        L.orc_vec_grid4.argtypes = [C.c_int]
+        L._mixed_bound = True
+    p = Problem()
+    p.format = 1
+    p.precond = {"none": 0, "jacobi": 1, "poly": 2}[precond]
+    pl = None
+    if precond == "poly":
+        if poly is None:
+            raise ValueError("precond 'poly' needs poly=(coeffs, b)")
+        pl = make_poly(*poly)
+    p.max_iter = max_iter
+    p.tol = tol
+    p.N = n
+    ptr, idx, val = (np.ascontiguousarray(a) for a in crs)
+    p.ptr, p.index, p.value = ptr.ctypes.data, idx.ctypes.data, val.ctypes.data
+    p.nz = int(ptr[-1])
+    keep = [ptr, idx, val]
+    if precond in ("jacobi", "poly"):
+        cj = np.ascontiguousarray(C_jac, dtype=np.float64)
+        keep.append(cj)
+        p.C = cj.ctypes.data
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    hist = {key: np.full(max_iter + 1, np.nan) for key in ("r2", "rho", "pq", "alpha", "beta")}
+    ptrs = [hist[key].ctypes.data if history else None for key in ("r2", "rho", "pq", "alpha", "beta")]
+    res = MixedResult()
+    rc = L.orc_mixed_pcg_poly(C.byref(p), C.cast(C.byref(pl), C.c_void_p) if pl else None, k, x,
+                              delta, C.byref(res), *ptrs)
+    if rc:
+        raise ValueError(f"orc_mixed_pcg failed with code {rc}")
+    it = res.iterations
+    h = {"r2": hist["r2"][: it + 1], "rho": hist["rho"][: it + 1], "pq": hist["pq"][:it],
+         "alpha": hist["alpha"][:it], "beta": hist["beta"][:it]}
+    info = {"iterations": it, "updates": res.updates, "true_residual": res.true_residual,
+            "inner_residual": res.inner_residual}
+    return x, info, h
+
+
+def poly_apply(crs, C_jac, r32, coeffs, b):
+    """z = p(D^-1 A) D^-1 r in FP32, the library's expression order."""
+    L = lib()
+    ptr = np.ascontiguousarray(crs[0], dtype=np.int32)
+    idx = np.ascontiguousarray(crs[1], dtype=np.int32)
+    val = np.ascontiguousarray(crs[2], dtype=np.float64)
+    n = ptr.size - 1
+    cj = np.ascontiguousarray(C_jac, dtype=np.float64)
+    r = np.ascontiguousarray(r32, dtype=np.float32)
+    z = np.zeros(n, dtype=np.float32)
+    pl = make_poly(coeffs, b)
+    L.orc_poly_apply.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]
+    rc = L.orc_poly_apply(n, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, cj.ctypes.data,
+                          r.ctypes.data, C.cast(C.byref(pl), C.c_void_p), z.ctypes.data)
+    if rc:
+        raise ValueError(f"orc_poly_apply failed with code {rc}")
+    return z

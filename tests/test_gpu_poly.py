@@ -1,0 +1,125 @@
+"""The polynomial preconditioner of the mixed solver on the GPU.
+
+Bit-exact against the oracle's restatement (same FP32 Clenshaw expression
+order, same reduction trees), for degrees that exercise every rotation of
+the three work vectors; argument checks of cghip_set_poly; the FP64 true
+residual of the result recomputed on the host.
+"""
+import numpy as np
+import pytest
+
+from cgxx_amd import problems, spectrum
+from oracle import oracle
+from tests.cases import Case, assert_bitwise
+from tests.conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cgxx_amd.cghip import Context
+
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _setup(ctx, ptr, idx, val, k, C):
+    ctx.set_size(ptr.size - 1)
+    ctx.upload_crs(ptr, idx, val)
+    ctx.set_format("CRS")
+    ctx.upload_jacobi(C)
+    ctx.prepare_fp32()
+    ctx.upload("K", k)
+    ctx.upload("X", np.zeros(ptr.size - 1))
+
+
+def _coeffs(ptr, idx, val, degree, ratio=100.0):
+    b = spectrum.gershgorin_bound(ptr, idx, val)
+    return spectrum.cheb_residual_coeffs(b / ratio, b, degree), b
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 8, 16])
+@pytest.mark.parametrize("name", ["lap2d_64", "diff3d_24", "rand_gen_300"])
+def test_poly_bitwise_vs_oracle(ctx, name, degree, tmp_path):
+    fx = [f for f in load_golden() if f["case"] == name][0]
+    case = Case(fx, str(tmp_path))
+    ptr, idx, val = case.crs
+    _setup(ctx, ptr, idx, val, case.k, case.C)
+    c, b = _coeffs(ptr, idx, val, degree)
+    ctx.set_poly(c, b)
+    info = ctx.solve_mixed(precond="poly", max_iter=20000, tol=1e-9)
+    x = ctx.download("X")
+    ox, oinfo, oh = oracle.mixed_pcg(n=case.n, crs=case.crs, k=case.k, precond="poly", C_jac=case.C,
+                                     poly=(c, b), max_iter=20000, tol=1e-9)
+    assert (info["iterations"], info["updates"]) == (oinfo["iterations"], oinfo["updates"])
+    it = info["iterations"]
+    assert_bitwise(ctx.history("r2", it + 1), oh["r2"], "inner r2")
+    assert_bitwise(ctx.history("rho", it + 1), oh["rho"], "rho = <r, z>")
+    assert_bitwise(ctx.history("alpha", it), oh["alpha"], "alpha")
+    assert_bitwise(x, ox, "x")
+    r = case.k - case.gen.matvec(x)
+    assert np.linalg.norm(r) / np.linalg.norm(case.k) <= 1.05e-9
+    assert np.abs(x - 1.0).max() <= 1e-5
+
+
+def test_poly_profile_and_batch_agree(ctx, tmp_path):
+    fx = [f for f in load_golden() if f["case"] == "diff3d_24"][0]
+    case = Case(fx, str(tmp_path))
+    ref = None
+    for batch, profile in [(0, False), (1, False), (7, False), (0, True)]:
+        _setup(ctx, *case.crs, case.k, case.C)
+        ctx.set_poly(*_coeffs(*case.crs, 8))
+        info = ctx.solve_mixed(precond="poly", batch=batch, profile=profile)
+        x = ctx.download("X")
+        if ref is None:
+            ref = (info["iterations"], x)
+        assert info["iterations"] == ref[0]
+        assert_bitwise(x, ref[1], f"batch={batch} profile={profile}")
+        if profile:
+            assert info["n_matvec"] == info["iterations"] and info["n_refresh"] == info["updates"] + 1
+
+
+def test_set_poly_rejects(ctx, tmp_path):
+    from cgxx_amd.cghip import CGHIPError
+
+    fx = [f for f in load_golden() if f["case"] == "lap2d_16"][0]
+    case = Case(fx, str(tmp_path))
+    _setup(ctx, *case.crs, case.k, case.C)
+    with pytest.raises(CGHIPError, match="cghip_set_poly"):
+        ctx.solve_mixed(precond="poly")
+    bad = [
+        ([1.0], 2.0),                      # degree 0
+        ([1.0] * 18, 2.0),                 # degree 17
+        ([1.0, 0.5], 0.0),                 # b <= 0
+        ([1.0, 0.5], float("nan")),
+        ([0.5, 1.0], 2.0),                 # p(0) = 0.5 - 1.0 < 0
+        ([1.0, float("inf")], 2.0),
+    ]
+    for c, b in bad:
+        with pytest.raises(CGHIPError):
+            ctx.set_poly(c, b)
+    # a matrix upload drops the polynomial with the FP32 data
+    ctx.set_poly(*_coeffs(*case.crs, 4))
+    ctx.upload_crs(*case.crs)
+    ctx.prepare_fp32()
+    with pytest.raises(CGHIPError):
+        ctx.solve_mixed(precond="poly")
+
+
+def test_poly_full_size_c2(ctx):
+    """128^3 Laplacian: degree-8 polynomial reaches the FP64 true residual."""
+    m = problems.laplace3d(128)
+    k = m.ones_rhs()
+    C = 1.0 / m.diagonal()
+    _setup(ctx, m.ptr, m.index, m.value, k, C)
+    ctx.set_poly(*_coeffs(m.ptr, m.index, m.value, 8, ratio=300.0))
+    info = ctx.solve_mixed(precond="poly")
+    nr, nb = ctx.true_residual("X", "K")
+    assert nr / nb <= 1e-9
+    x = ctx.download("X")
+    assert np.abs(x - 1.0).max() <= 1e-5
+    ctx.fill("X", 0.0)
+    infoj = ctx.solve_mixed(precond="jacobi")
+    assert info["iterations"] < infoj["iterations"] / 3

@@ -1,0 +1,152 @@
+"""Polynomial preconditioner and spectral estimates (host logic + oracle).
+
+Pins (none of these has a reference counterpart):
+  - Ritz values from PCG alpha/beta against the closed-form spectrum of the
+    1D Dirichlet Laplacian, lambda_k = 2 - 2 cos(k pi / (n + 1)), Jacobi-
+    scaled by its constant diagonal 2; inside [lambda_min, lambda_max] and
+    monotone in the number of steps (Cauchy interlacing).
+  - Chebyshev residual polynomial: max |1 - l p(l)| on [a, b] equals the
+    closed form 1 / T_{k+1}((b + a) / (b - a)); p > 0 on [0, b].
+  - The oracle's FP32 Clenshaw application against the dense FP64 operator
+    p(D^-1 A) D^-1 built by the three-term recurrence T_{j+1} = 2 Y T_j - T_{j-1}
+    (brute force), and its symmetry <B u, v> = <u, B v>.
+  - The oracle's mixed solver with the polynomial reaches the FP64 true
+    residual and x = 1.
+"""
+import numpy as np
+import pytest
+
+from cgxx_amd import problems, spectrum
+from cgxx_amd.problems import CRS
+from oracle import oracle
+
+
+def laplace1d(n: int) -> CRS:
+    ptr, idx, val = [0], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                idx.append(j)
+                val.append(v)
+        ptr.append(len(idx))
+    return CRS(n, np.array(ptr, np.int32), np.array(idx, np.int32), np.array(val))
+
+
+def dense(m: CRS) -> np.ndarray:
+    A = np.zeros((m.n, m.n))
+    for i in range(m.n):
+        A[i, m.index[m.ptr[i]:m.ptr[i + 1]]] = m.value[m.ptr[i]:m.ptr[i + 1]]
+    return A
+
+
+def dense_poly_operator(A, C, coeffs, b):
+    """p(D^-1 A) D^-1 by the Chebyshev three-term recurrence, FP64."""
+    n = A.shape[0]
+    Y = 2.0 * (np.diag(C) @ A) / b - np.eye(n)
+    T0, T1 = np.eye(n), Y
+    P = coeffs[0] * T0 + coeffs[1] * T1
+    for j in range(2, len(coeffs)):
+        T0, T1 = T1, 2.0 * Y @ T1 - T0
+        P = P + coeffs[j] * T1
+    return P @ np.diag(C)
+
+
+def test_ritz_closed_form_laplace1d():
+    n = 40
+    m = laplace1d(n)
+    C = np.full(n, 0.5)
+    lam = np.sort(1.0 - np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))  # D^-1 A, D = 2
+    k = np.random.default_rng(7).uniform(-1, 1, n)
+    _, it, _, h = oracle.pcg(n=n, fmt="CRS", precond="jacobi", order=0, k=k, tol=1e-300,
+                             max_iter=n, crs=(m.ptr, m.index, m.value), C_jac=C)
+    lo_prev, hi_prev = np.inf, -np.inf
+    for steps in range(1, 16):
+        th = spectrum.ritz_values(h["alpha"][:steps], h["beta"][:steps - 1])
+        assert th[0] >= lam[0] * (1 - 1e-10) and th[-1] <= lam[-1] * (1 + 1e-10)
+        assert th[0] <= lo_prev * (1 + 1e-12) and th[-1] >= hi_prev * (1 - 1e-12)
+        lo_prev, hi_prev = th[0], th[-1]
+    # after n steps in exact arithmetic the Ritz values are the eigenvalues;
+    # the extreme ones survive the loss of orthogonality
+    th = spectrum.ritz_values(h["alpha"][:n], h["beta"][:n - 1])
+    assert abs(th[-1] - lam[-1]) <= 1e-9 and abs(th[0] - lam[0]) <= 1e-9
+
+
+def test_ritz_rejects_bad_input():
+    with pytest.raises(ValueError):
+        spectrum.ritz_values([], [])
+    with pytest.raises(ValueError):
+        spectrum.ritz_values([1.0, 2.0, 3.0], [0.5])
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 8, 16])
+def test_chebyshev_residual_closed_form(degree):
+    a, b = 0.003, 2.0
+    c = spectrum.cheb_residual_coeffs(a, b, degree)
+    assert c.size == degree + 1
+    lam = np.linspace(a, b, 20001)
+    res = 1.0 - lam * spectrum.poly_eval(c, b, lam)
+    bound = 1.0 / np.cosh((degree + 1) * np.arccosh((b + a) / (b - a)))
+    assert abs(np.abs(res).max() - bound) <= 1e-6 * max(bound, 1e-3)
+    assert spectrum.poly_eval(c, b, np.linspace(0, b, 4096)).min() > 0
+
+
+def test_gershgorin_bound():
+    m = problems.laplace2d(8)
+    assert spectrum.gershgorin_bound(m.ptr, m.index, m.value) == 2.0
+    m = problems.random_spd_dense(30, 0.5, 3)
+    lam = np.linalg.eigvals(np.diag(1 / m.diagonal()) @ dense(m)).real
+    assert lam.max() <= spectrum.gershgorin_bound(m.ptr, m.index, m.value) * (1 + 1e-12)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 8, 16])
+@pytest.mark.parametrize("name", ["lap2d", "spd"])
+def test_oracle_poly_apply_brute_force(name, degree):
+    m = problems.laplace2d(7) if name == "lap2d" else problems.random_spd_dense(40, 0.3, 11)
+    C = 1.0 / m.diagonal()
+    b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+    c = spectrum.cheb_residual_coeffs(b / 50, b, degree)
+    c32 = c.astype(np.float32).astype(np.float64)
+    b32 = 2.0 / float(np.float32(2.0 / b))
+    B = dense_poly_operator(dense(m), C, c32, b32)
+    rng = np.random.default_rng(degree)
+    for _ in range(3):
+        r = rng.standard_normal(m.n).astype(np.float32)
+        z = oracle.poly_apply((m.ptr, m.index, m.value), C, r, c, b)
+        zr = B @ r.astype(np.float64)
+        # FP32 Clenshaw: error grows with the degree and with sum |c_j|
+        tol = 2e-6 * (degree + 1) * np.abs(c32).sum() * np.abs(C).max() * np.abs(r).max()
+        assert np.abs(z - zr).max() <= tol
+    # symmetry of the operator (FP32 application)
+    u = rng.standard_normal(m.n).astype(np.float32)
+    v = rng.standard_normal(m.n).astype(np.float32)
+    bu = oracle.poly_apply((m.ptr, m.index, m.value), C, u, c, b).astype(np.float64)
+    bv = oracle.poly_apply((m.ptr, m.index, m.value), C, v, c, b).astype(np.float64)
+    lhs, rhs = bu @ v, u @ bv
+    assert abs(lhs - rhs) <= 1e-5 * np.abs(B).max() * m.n
+    np.testing.assert_allclose(B, B.T, rtol=0, atol=1e-10 * np.abs(B).max())
+
+
+def test_oracle_poly_is_textbook_jacobi_for_constant():
+    """degree 1 with c = (c0, 0): z = c0 D^-1 r exactly (the Jacobi step)."""
+    m = problems.laplace2d(6)
+    C = 1.0 / m.diagonal()
+    r = np.random.default_rng(0).standard_normal(m.n).astype(np.float32)
+    z = oracle.poly_apply((m.ptr, m.index, m.value), C, r, [1.0, 0.0], 2.0)
+    np.testing.assert_array_equal(z, np.float32(1.0) * (C.astype(np.float32) * r))
+
+
+@pytest.mark.parametrize("degree", [2, 8])
+def test_oracle_mixed_poly_converges(degree):
+    m = problems.diffusion3d(12, 1.0, 3.0, 5)
+    k = m.ones_rhs()
+    C = 1.0 / m.diagonal()
+    b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+    c = spectrum.cheb_residual_coeffs(b / 200, b, degree)
+    crs = (m.ptr, m.index, m.value)
+    x, info, h = oracle.mixed_pcg(n=m.n, crs=crs, k=k, precond="poly", C_jac=C, poly=(c, b))
+    xj, infoj, _ = oracle.mixed_pcg(n=m.n, crs=crs, k=k, precond="jacobi", C_jac=C)
+    r = k - m.matvec(x)
+    assert np.linalg.norm(r) / np.linalg.norm(k) <= 1.05e-9
+    assert info["true_residual"] <= 1e-9
+    assert np.abs(x - 1).max() <= 1e-5
+    assert info["iterations"] < infoj["iterations"]
