@@ -1,0 +1,146 @@
+"""What the trained network gains over the method without it, on held-out
+systems of the anisotropic diffusion family, measured on the GPU.
+
+For each held-out seed, four solves from x = 0 to FP64 true relative
+residual 1e-9:
+  learned  mixed precision, polynomial from the network (features and
+           inference timed with the solve), Jacobi fallback
+  fixed    mixed precision, Chebyshev on [theta_min(10) / rho*, b] (rho*
+           tuned on the training systems), the same fallback
+  ritz     the same with rho = 1: Chebyshev on the 10-step Ritz interval
+  jacobi   mixed precision, Jacobi
+  fp64     the plain FP64 Jacobi-PCG loop (the reference's method)
+
+    python -m cgxx_amd.ablation --seeds 100000:100200 [--model PATH] [--out res.npz]
+
+Prints a progress line per 10 systems and a JSON summary at the end.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import sys
+import time
+
+import numpy as np
+
+
+def _generate(seed: int):
+    from cgxx_amd import problems
+
+    m = problems.anisotropic_family(seed)
+    return seed, m.ptr, m.index, m.value, m.ones_rhs(), 1.0 / m.diagonal()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--seeds", required=True)
+    ap.add_argument("--model", default=None)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--inflight", type=int, default=12)
+    args = ap.parse_args()
+    lo, hi = (int(v) for v in args.seeds.split(":"))
+    seeds = list(range(lo, hi))
+    pool = mp.get_context("spawn").Pool(args.workers)
+
+    from cgxx_amd import learned
+    from cgxx_amd.cghip import Context, Mlp
+
+    model = learned.load_model(args.model or learned.MODEL_PATH)
+    rho = float(model["rho_star"])
+    ctx = Context(0)
+    mlp = Mlp(tuple(int(v) for v in model["dims"]), 1)
+    mlp.set_params(model["params"])
+    rows = []
+    q = []
+    nxt = 0
+
+    def top_up():
+        nonlocal nxt
+        while nxt < len(seeds) and len(q) < args.inflight:
+            q.append(pool.apply_async(_generate, (seeds[nxt],)))
+            nxt += 1
+
+    top_up()
+    t0 = time.perf_counter()
+    while q:
+        seed, ptr, idx, val, k, C = q.pop(0).get()
+        top_up()
+        n = ptr.size - 1
+        ctx.set_size(n)
+        ctx.upload_crs(ptr, idx, val)
+        ctx.set_format("CRS")
+        ctx.upload_jacobi(C)
+        ctx.prepare_fp32()
+        ctx.upload("K", k)
+        del ptr, idx, val, k, C
+        r = {"seed": seed, "n": n}
+        lr = learned.solve_learned(ctx, mlp, model, n)
+        r.update({f"learned_{key}": lr[key] for key in ("spmvs", "wall_s", "true_residual",
+                                                           "poly_iterations", "jacobi_iterations")})
+        r["learned_method"] = lr["method"]
+        t1 = time.perf_counter()
+        f, b, theta10 = learned.system_features(ctx, n)
+        fr = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b)
+        fr["wall_s"] = time.perf_counter() - t1
+        r.update({f"fixed_{key}": fr[key] for key in ("spmvs", "wall_s", "true_residual",
+                                                         "poly_iterations", "jacobi_iterations")})
+        r["fixed_method"] = fr["method"]
+        t1 = time.perf_counter()
+        f, b, theta10 = learned.system_features(ctx, n)
+        rr_ = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, 1.0), b)
+        rr_["wall_s"] = time.perf_counter() - t1
+        r.update({f"ritz_{key}": rr_[key] for key in ("spmvs", "wall_s", "true_residual",
+                                                         "poly_iterations", "jacobi_iterations")})
+        r["ritz_method"] = rr_["method"]
+        ctx.fill("X", 0.0)
+        jm = ctx.solve_mixed(precond="jacobi")
+        r.update({"jacobi_spmvs": jm["iterations"], "jacobi_wall_s": jm["wall_s"],
+                  "jacobi_true_residual": jm["true_residual"]})
+        ctx.fill("X", 0.0)
+        f64 = ctx.solve(precond="jacobi", max_iter=100000, tol=1e-9)
+        nr, nb = ctx.true_residual("X", "K")
+        r.update({"fp64_spmvs": f64["iterations"], "fp64_wall_s": f64["wall_s"],
+                  "fp64_true_residual": nr / nb})
+        rows.append(r)
+        if len(rows) % 10 == 0 or len(rows) == len(seeds):
+            print(f"[ablation] {len(rows)}/{len(seeds)} systems, {time.perf_counter() - t0:.0f} s "
+                  f"(seed {seed}: learned {r['learned_spmvs']} / fixed {r['fixed_spmvs']} / "
+                  f"jacobi {r['jacobi_spmvs']} SpMVs)", flush=True)
+    mlp.close()
+    ctx.close()
+    pool.close()
+    pool.join()
+
+    cols = {key: np.array([r[key] for r in rows]) for key in rows[0]}
+    if args.out:
+        np.savez_compressed(args.out, **cols)
+
+    def tot(key):
+        return float(cols[key].sum())
+
+    summary = {
+        "systems": len(rows),
+        "rho_star": rho,
+        "spmvs": {m: tot(f"{m}_spmvs") for m in ("learned", "fixed", "ritz", "jacobi", "fp64")},
+        "seconds": {m: tot(f"{m}_wall_s") for m in ("learned", "fixed", "ritz", "jacobi", "fp64")},
+        "learned_vs_fixed_spmvs": tot("fixed_spmvs") / tot("learned_spmvs"),
+        "learned_vs_ritz_spmvs": tot("ritz_spmvs") / tot("learned_spmvs"),
+        "learned_vs_jacobi_spmvs": tot("jacobi_spmvs") / tot("learned_spmvs"),
+        "learned_vs_fixed_seconds": tot("fixed_wall_s") / tot("learned_wall_s"),
+        "learned_vs_jacobi_mixed_seconds": tot("jacobi_wall_s") / tot("learned_wall_s"),
+        "learned_vs_fp64_seconds": tot("fp64_wall_s") / tot("learned_wall_s"),
+        "learned_wins": int(np.sum(cols["learned_spmvs"] < cols["fixed_spmvs"])),
+        "learned_losses": int(np.sum(cols["learned_spmvs"] > cols["fixed_spmvs"])),
+        "fallbacks": {m: int(np.sum(cols[f"{m}_method"] != "poly")) for m in ("learned", "fixed", "ritz")},
+        "max_true_residual": {m: float(cols[f"{m}_true_residual"].max())
+                              for m in ("learned", "fixed", "ritz", "jacobi", "fp64")},
+    }
+    print(json.dumps(summary), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -29,6 +29,7 @@ grid of lower ends a.
 from __future__ import annotations
 
 import os
+import time
 
 import numpy as np
 
@@ -216,3 +217,67 @@ def predict_coeffs(mlp, model, f, b: float) -> np.ndarray:
     yn = mlp.forward(normalize(f, model))[0].astype(np.float64)
     y = yn * model["out_std"] + model["out_mean"]
     return decode_coeffs(y, b)
+
+
+def fixed_rule_coeffs(theta10: float, b: float, rho: float) -> np.ndarray:
+    """The method without the network: Chebyshev on [theta_min(10) / rho, b]."""
+    return spectrum.cheb_residual_coeffs(min(theta10 / rho, 0.25 * b), b, DEGREE)
+
+
+def solve_poly_with_fallback(ctx, coeffs, b: float, tol: float = 1e-9, max_iter: int = 100000,
+                             budget: int | None = None) -> dict:
+    """Mixed-precision solve of the uploaded system from X = 0 with the
+    polynomial preconditioner, falling back to Jacobi.
+
+    The polynomial run may use `budget` SpMVs (default: max_iter, the budget
+    a Jacobi run would get).  If the library rejects the polynomial (not
+    positive on [0, b]) the solve runs with Jacobi from the start; if the
+    polynomial run ends above tol (FP64 true residual), Jacobi continues
+    from its X until the true residual relative to the initial one is
+    <= tol.  Returns the method used, iteration and SpMV counts, the final
+    FP64 true relative residual and the wall time."""
+    from .cghip import CGHIPError
+
+    t0 = time.perf_counter()
+    k = len(coeffs) - 1
+    budget = max_iter if budget is None else int(budget)
+    ctx.fill("X", 0.0)
+    res = {"method": "poly", "poly_iterations": 0, "jacobi_iterations": 0, "updates": 0}
+    try:
+        ctx.set_poly(coeffs, b)
+        ok = True
+    except CGHIPError as e:
+        ok = False
+        res["method"] = "jacobi"
+        res["rejected"] = str(e)
+    rel = 1.0
+    if ok:
+        info = ctx.solve_mixed(precond="poly", max_iter=max(1, budget // (k + 1)), tol=tol)
+        res["poly_iterations"] = info["iterations"]
+        res["updates"] += info["updates"]
+        rel = info["true_residual"]
+    if rel > tol:
+        if ok:
+            res["method"] = "poly+jacobi"
+        info = ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=tol / rel)
+        res["jacobi_iterations"] = info["iterations"]
+        res["updates"] += info["updates"]
+        rel = rel * info["true_residual"]
+    res["true_residual"] = rel
+    res["spmvs"] = res["poly_iterations"] * (k + 1) + res["jacobi_iterations"]
+    res["wall_s"] = time.perf_counter() - t0
+    return res
+
+
+def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 100000,
+                  budget: int | None = None) -> dict:
+    """Features on the device, coefficients from the network (on the GPU),
+    then solve_poly_with_fallback.  The wall time includes the features."""
+    t0 = time.perf_counter()
+    f, b, _ = system_features(ctx, n)
+    c = predict_coeffs(mlp, model, f, b)
+    res = solve_poly_with_fallback(ctx, c, b, tol, max_iter, budget)
+    res["wall_s"] = time.perf_counter() - t0
+    res["coeffs"] = c
+    res["b"] = b
+    return res

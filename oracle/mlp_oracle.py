@@ -6,7 +6,7 @@ error L = sum (Y - T)^2 / (rows * out), its analytic gradient by
 back-propagation, and one Adam step (Kingma & Ba, "Adam: A Method for
 Stochastic Optimization", ICLR 2015, Algorithm 1) in the form the kernel
 evaluates it: m_hat = m * corr1, v_hat = v * corr2 with corr = 1 / (1 - beta^t)
-rounded to float32.  No reference counterpart: the reference has no learned
+formed in FP64 from the float beta and rounded to float32.  No reference counterpart: the reference has no learned
 component.  The analytic gradient is pinned by central finite differences
 in tests, the Adam step by hand-computed steps.
 
@@ -69,8 +69,9 @@ def adam_step(params, m, v, grad, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
     p = np.asarray(params, f).copy()
     g = np.asarray(grad, f)
     b1, b2, lr_, eps_ = f(beta1), f(beta2), f(lr), f(eps)
-    corr1 = f(1.0 / (1.0 - float(beta1) ** t))
-    corr2 = f(1.0 / (1.0 - float(beta2) ** t))
+    # the library receives beta as float and forms the corrections in FP64
+    corr1 = f(1.0 / (1.0 - float(np.float32(beta1)) ** t))
+    corr2 = f(1.0 / (1.0 - float(np.float32(beta2)) ** t))
     m = b1 * np.asarray(m, f) + (f(1) - b1) * g
     v = b2 * np.asarray(v, f) + (f(1) - b2) * (g * g)
     mhat = m * corr1
