@@ -10,14 +10,21 @@ complete solve from x = 0 with the matrix and vectors already resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 256]
 
 Prints one JSON line.  `value` is the mean time-to-solution (s) of the
-measured method: FP32-inner PCG with FP64 reliable updates and the Jacobi
-preconditioner, stopping on the FP64 true residual.  The plain FP64
-Jacobi-PCG (the reference's method, as the fused device loop) is timed on
-the same GPU beside it.  Kernel durations come from one more solve of each
-method run kernel by kernel with HIP events on the library's stream;
-`roofline` is the measured method's dominant kernel (the FP32 SpMV + <p,q>).
-`cpu_baseline` is the oracle's serial port of the reference's SerialCG on this
-host over a bounded number of iterations, scaled to the FP64 iteration count.
+measured method, mixed precision with the learned polynomial preconditioner:
+device matrix statistics and 10 FP64 Jacobi-PCG steps for the features, the
+network on the GPU for the coefficients, then the FP32-inner CG on the
+scaled operator with the degree-8 polynomial and FP64 reliable updates,
+stopping on the FP64 true relative residual (Jacobi fallback if the
+polynomial is refused or misses its SpMV budget); all of it inside the
+timed step.  Beside it, on the same GPU: the plain FP64 Jacobi-PCG (the
+reference's method, CG.cpp:284-385, as the fused device loop) in CRS and in
+ELL, mixed precision with Jacobi, and mixed precision with the polynomial
+from the fixed rule that the network replaces.  Kernel durations come from
+one more solve of each method run kernel by kernel with HIP events on the
+library's stream; `roofline` is the measured method's dominant kernel, the
+Clenshaw SpMV step on the FP32 scaled operator.  `cpu_baseline` is the
+oracle's serial port of the reference's SerialCG on this host over a bounded
+number of iterations, scaled to the FP64 iteration count.
 """
 from __future__ import annotations
 
@@ -38,18 +45,24 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(n: int, nnz: int, value_bytes: int, vec_bytes: int) -> int:
+def crs_bytes(n: int, nnz: int, value_bytes: int, vec_bytes: int) -> int:
     """Algorithmic bytes of one CRS SpMV: values, indices, row pointer, x, y."""
     return value_bytes * nnz + 4 * nnz + 4 * (n + 1) + 2 * vec_bytes * n
 
 
+def ell_bytes(n: int, nnz: int, value_bytes: int, vec_bytes: int) -> int:
+    """Algorithmic bytes of one ELL SpMV: live slots, lengths, x, y."""
+    return value_bytes * nnz + 4 * nnz + 4 * n + 2 * vec_bytes * n
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch of a kernel from the committed PMC summary."""
-    path = os.path.join(REPO, "profiles", f"traffic_{name}.json")
+    path = os.path.join(REPO, "profiles", "traffic_c4.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        k = json.load(f).get("kernels", {}).get(name)
+    return None if k is None else k.get("bytes_per_launch")
 
 
 def cpu_baseline(m, k, c, iterations: int, sample_iters: int) -> dict:
@@ -79,12 +92,10 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=256)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--degree", type=int, default=8, help="polynomial preconditioner degree")
-    ap.add_argument("--ritz-div", type=float, default=8.0,
-                    help="interval lower end = smallest 10-step Ritz value / this")
     ap.add_argument("--pmc", action="store_true",
                     help="short run for rocprofv3 --pmc: one D2D copy of a vector (the "
-                         "calibration kernel of known size), 20 FP64 and 20 mixed iterations")
+                         "calibration kernel of known size), 20 FP64 CRS, 20 FP64 ELL, 20 mixed "
+                         "Jacobi and 5 mixed polynomial iterations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -93,8 +104,8 @@ def main() -> int:
 
     import numpy as np
 
-    from cgxx_amd import problems, spectrum
-    from cgxx_amd.cghip import Context
+    from cgxx_amd import learned, problems
+    from cgxx_amd.cghip import Context, Mlp
 
     t = time.perf_counter()
     m = problems.diffusion3d(args.side, sigma=1.0, corr=8.0, seed=20261202)
@@ -105,110 +116,150 @@ def main() -> int:
 
     ctx = Context(0)
     ctx.upload_crs(m.ptr, m.index, m.value)
-    ctx.set_format("CRS")
+    ctx.upload_ell(*m.to_ell())
     ctx.upload_jacobi(c)
+    ctx.set_format("CRS")
     ctx.prepare_fp32()
     ctx.upload("K", k)
-    log("uploaded (FP64 CRS, FP32 copy of the values, Jacobi)")
+    nnz_nd = nnz - n  # the scaled operator stores no diagonal
+    log("uploaded FP64 CRS + ELL, Jacobi, FP32 copy and scaled FP32 operator")
+
+    model = learned.load_model()
+    mlp = Mlp(tuple(int(v) for v in model["dims"]), 1)
+    mlp.set_params(model["params"])
+    rho = float(model["rho_star"])
 
     max_iter = 100000
     if args.pmc:
         ctx.fill("P", 1.0)
         ctx.cpy("R", "P")  # reads and writes exactly 8*N bytes each
+        for fmt in ("CRS", "ELL"):
+            ctx.set_format(fmt)
+            ctx.fill("X", 0.0)
+            ctx.solve(precond="jacobi", max_iter=20, tol=1e-9, batch=1)
+        ctx.set_format("CRS")
         ctx.fill("X", 0.0)
-        info = ctx.solve(precond="jacobi", max_iter=20, tol=1e-9, batch=1)
+        ctx.solve_mixed(precond="jacobi", max_iter=20, tol=1e-9, batch=1)
+        f, b, _ = learned.system_features(ctx, n)
+        ctx.set_poly(learned.predict_coeffs(mlp, model, f, b), b)
         ctx.fill("X", 0.0)
-        info2 = ctx.solve_mixed(precond="jacobi", max_iter=20, tol=1e-9, batch=1)
-        log(f"pmc run: copy of {8 * n} bytes, {info['iterations']} FP64 and "
-            f"{info2['iterations']} mixed iterations")
+        ctx.solve_mixed(precond="poly", max_iter=5, tol=1e-9, batch=1)
+        log(f"pmc run: copy of {8 * n} bytes, 20+20 FP64, 20 mixed Jacobi, 5 mixed poly iterations")
+        mlp.close()
         ctx.close()
         return 0
-
-    def fp64(profile=0):
-        ctx.fill("X", 0.0)
-        return ctx.solve(precond="jacobi", max_iter=max_iter, tol=1e-9, profile=profile)
-
-    def mixed(profile=0):
-        ctx.fill("X", 0.0)
-        return ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=1e-9, profile=profile)
 
     def timed(fn, label):
         for i in range(args.warmup):
             info = fn()
-            log(f"{label} warmup {i}: {info['iterations']} iterations, {info['wall_s']:.4f} s")
+            log(f"{label} warmup {i}: {info['wall_s']:.4f} s")
         t0 = time.perf_counter()
         infos = [fn() for _ in range(args.steps)]
         sec = (time.perf_counter() - t0) / args.steps
-        log(f"{label}: {args.steps} timed solves, {sec:.4f} s each, {infos[-1]['iterations']} iterations")
+        log(f"{label}: {args.steps} timed solves, {sec:.4f} s each")
         return sec, infos[-1]
 
-    # Plain FP64 Jacobi-PCG (the reference's method) on the same GPU.
-    sec64, info64 = timed(fp64, "fp64 jacobi")
-    nr, nb = ctx.true_residual("X", "K")
-    x64 = ctx.download("X")
-    prof64 = fp64(profile=1)
-    spmv64_ms = prof64["ms_matvec"] / max(prof64["n_matvec"], 1)
-    spmv64_bytes = spmv_bytes(n, nnz, 8, 8)
-    spmv64_gbps = spmv64_bytes / (spmv64_ms * 1e-3) / 1e9
-    log(f"fp64: true residual {nr / nb:.3e}; spmv+dot {spmv64_ms:.4f} ms = {spmv64_gbps:.0f} GB/s, "
-        f"update {prof64['ms_update'] / prof64['n_update']:.4f} ms, "
-        f"p update {prof64['ms_xpay'] / prof64['n_xpay']:.4f} ms")
+    def per(info, key):
+        return info[f"ms_{key}"] / max(info[f"n_{key}"], 1)
 
-    # Measured method: FP32 inner PCG with FP64 reliable updates.
-    secmp, infomp = timed(mixed, "mixed")
-    nrm, nbm = ctx.true_residual("X", "K")
-    xmp = ctx.download("X")
-    profmp = mixed(profile=1)
-    spmv32_ms = profmp["ms_matvec"] / max(profmp["n_matvec"], 1)
-    spmv32_bytes = spmv_bytes(n, nnz, 4, 4)
-    spmv32_gbps = spmv32_bytes / (spmv32_ms * 1e-3) / 1e9
-    upd32_ms = profmp["ms_update"] / max(profmp["n_update"], 1)
-    xpay32_ms = profmp["ms_xpay"] / max(profmp["n_xpay"], 1)
-    refresh_ms = profmp["ms_refresh"] / max(profmp["n_refresh"], 1)
-    log(f"mixed: true residual {nrm / nbm:.3e} ({infomp['updates']} reliable updates); "
-        f"fp32 spmv+dot {spmv32_ms:.4f} ms = {spmv32_gbps:.0f} GB/s, update {upd32_ms:.4f} ms, "
-        f"p update {xpay32_ms:.4f} ms, reliable update {refresh_ms:.4f} ms")
-    err = float(np.abs(xmp - 1.0).max())
-    diff = float(np.abs(xmp - x64).max())
+    # ---- plain FP64 Jacobi-PCG (the reference's method), CRS and ELL ----
+    fp64 = {}
+    xs = {}
+    for fmt in ("CRS", "ELL"):
+        ctx.set_format(fmt)
 
-    # Mixed precision with a degree-8 Chebyshev polynomial preconditioner on
-    # [a, b]: b = Gershgorin bound of D^-1 A, a from the smallest Ritz value
-    # of 10 FP64 Jacobi-PCG steps (placed below it: the 10-step estimate
-    # lies above lambda_min).
-    ctx.fill("X", 0.0)
-    ctx.solve(precond="jacobi", max_iter=10, tol=1e-300)
-    theta = spectrum.ritz_values(ctx.history("alpha", 10), ctx.history("beta", 9))
-    bnd = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
-    poly_a = theta[0] / args.ritz_div
-    coeffs = spectrum.cheb_residual_coeffs(poly_a, bnd, args.degree)
-    ctx.set_poly(coeffs, bnd)
+        def run64(profile=0):
+            ctx.fill("X", 0.0)
+            return ctx.solve(precond="jacobi", max_iter=max_iter, tol=1e-9, profile=profile)
 
-    def mixed_poly(profile=0):
+        sec, info = timed(run64, f"fp64 {fmt}")
+        nr, nb = ctx.true_residual("X", "K")
+        xs[fmt] = ctx.download("X")
+        prof = run64(profile=1)
+        spmv_ms = per(prof, "matvec")
+        by = crs_bytes(n, nnz, 8, 8) if fmt == "CRS" else ell_bytes(n, nnz, 8, 8)
+        plain_ms = ctx.time_matvec("P", "Q", reps=20)
+        fp64[fmt] = {
+            "seconds": sec, "iterations": info["iterations"], "true_relative_residual": nr / nb,
+            "spmv_dot_ms": spmv_ms, "spmv_dot_gbps": by / (spmv_ms * 1e-3) / 1e9,
+            "spmv_plain_ms": plain_ms, "spmv_plain_gbps": by / (plain_ms * 1e-3) / 1e9,
+            "spmv_plain_hbm_frac": by / (plain_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "update_ms": per(prof, "update"), "xpay_ms": per(prof, "xpay"),
+            "algorithmic_spmv_bytes": by,
+        }
+        log(f"fp64 {fmt}: {info['iterations']} iterations, true residual {nr / nb:.3e}, "
+            f"spmv+dot {spmv_ms:.4f} ms = {fp64[fmt]['spmv_dot_gbps']:.0f} GB/s, plain spmv "
+            f"{plain_ms:.4f} ms = {fp64[fmt]['spmv_plain_gbps']:.0f} GB/s")
+    base_fmt = min(fp64, key=lambda f: fp64[f]["seconds"])
+    sec64 = fp64[base_fmt]["seconds"]
+    x64 = xs[base_fmt]
+    ctx.set_format("CRS")
+
+    # ---- mixed precision, Jacobi (scaled operator) ----
+    def mixed(profile=0):
         ctx.fill("X", 0.0)
-        return ctx.solve_mixed(precond="poly", max_iter=max_iter, tol=1e-9, profile=profile)
+        return ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=1e-9, profile=profile)
 
-    secpl, infopl = timed(mixed_poly, "mixed+poly")
-    nrp, nbp = ctx.true_residual("X", "K")
-    xpl = ctx.download("X")
-    profpl = mixed_poly(profile=1)
-    log(f"mixed+poly: degree {args.degree} on [{poly_a:.3e}, {bnd:.3f}] (Ritz {theta[0]:.3e}.."
-        f"{theta[-1]:.3f}): {infopl['iterations']} iterations ({infopl['iterations'] * (args.degree + 1)} "
-        f"FP32 SpMVs), true residual {nrp / nbp:.3e}, {secpl:.4f} s")
-    plain64_ms = ctx.time_matvec("P", "Q", reps=20)
+    secmj, infomj = timed(mixed, "mixed jacobi")
+    profmj = mixed(profile=1)
+    spmv_nd_bytes = 8 * nnz_nd + 4 * n + 8 * n  # FP32 ELL scaled operator, x and y
+    mixed_j = {
+        "seconds": secmj, "iterations": infomj["iterations"],
+        "reliable_updates": infomj["updates"], "true_relative_residual": infomj["true_residual"],
+        "spmv_dot_ms": per(profmj, "matvec"),
+        "spmv_dot_gbps": spmv_nd_bytes / (per(profmj, "matvec") * 1e-3) / 1e9,
+        "update_ms": per(profmj, "update"), "xpay_ms": per(profmj, "xpay"),
+        "reliable_update_ms": per(profmj, "refresh"),
+        "speedup_vs_fp64": sec64 / secmj,
+    }
+    log(f"mixed jacobi: {infomj['iterations']} iterations, {secmj:.4f} s; spmv+dot "
+        f"{mixed_j['spmv_dot_ms']:.4f} ms, update {mixed_j['update_ms']:.4f} ms, p update "
+        f"{mixed_j['xpay_ms']:.4f} ms")
+
+    # ---- mixed precision, learned polynomial (the measured method) ----
+    def learned_solve():
+        return learned.solve_learned(ctx, mlp, model, n, tol=1e-9, max_iter=max_iter)
+
+    secl, infol = timed(learned_solve, "mixed+learned")
+    nrl, nbl = ctx.true_residual("X", "K")
+    xl = ctx.download("X")
+    ctx.set_poly(infol["coeffs"], infol["b"])
+    ctx.fill("X", 0.0)
+    profl = ctx.solve_mixed(precond="poly", max_iter=max_iter, tol=1e-9, profile=1)
+    deg = learned.DEGREE
+    step_ms = per(profl, "poly")
+    # Clenshaw step j: the operator stream, B_{j+1} gathered, r read, B_j
+    # written, B_{j+2} read in all but the first step of the recurrence
+    step_bytes = 8 * nnz_nd + 4 * n + 4 * n + 4 * n + 4 * n + (deg - 1) / deg * 4 * n
+    step_gbps = step_bytes / (step_ms * 1e-3) / 1e9
+    log(f"mixed+learned: method {infol['method']}, {infol['poly_iterations']} poly + "
+        f"{infol['jacobi_iterations']} Jacobi iterations ({infol['spmvs']} FP32 SpMVs), true residual "
+        f"{nrl / nbl:.3e}, {secl:.4f} s; Clenshaw step {step_ms:.4f} ms = {step_gbps:.0f} GB/s")
+
+    # ---- the method without the network: Chebyshev on [theta10 / rho*, b] ----
+    def fixed_solve():
+        t0 = time.perf_counter()
+        f, b, theta10 = learned.system_features(ctx, n)
+        r = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b,
+                                             tol=1e-9, max_iter=max_iter)
+        r["wall_s"] = time.perf_counter() - t0
+        return r
+
+    secf, infof = timed(fixed_solve, "mixed+fixed-rule poly")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(m, k, c, info64["iterations"], args.cpu_iters)
+        cpu = cpu_baseline(m, k, c, fp64[base_fmt]["iterations"], args.cpu_iters)
         log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['sample']})")
 
     out = {
         "metric": METRIC,
-        "value": secmp,
+        "value": secl,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": secmp * 1e3,
+        "ms_per_step": secl * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -218,67 +269,44 @@ def main() -> int:
             "workload": f"3D 7-point variable-coefficient diffusion {args.side}^3 "
                         "(log-normal kappa, sigma=1, corr=8, seed 20261202), k=A*1, x0=0, "
                         "FP64 true relative residual 1e-9",
-            "method": "FP32-inner Jacobi-PCG with FP64 reliable updates (delta 0.1), fused "
-                      "device loop; learned polynomial preconditioner not in this measurement",
+            "method": "mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 (ELL, FP32 values, int32 "
+                      "indices) with a degree-8 polynomial preconditioner whose coefficients come "
+                      "from the trained network, FP64 reliable updates (delta 0.1); features, "
+                      "inference and fallback inside the timed step",
             "n": n,
             "nnz": nnz,
-            "format": "CRS",
+            "format": "CRS upload, scaled FP32 operator in ELL",
         },
-        "iterations": infomp["iterations"],
-        "reliable_updates": infomp["updates"],
-        "true_relative_residual": nrm / nbm,
-        "max_abs_error_vs_ones": err,
-        "max_abs_diff_vs_fp64_solution": diff,
-        "fp64_jacobi": {
-            "seconds": sec64,
-            "iterations": info64["iterations"],
-            "true_relative_residual": nr / nb,
-            "spmv_dot_ms": spmv64_ms,
-            "spmv_gbps": spmv64_gbps,
-            "spmv_hbm_frac": spmv64_gbps / HBM_PEAK_GBS,
-            "spmv_plain_ms": plain64_ms,
-            "spmv_plain_gbps": spmv64_bytes / (plain64_ms * 1e-3) / 1e9,
-            "update_ms": prof64["ms_update"] / max(prof64["n_update"], 1),
-            "xpay_ms": prof64["ms_xpay"] / max(prof64["n_xpay"], 1),
-            "traffic_spmv_dot": traffic_from_profiles("spmv_dot_crs_fp64_256"),
-        },
-        "mixed_poly": {
-            "seconds": secpl,
-            "degree": args.degree,
-            "interval": [poly_a, bnd],
-            "ritz_10": [float(theta[0]), float(theta[-1])],
-            "iterations": infopl["iterations"],
-            "fp32_spmvs": infopl["iterations"] * (args.degree + 1),
-            "reliable_updates": infopl["updates"],
-            "true_relative_residual": nrp / nbp,
-            "max_abs_error_vs_ones": float(np.abs(xpl - 1.0).max()),
-            "max_abs_diff_vs_fp64_solution": float(np.abs(xpl - x64).max()),
-            "spmv_dot_ms": profpl["ms_matvec"] / max(profpl["n_matvec"], 1),
-            "update_ms": profpl["ms_update"] / max(profpl["n_update"], 1),
-            "clenshaw_and_xpay_ms": profpl["ms_xpay"] / max(profpl["n_xpay"], 1),
-            "speedup_vs_fp64_jacobi": sec64 / secpl,
-        },
-        "speedup_vs_fp64_jacobi": sec64 / secmp,
-        "spmv_gbps": spmv32_gbps,
-        "kernel_ms": {"spmv_dot_fp32": spmv32_ms, "update_fp32": upd32_ms, "xpay_fp32": xpay32_ms,
-                      "reliable_update": refresh_ms},
-        "kernel_gbps": {
-            "spmv_dot_fp32": spmv32_gbps,
-            "update_fp32": 28 * n / (upd32_ms * 1e-3) / 1e9,
-            "xpay_fp32": 16 * n / (xpay32_ms * 1e-3) / 1e9,
-        },
+        "iterations": infol["poly_iterations"] + infol["jacobi_iterations"],
+        "fp32_spmvs": infol["spmvs"],
+        "fallback_method": infol["method"],
+        "reliable_updates": infol["updates"],
+        "true_relative_residual": nrl / nbl,
+        "max_abs_error_vs_ones": float(np.abs(xl - 1.0).max()),
+        "max_abs_diff_vs_fp64_solution": float(np.abs(xl - x64).max()),
+        "speedup_vs_fp64_jacobi": sec64 / secl,
+        "fp64_baseline_format": base_fmt,
+        "fp64_jacobi": fp64,
+        "mixed_jacobi": mixed_j,
+        "mixed_fixed_rule_poly": {"seconds": secf, "rho": rho, "fp32_spmvs": infof["spmvs"],
+                                  "true_relative_residual": infof["true_residual"],
+                                  "speedup_vs_fp64": sec64 / secf},
+        "kernel_ms": {"clenshaw_step": step_ms, "spmv_dot_fp32": per(profl, "matvec"),
+                      "update_fp32": per(profl, "update"), "xpay_fp32": per(profl, "xpay"),
+                      "reliable_update": per(profl, "refresh")},
         "roofline": {
             "bound": "hbm",
-            "achieved": spmv32_gbps,
+            "achieved": step_gbps,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": spmv32_gbps / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("spmv_dot_crs_fp32_256"),
-            "kernel": "k_mp_matvec_dot_crs",
-            "algorithmic_bytes_per_launch": spmv32_bytes,
+            "frac": step_gbps / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("k_poly_step<false, true>"),
+            "kernel": "k_poly_step<false, true> (Clenshaw step, FP32 ELL scaled operator)",
+            "algorithmic_bytes_per_launch": step_bytes,
         },
         "cpu_baseline": cpu,
     }
+    mlp.close()
     ctx.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -66,8 +66,9 @@ class MixedInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("updates", C.c_int), ("true_residual", C.c_double),
                 ("inner_residual", C.c_double), ("wall_s", C.c_double),
                 ("ms_matvec", C.c_double), ("ms_update", C.c_double), ("ms_xpay", C.c_double),
-                ("ms_refresh", C.c_double), ("n_matvec", C.c_int), ("n_update", C.c_int),
-                ("n_xpay", C.c_int), ("n_refresh", C.c_int)]
+                ("ms_refresh", C.c_double), ("ms_poly", C.c_double), ("n_matvec", C.c_int),
+                ("n_update", C.c_int), ("n_xpay", C.c_int), ("n_refresh", C.c_int),
+                ("n_poly", C.c_int), ("restarts", C.c_int)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -91,7 +92,7 @@ def load() -> C.CDLL:
     L.cghip_destroy.argtypes = [vp]
     L.cghip_last_error.argtypes = [vp]
     L.cghip_last_error.restype = C.c_char_p
-    L.cghip_geometry.argtypes = [C.POINTER(C.c_int)] * 5
+    L.cghip_geometry.argtypes = [C.POINTER(C.c_int)] * 6
     L.cghip_set_size.argtypes = [vp, C.c_int]
     L.cghip_upload_crs.argtypes = [vp, C.c_int, C.c_int, _i32p, _i32p, _f64p]
     L.cghip_upload_ell.argtypes = [vp, C.c_int, C.c_int, _i32p, _i32p, _f64p]
@@ -131,9 +132,10 @@ def load() -> C.CDLL:
 
 
 def geometry() -> dict:
-    vals = [C.c_int() for _ in range(5)]
+    vals = [C.c_int() for _ in range(6)]
     load().cghip_geometry(*[C.byref(v) for v in vals])
-    return dict(zip(["block", "vec_gmax", "rows_max", "nnz_cap", "spmv_gmax"], [v.value for v in vals]))
+    return dict(zip(["block", "vec_gmax", "rows_max", "nnz_cap", "spmv_gmax", "ell_gmax"],
+                    [v.value for v in vals]))
 
 
 class Context:

@@ -49,7 +49,7 @@ struct cghip_ctx {
   int dot_mode = CGHIP_DOT_TREE;
 
   // reduction scratch
-  // 4 x kVecGMax: <p,q> | <r,r> | <r,z> | ||k - A x||^2 (kSpmvGMax <= kVecGMax)
+  // 4 x kPartialSlot: <p,q> | <r,r> | <r,z> | ||k - A x||^2
   double *partial = nullptr;
   double *dot_out = nullptr;
 
@@ -65,10 +65,22 @@ struct cghip_ctx {
   hipGraphExec_t graph = nullptr;
   int graph_batch = 0, graph_format = 0, graph_precond = -1;
 
-  // mixed precision: FP32 copies of the CRS values and the Jacobi diagonal,
-  // FP32 work vectors x32, r32, p32, q32, the inner loop's state and graph
+  // mixed precision: the FP32 copy of the CRS values; the scaled operator
+  // D^-1/2 A D^-1/2 with implicit unit diagonal (off-diagonal CRS + its row
+  // blocks, s = 1/sqrt(diag) in FP64); FP32 work vectors y32, r32, p32, q32;
+  // the inner loop's state and graph
   bool has_fp32 = false;
-  float *value32 = nullptr, *jac32 = nullptr;
+  float *value32 = nullptr;
+  bool has_scaled = false;
+  int nnz_nd = 0, nrb_nd = 0;
+  int *ptr_nd = nullptr, *index_nd = nullptr, *rb_nd = nullptr;
+  float *value_nd = nullptr;
+  double *scale64 = nullptr;
+  // the scaled operator in column-major ELL when its padding is small
+  bool ell_nd = false;
+  int ell_w_nd = 0;
+  int *ell_len_nd = nullptr, *ell_idx_nd = nullptr;
+  float *ell_val_nd = nullptr;
   float *v32[4] = {};
   MixedState *d_mstate = nullptr;
   MixedState *h_mstate = nullptr;  // pinned
@@ -142,15 +154,21 @@ void drop_graph(cghip_ctx *c) {
 }
 
 void free_fp32(cghip_ctx *c) {
-  if (c->value32) (void)hipFree(c->value32);
-  if (c->jac32) (void)hipFree(c->jac32);
-  for (auto &v : c->v32)
-    if (v) (void)hipFree(v);
-  for (auto &v : c->w32)
-    if (v) (void)hipFree(v);
-  c->value32 = c->jac32 = nullptr;
-  for (auto &v : c->v32) v = nullptr;
-  for (auto &v : c->w32) v = nullptr;
+  dfree(c->value32);
+  dfree(c->ptr_nd);
+  dfree(c->index_nd);
+  dfree(c->rb_nd);
+  dfree(c->value_nd);
+  dfree(c->scale64);
+  dfree(c->ell_len_nd);
+  dfree(c->ell_idx_nd);
+  dfree(c->ell_val_nd);
+  c->ell_nd = false;
+  c->ell_w_nd = 0;
+  for (auto &v : c->v32) dfree(v);
+  for (auto &v : c->w32) dfree(v);
+  c->has_scaled = false;
+  c->nnz_nd = c->nrb_nd = 0;
   c->has_fp32 = false;
   c->has_poly = false;
   c->poly_degree = 0;
@@ -177,12 +195,12 @@ bool valid_vec(int id) { return id >= CGHIP_VEC_K && id <= CGHIP_VEC_Z; }
 // true residual of a reliable update.
 Partials pq_of(cghip_ctx *c) { return Partials{c->partial, nullptr}; }
 Partials pr_of(cghip_ctx *c) {
-  return Partials{c->partial + kVecGMax, c->partial + 2 * (size_t)kVecGMax};
+  return Partials{c->partial + kPartialSlot, c->partial + 2 * (size_t)kPartialSlot};
 }
-Partials pt_of(cghip_ctx *c) { return Partials{c->partial + 3 * (size_t)kVecGMax, nullptr}; }
+Partials pt_of(cghip_ctx *c) { return Partials{c->partial + 3 * (size_t)kPartialSlot, nullptr}; }
 
 int spmv_blocks(cghip_ctx *c) {
-  return spmv_grid(c->format == CGHIP_FORMAT_CRS ? c->crs_nrb : (c->n + kBlock - 1) / kBlock);
+  return c->format == CGHIP_FORMAT_CRS ? spmv_grid(c->crs_nrb) : ell64_grid(c->n);
 }
 
 Crs<double> crs_of(cghip_ctx *c) {
@@ -219,12 +237,14 @@ int sync(cghip_ctx *c) {
 
 extern "C" {
 
-int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap, int *spmv_gmax) {
+int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap, int *spmv_gmax,
+                   int *ell_gmax) {
   if (block) *block = kBlock;
   if (vec_gmax) *vec_gmax = kVecGMax;
   if (rows_max) *rows_max = kRowsMax;
   if (nnz_cap) *nnz_cap = kNnzCap;
   if (spmv_gmax) *spmv_gmax = kSpmvGMax;
+  if (ell_gmax) *ell_gmax = kEllGMax;
   return CGHIP_OK;
 }
 
@@ -248,7 +268,7 @@ int cghip_create(int device, cghip_ctx **out) {
   int rc = CGHIP_OK;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     rc = fail(nullptr, CGHIP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
-  if (!rc) rc = dalloc(c, &c->partial, 4 * (size_t)kVecGMax);
+  if (!rc) rc = dalloc(c, &c->partial, 4 * (size_t)kPartialSlot);
   if (!rc) rc = dalloc(c, &c->d_poly, 1);
   if (!rc) rc = dalloc(c, &c->dot_out, 8);
   if (!rc) rc = dalloc(c, &c->d_state, 1);
@@ -685,10 +705,11 @@ int cghip_prepare_fp32(cghip_ctx *c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drop_graph(c);
   free_fp32(c);
+  const int n = c->n;
   int rc = dalloc(c, &c->value32, (size_t)c->crs_nnz);
   for (auto &v : c->v32)
-    if (!rc) rc = dalloc(c, &v, (size_t)c->n);
-  if (!rc && c->has_jac) rc = dalloc(c, &c->jac32, (size_t)c->n);
+    if (!rc) rc = dalloc(c, &v, (size_t)n);
+  if (!rc) rc = dalloc(c, &c->scale64, (size_t)n);
   if (!rc && !c->d_mstate) rc = dalloc(c, &c->d_mstate, 1);
   if (!rc && !c->h_mstate) {
     hipError_t e = hipHostMalloc((void **)&c->h_mstate, sizeof(MixedState), 0);
@@ -699,9 +720,54 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     return rc;
   }
   HIPCHK(c, convert_f64_f32(c->crs_nnz, c->crs_value, c->value32, c->stream));
-  if (c->has_jac) HIPCHK(c, convert_f64_f32(c->n, c->jac, c->jac32, c->stream));
-  for (auto &v : c->v32) HIPCHK(c, hipMemsetAsync(v, 0, sizeof(float) * (size_t)c->n, c->stream));
+  for (auto &v : c->v32) HIPCHK(c, hipMemsetAsync(v, 0, sizeof(float) * (size_t)n, c->stream));
+  // Scaled operator: needs exactly one positive diagonal entry per row.
+  int *d_bad = reinterpret_cast<int *>(c->dot_out);
+  HIPCHK(c, hipMemsetAsync(d_bad, 0, sizeof(int), c->stream));
+  HIPCHK(c, scaled_diag(crs_of(c), c->scale64, d_bad, c->stream));
+  int bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if (int r = sync(c)) return r;
+  if (!bad) {
+    // row i keeps ptr[i+1] - ptr[i] - 1 entries: the off-diagonal row
+    // pointer is ptr[i] - i
+    std::vector<int> ptr(n + 1);
+    HIPCHK(c, hipMemcpy(ptr.data(), c->crs_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    for (int i = 0; i <= n; i++) ptr[i] -= i;
+    c->nnz_nd = ptr[n];
+    std::vector<int> starts(n + 1);
+    c->nrb_nd = crs_rowblocks(n, ptr.data(), starts.data());
+    rc = dalloc(c, &c->ptr_nd, (size_t)n + 1);
+    if (!rc) rc = dalloc(c, &c->index_nd, (size_t)c->nnz_nd);
+    if (!rc) rc = dalloc(c, &c->value_nd, (size_t)c->nnz_nd);
+    if (!rc) rc = dalloc(c, &c->rb_nd, (size_t)c->nrb_nd + 1);
+    if (rc) {
+      free_fp32(c);
+      return rc;
+    }
+    HIPCHK(c, hipMemcpy(c->ptr_nd, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->rb_nd, starts.data(), sizeof(int) * (c->nrb_nd + 1),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, scaled_copy(crs_of(c), c->scale64, c->index_nd, c->value_nd, c->stream));
+    int w = 0;
+    for (int i = 0; i < n; i++) w = std::max(w, ptr[i + 1] - ptr[i]);
+    if (ell_pays(w, n, c->nnz_nd)) {
+      const size_t slots = (size_t)w * (size_t)n;
+      rc = dalloc(c, &c->ell_len_nd, (size_t)n);
+      if (!rc) rc = dalloc(c, &c->ell_idx_nd, slots);
+      if (!rc) rc = dalloc(c, &c->ell_val_nd, slots);
+      if (rc) {
+        free_fp32(c);
+        return rc;
+      }
+      HIPCHK(c, scaled_ell(crs_of(c), c->scale64, c->ell_len_nd, c->ell_idx_nd, c->ell_val_nd,
+                           c->stream));
+      c->ell_nd = true;
+      c->ell_w_nd = w;
+    }
+    if (int r = sync(c)) return r;
+    c->has_scaled = true;
+  }
   c->has_fp32 = true;
   return CGHIP_OK;
 }
@@ -709,8 +775,9 @@ int cghip_prepare_fp32(cghip_ctx *c) {
 int cghip_set_poly(cghip_ctx *c, int degree, const double *coeffs, double b) {
   if (!c) return CGHIP_ERR_ARG;
   if (!c->has_fp32) return fail(c, CGHIP_ERR_STATE, "call cghip_prepare_fp32 first");
-  if (!c->jac32)
-    return fail(c, CGHIP_ERR_STATE, "the polynomial preconditioner needs the Jacobi diagonal");
+  if (!c->has_scaled)
+    return fail(c, CGHIP_ERR_STATE,
+                "the polynomial preconditioner needs one positive diagonal entry per row");
   if (degree < 1 || degree > kPolyMaxDegree)
     return fail(c, CGHIP_ERR_ARG, "degree %d outside [1, %d]", degree, kPolyMaxDegree);
   if (!coeffs) return fail(c, CGHIP_ERR_ARG, "coeffs is NULL");
@@ -782,8 +849,8 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   if (o->precond != CGHIP_PRECOND_NONE && o->precond != CGHIP_PRECOND_JACOBI &&
       o->precond != CGHIP_PRECOND_POLY)
     return fail(c, CGHIP_ERR_ARG, "unsupported preconditioner %d", o->precond);
-  if (o->precond == CGHIP_PRECOND_JACOBI && !c->jac32)
-    return fail(c, CGHIP_ERR_STATE, "Jacobi requested but no diagonal uploaded");
+  if (o->precond == CGHIP_PRECOND_JACOBI && !c->has_scaled)
+    return fail(c, CGHIP_ERR_STATE, "Jacobi needs one positive diagonal entry per row");
   if (o->precond == CGHIP_PRECOND_POLY && !c->has_poly)
     return fail(c, CGHIP_ERR_STATE, "polynomial requested: call cghip_set_poly first");
   if (o->profile != CGHIP_PROFILE_NONE && o->profile != CGHIP_PROFILE_KERNELS)
@@ -796,17 +863,29 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   const int n = c->n;
   const bool poly = o->precond == CGHIP_PRECOND_POLY;
   const int degree = poly ? c->poly_degree : 0;
-  const float *C32 = o->precond != CGHIP_PRECOND_NONE ? c->jac32 : nullptr;
+  // Jacobi and the polynomial run on the scaled operator (unit diagonal
+  // implicit); no preconditioner runs on the FP32 copy of A.
+  const bool scaled = o->precond != CGHIP_PRECOND_NONE;
+  const double *scale = scaled ? c->scale64 : nullptr;
   float *const w[3] = {c->w32[0], c->w32[1], c->w32[2]};
-  float *x32 = c->v32[0], *r32 = c->v32[1], *p32 = c->v32[2], *q32 = c->v32[3];
+  float *y32 = c->v32[0], *r32 = c->v32[1], *p32 = c->v32[2], *q32 = c->v32[3];
   double *x64 = c->vec[CGHIP_VEC_X], *k64 = c->vec[CGHIP_VEC_K], *r64 = c->vec[CGHIP_VEC_R],
          *q64 = c->vec[CGHIP_VEC_Q];
-  const Crs<float> A32{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb};
-  // slots: <p,q> (SpMV grid), <r,r> and <r,z> (quads grid; <r,z> on the
-  // SpMV grid with the polynomial), ||k - A x||^2 (pairs grid)
+  Op32 A32{};
+  if (scaled) {
+    A32.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, n, c->nrb_nd, 1};
+    A32.ell = Ell<float>{c->ell_len_nd, c->ell_idx_nd, c->ell_val_nd, n, c->ell_w_nd, 1};
+    A32.is_ell = c->ell_nd ? 1 : 0;
+  } else {
+    A32.crs = Crs<float>{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb, 0};
+  }
+  const int gpq = op32_grid(A32);
+  // slots: <p,q> (SpMV grid), <r,r> (quads grid), <r,z> (SpMV grid, the
+  // polynomial only: otherwise z = r), ||k - A x||^2 (pairs grid)
   const Partials pa = pq_of(c), pb = pr_of(c), pt = pt_of(c);
   const Partials prr{pb.a, nullptr}, prz{pb.b, nullptr};
-  const int gz = poly ? spmv_grid(c->crs_nrb) : vec_grid4(n);
+  const double *rz_part = poly ? prz.a : prr.a;
+  const int gz = poly ? gpq : vec_grid4(n);
   const double delta = o->delta > 0 ? o->delta : 0.1;
 
   MixedState init{};
@@ -820,7 +899,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
 
   const bool profile = o->profile == CGHIP_PROFILE_KERNELS;
   const int batch = profile ? 1 : (o->batch > 0 ? o->batch : (n >= (1 << 20) ? 8 : 32));
-  if (profile && ensure_events(c, 6)) return CGHIP_ERR_HIP;
+  if (profile && ensure_events(c, 7)) return CGHIP_ERR_HIP;
   hipEvent_t *ev = profile ? c->events.data() : nullptr;
 
   auto read_state = [&]() -> int {
@@ -828,21 +907,21 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
                              c->stream));
     return sync(c);
   };
-  // r = k - A x64 in FP64 (and x64 += x32 first unless this is the start);
-  // r32 restarts from it.
+  // x64 += s o y32 (unless this is the start), r = k - A x64 in FP64, and
+  // the inner iteration restarts from r32 = (float)(s o r).
   auto refresh = [&](bool first) -> int {
     if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    if (!first) HIPCHK(c, mp_accumulate(n, x64, x32, c->stream));
+    if (!first) HIPCHK(c, mp_accumulate(n, x64, y32, scale, c->stream));
     HIPCHK(c, matvec_crs(crs_of(c), x64, q64, c->stream));
     HIPCHK(c, mp_residual(n, k64, q64, r64, pt, c->stream));
     if (poly) {
-      HIPCHK(c, mp_restart_poly(n, r64, r32, x32, C32, c->d_poly, w[degree % 3], prr, c->stream));
-      HIPCHK(c, mp_poly_apply(A32, r32, C32, c->d_poly, degree, w, first ? p32 : nullptr, prz,
-                              nullptr, c->stream));
+      HIPCHK(c, mp_restart_poly(n, r64, scale, r32, y32, c->d_poly, w[degree % 3], prr, c->stream));
+      HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, first ? p32 : nullptr, prz, nullptr,
+                              c->stream));
     } else {
-      HIPCHK(c, mp_restart(n, r64, r32, x32, p32, C32, first, pb, c->stream));
+      HIPCHK(c, mp_restart(n, r64, scale, r32, y32, p32, first, prr, c->stream));
     }
-    HIPCHK(c, mp_restart_fold(n, pt, prr, prz.a, gz, first, c->d_mstate, h, c->stream));
+    HIPCHK(c, mp_restart_fold(n, pt, prr, rz_part, gz, first, c->d_mstate, h, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     if (int rc = read_state()) return rc;
     if (ev) {
@@ -853,23 +932,23 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   };
   auto enqueue = [&](hipEvent_t *e) -> int {
     if (e) HIPCHK(c, hipEventRecord(e[0], c->stream));
-    HIPCHK(c, mp_matvec_dot_crs(A32, p32, q32, pa, c->d_mstate, c->stream));
+    HIPCHK(c, mp_matvec_dot(A32, p32, q32, pa, c->d_mstate, c->stream));
     if (e) HIPCHK(c, hipEventRecord(e[1], c->stream));
     if (poly) {
-      // the p update times the Clenshaw steps and the p update together
-      HIPCHK(c, mp_update_poly(n, spmv_grid(c->crs_nrb), x32, r32, p32, q32, C32, c->d_poly,
-                               w[degree % 3], pa, prr, c->d_mstate, h, c->stream));
+      HIPCHK(c, mp_update_poly(n, gpq, y32, r32, p32, q32, c->d_poly, w[degree % 3], pa, prr,
+                               c->d_mstate, h, c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
-      HIPCHK(c, mp_poly_apply(A32, r32, C32, c->d_poly, degree, w, nullptr, prz, c->d_mstate,
+      HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, nullptr, prz, c->d_mstate,
                               c->stream));
+      if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
       HIPCHK(c, mp_xpay_poly(n, p32, w[0], prr, prz, gz, c->d_mstate, h, c->stream));
     } else {
-      HIPCHK(c, mp_update(n, spmv_grid(c->crs_nrb), x32, r32, p32, q32, C32, pa, pb, c->d_mstate,
-                          h, c->stream));
+      HIPCHK(c, mp_update(n, gpq, y32, r32, p32, q32, pa, prr, c->d_mstate, h, c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
-      HIPCHK(c, mp_xpay(n, p32, r32, C32, pb, c->d_mstate, h, c->stream));
+      if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
+      HIPCHK(c, mp_xpay(n, p32, r32, prr, c->d_mstate, h, c->stream));
     }
-    if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
+    if (e) HIPCHK(c, hipEventRecord(e[4], c->stream));
     return CGHIP_OK;
   };
 
@@ -909,12 +988,21 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
       return fail(c, CGHIP_ERR_STATE, "mixed solver did not stop after %ld passes", pass);
     const MixedState &s = *c->h_mstate;
     if (s.next.done == 1) break;  // true residual <= tol
-    if (s.next.done == 2) {       // reliable update requested by the inner loop
+    if (s.next.done == 2) {  // reliable update requested by the inner loop
+      const double prev = s.true_res;
       if (int rc = refresh(false)) return rc;
+      // A phase that cut the true residual by more than 1 / kResetRatio
+      // (e.g. one step down to FP32 resolution) leaves a search direction
+      // with no accurate information about the new residual: p = z.
+      if (c->h_mstate->true_res < kResetRatio * prev) {
+        HIPCHK(c, hipMemcpyAsync(p32, poly ? w[0] : r32, sizeof(float) * (size_t)n,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        info->restarts++;
+      }
       continue;
     }
     if (s.cur.done == 1 || s.next.iteration >= o->max_iter) {
-      if (int rc = refresh(false)) return rc;  // fold x32 into x64, final residual
+      if (int rc = refresh(false)) return rc;  // fold y32 into x64, final residual
       break;
     }
     if (profile) {
@@ -922,10 +1010,14 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
       if (int rc = read_state()) return rc;
       info->ms_matvec += elapsed(ev[2], ev[3]);
       info->ms_update += elapsed(ev[3], ev[4]);
-      info->ms_xpay += elapsed(ev[4], ev[5]);
+      info->ms_xpay += elapsed(ev[5], ev[6]);
       info->n_matvec++;
       info->n_update++;
       info->n_xpay++;
+      if (poly) {
+        info->ms_poly += elapsed(ev[4], ev[5]);
+        info->n_poly += degree;
+      }
     } else {
       HIPCHK(c, hipGraphLaunch(c->graph_mp, c->stream));
       if (int rc = read_state()) return rc;

@@ -16,7 +16,9 @@ constexpr int kBlock = 256;      // threads per block (4 wave64s)
 constexpr int kVecGMax = 2048;   // max blocks of grid-stride vector kernels
 constexpr int kRowsMax = 256;    // max rows in one CRS row block (one per thread)
 constexpr int kNnzCap = 2048;    // max nonzeros staged in LDS per row block
-constexpr int kSpmvGMax = 2048;  // max blocks of the SpMV kernels
+constexpr int kSpmvGMax = 2048;  // max blocks of the CRS SpMV and FP32 ELL kernels
+constexpr int kEllGMax = 8192;   // max blocks of the FP64 ELL SpMV kernels
+constexpr int kPartialSlot = 8192;  // entries per partial-sum slot (>= every grid above)
 
 // Vector kernels work on element pairs: G = ceil(pairs / 256), at most 2048.
 inline int vec_grid(int n) {
@@ -44,6 +46,17 @@ inline int spmv_grid(int nrb) {
   return (int)g;
 }
 
+// ELL kernels: 256-row chunks, one row per thread; FP64 with up to 8192
+// blocks (more waves in flight for the 12-byte slots), FP32 up to 2048.
+__host__ __device__ inline int ell_chunks(int n) { return (int)(((int64_t)n + kBlock - 1) / kBlock); }
+inline int ell64_grid(int n) {
+  int64_t g = ell_chunks(n);
+  if (g > kEllGMax) g = kEllGMax;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+inline int ell32_grid(int n) { return spmv_grid(ell_chunks(n)); }
+
 // Scalars of the fused PCG loop (CG.cpp:288-291 plus the loop counter and
 // the stop flag).  No kernel reads a field that the same kernel writes:
 // the p-update kernel (last of an iteration) writes `next`, the SpMV kernel
@@ -70,7 +83,7 @@ struct History {
 // kernel-boundary coherence point) folds them in every block, in the fixed
 // order, so no cross-block handoff ever happens inside a kernel.
 struct Partials {
-  double *a, *b;  // kVecGMax entries each
+  double *a, *b;  // kPartialSlot entries each
 };
 
 template <typename V>
@@ -79,6 +92,7 @@ struct Crs {
   const V *value;
   const int *rb_start;  // row-block starts, nrb + 1 entries
   int n, nrb;
+  int unit;  // 1: implicit unit diagonal, y_i = x_i + (sum of the stored products)
 };
 
 template <typename V>
@@ -86,6 +100,7 @@ struct Ell {
   const int *length, *index;
   const V *data;
   int n, width;
+  int unit;  // 1: implicit unit diagonal, y_i = x_i + (sum of the stored products)
 };
 
 // Row blocks of the CRS kernels: greedy, at most kRowsMax rows and at most
@@ -123,17 +138,21 @@ hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, Partials
 
 // ---- mixed precision: FP32 inner PCG with FP64 reliable updates ----------
 //
-// Inner iteration in FP32 (matrix values, x32/r32/p32/q32, the Jacobi
-// diagonal), every dot accumulated in FP64 from exact float x float
-// products.  When the recursive residual has dropped by `delta` since the
-// last reliable update (or reached the tolerance), the inner loop pauses
-// (done = 2): the host then adds x32 into the FP64 solution, recomputes the
-// true residual k - A x in FP64, restarts r32 from it and keeps p32.
+// Inner iteration in FP32 on an FP32 operator: either the FP32 copy of A,
+// or (Jacobi and polynomial) the scaled operator A_hat = D^-1/2 A D^-1/2
+// with an implicit unit diagonal (off-diagonal values (float)((a_ij s_i) s_j),
+// s_i = 1 / sqrt(a_ii) in FP64): Jacobi-PCG as a symmetric scaling, so no
+// preconditioner vector is read in the inner loop.  Every dot is
+// accumulated in FP64 from exact float x float products.  When the inner
+// residual has dropped by `delta` since the last reliable update (or
+// reached the tolerance), the inner loop pauses (done = 2): the host then
+// adds s o y32 into the FP64 solution, recomputes the true residual
+// k - A x in FP64, restarts r32 = (float)(s o r) and keeps p32.
 
 struct MixedScalars {
   double rho;       // <r32, z32>
   double r2;        // <r32, r32>
-  double residual;  // sqrt(r2) / nrm2_0, the recursive relative residual
+  double residual;  // sqrt(r2) / inner0, the inner relative residual
   double r2_ref;    // r2 right after the last reliable update
   int iteration;    // inner iterations done
   int done;         // 0 run, 1 stop, 2 reliable update requested
@@ -142,56 +161,85 @@ struct MixedScalars {
 struct MixedState {
   MixedScalars cur, next;
   double nrm2_0;     // ||k - A x0|| (FP64)
+  double inner0;     // ||r32|| at the first reliable update
   double true_res;   // ||k - A x|| / nrm2_0 at the last reliable update
   double tol, delta2;
   int max_iter, updates;
 };
 
-// Polynomial preconditioner z = p(M) D^-1 r with M = D^-1 A and
-// p(l) = sum_j c[j] T_j(2 l / b - 1) (Chebyshev basis on [0, b]), applied by
-// Clenshaw's recurrence: B_j = c_j u + 2 s(M) B_{j+1} - B_{j+2} for
-// j = k-1 .. 1, z = c_0 u + s(M) B_1 - B_2, with u = D^-1 r,
-// s(M) v = (2/b) D^-1 A v - v and B_k = c_k u.  One SpMV per step.
+// Polynomial preconditioner z = p(A_hat) r (the matrix p(D^-1 A) D^-1 in
+// scaled form), p(l) = sum_j c[j] T_j(2 l / b - 1) (Chebyshev basis on
+// [0, b]), applied by Clenshaw's recurrence: B_k = c_k r,
+// B_j = c_j r + 2 s(A_hat) B_{j+1} - B_{j+2} for j = k-1 .. 1,
+// z = c_0 r + s(A_hat) B_1 - B_2 with s(A_hat) v = (2/b) A_hat v - v.
+// One SpMV per step.
 constexpr int kPolyMaxDegree = 16;
+// A reliable update whose true residual fell below this fraction of the
+// previous one restarts the search direction (p = z).
+constexpr double kResetRatio = 1e-3;
 struct PolyParams {
   float c[kPolyMaxDegree + 1];
   float two_over_b;
   int degree;
 };
 
+// The FP32 operator of the inner iteration: CRS (row blocks) or, for the
+// scaled operator when the padding is small, ELL (256-row chunks).
+struct Op32 {
+  Crs<float> crs;
+  Ell<float> ell;
+  int is_ell;
+};
+// Blocks of the SpMV kernels on op (and partials of their dots).
+inline int op32_grid(const Op32 &op) {
+  return op.is_ell ? ell32_grid(op.ell.n) : spmv_grid(op.crs.nrb);
+}
+// The ELL layout is used when W * n <= 2 (nnz + n), W the longest row.
+inline bool ell_pays(int64_t width, int64_t n, int64_t nnz) { return width * n <= 2 * (nnz + n); }
+
 // y = (float)x
 hipError_t convert_f64_f32(int64_t n, const double *x, float *y, hipStream_t s);
-// x64 += (double)x32
-hipError_t mp_accumulate(int n, double *x64, const float *x32, hipStream_t s);
+// Scaled operator set-up.  scaled_diag: s[i] = 1 / sqrt(a_ii) and
+// bad[0] = 1 if a row has no or more than one diagonal entry or a_ii <= 0.
+// scaled_copy: off-diagonal entries of row i to offset ptr[i] - i.
+hipError_t scaled_diag(const Crs<double> &A, double *s, int *bad, hipStream_t st);
+hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, float *value_nd,
+                       hipStream_t st);
+// The same entries in column-major ELL: slot (k, i) at k * n + i, length[i].
+hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, int *index, float *data,
+                      hipStream_t st);
+// x64 += scale o (double)y32 (scale nullable: plain add)
+hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s);
 // r64 = k - q64; partials.a = ||r64||^2 (pairs order)
 hipError_t mp_residual(int n, const double *k, const double *q64, double *r64, Partials pa,
                        hipStream_t s);
-// r32 = (float)r64; x32 = 0; z = C32 r32; p32 = z if first; partials of
-// <r32,r32> and <r32,z> (quads order) into pb.
-hipError_t mp_restart(int n, const double *r64, float *r32, float *x32, float *p32,
-                      const float *C32, bool first, Partials pb, hipStream_t s);
+// r32 = (float)(scale o r64); y32 = 0; p32 = r32 if first; partials of
+// <r32,r32> into pb.a and pb.b (quads order).
+hipError_t mp_restart(int n, const double *r64, const double *scale, float *r32, float *y32,
+                      float *p32, bool first, Partials pb, hipStream_t s);
 // One block: folds ||r64||^2 (pairs, prt.a), <r,r> (quads, prr.a) and <r,z>
 // (rz, gz partials) into the restarted state.
 hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, int gz, bool first,
                            MixedState *st, History h, hipStream_t s);
 // Polynomial variants.  Work buffers w[0..2] hold the Clenshaw B_j (B_j in
 // w[j % 3]); z ends in w[0].
-hipError_t mp_restart_poly(int n, const double *r64, float *r32, float *x32, const float *C32,
+hipError_t mp_restart_poly(int n, const double *r64, const double *scale, float *r32, float *y32,
                            const PolyParams *poly, float *bk, Partials prr, hipStream_t s);
-hipError_t mp_poly_apply(const Crs<float> &A, const float *r32, const float *C32,
-                         const PolyParams *poly, int degree, float *const w[3], float *pcopy,
-                         Partials prz, const MixedState *st, hipStream_t s);
-hipError_t mp_update_poly(int n, int gpq, float *x32, float *r32, const float *p32,
-                          const float *q32, const float *C32, const PolyParams *poly, float *bk,
-                          Partials pq, Partials prr, MixedState *st, History h, hipStream_t s);
+hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
+                         float *const w[3], float *pcopy, Partials prz, const MixedState *st,
+                         hipStream_t s);
+hipError_t mp_update_poly(int n, int gpq, float *y32, float *r32, const float *p32,
+                          const float *q32, const PolyParams *poly, float *bk, Partials pq,
+                          Partials prr, MixedState *st, History h, hipStream_t s);
 hipError_t mp_xpay_poly(int n, float *p32, const float *z32, Partials prr, Partials prz, int gz,
                         MixedState *st, History h, hipStream_t s);
-hipError_t mp_matvec_dot_crs(const Crs<float> &A, const float *p, float *q, Partials pq,
-                             MixedState *st, hipStream_t s);
-hipError_t mp_update(int n, int gpq, float *x32, float *r32, const float *p32, const float *q32,
-                     const float *C32, Partials pq, Partials pr, MixedState *st, History h,
-                     hipStream_t s);
-hipError_t mp_xpay(int n, float *p32, const float *r32, const float *C32, Partials pr,
-                   MixedState *st, History h, hipStream_t s);
+hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, MixedState *st,
+                         hipStream_t s);
+// alpha = rho / <p,q>; y32 += alpha p32; r32 -= alpha q32; partials of <r,r>
+hipError_t mp_update(int n, int gpq, float *y32, float *r32, const float *p32, const float *q32,
+                     Partials pq, Partials pr, MixedState *st, History h, hipStream_t s);
+// beta = <r,r>_new / <r,r>; p32 = r32 + beta p32; stop / reliable-update test
+hipError_t mp_xpay(int n, float *p32, const float *r32, Partials pr, MixedState *st, History h,
+                   hipStream_t s);
 
 }  // namespace cghip

@@ -117,14 +117,14 @@ __device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, 
   });
 }
 
-// ELL rows rb*256 + t: the thread adds its live slots left to right.
-template <typename V, typename X, bool WITH_DOT>
-__device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, X *__restrict__ y,
-                                      int rb) {
-  const int r = rb * kBlock + (int)threadIdx.x;
-  if (r >= A.n) return 0.0;
+// ELL row r (column-major slots j * n + r), the thread adds its live slots
+// left to right from 0 and calls epi(r, s).  The matrix streams (length,
+// index, data) are read once, so they are loaded non-temporally and leave
+// the caches to the gathered x.
+template <typename V, typename X, typename Epi>
+__device__ inline double ell_row_epi(const Ell<V> &A, const X *__restrict__ x, int r, Epi epi) {
   const int64_t n = A.n;
-  const int len = A.length[r];
+  const int len = __builtin_nontemporal_load(A.length + r);
   X s = X(0);
   for (int j0 = 0; j0 < len; j0 += kEllChunk) {
     int c[kEllChunk];
@@ -133,16 +133,28 @@ __device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, 
     for (int u = 0; u < kEllChunk; u++) {
       if (j0 + u < len) {
         const int64_t k = (int64_t)(j0 + u) * n + r;
-        c[u] = A.index[k];
-        v[u] = A.data[k];
+        c[u] = __builtin_nontemporal_load(A.index + k);
+        v[u] = __builtin_nontemporal_load(A.data + k);
       }
     }
 #pragma unroll
     for (int u = 0; u < kEllChunk; u++)
       if (j0 + u < len) s += X(v[u]) * x[c[u]];
   }
-  y[r] = s;
-  return WITH_DOT ? (double)x[r] * (double)s : 0.0;
+  return epi(r, s);
+}
+
+// ELL rows rb*256 + t: y = A x; with WITH_DOT, returns the owner's
+// x[r] * y[r] (in FP64) for the fused <x, A x>.
+template <typename V, typename X, bool WITH_DOT>
+__device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, X *__restrict__ y,
+                                      int rb) {
+  const int r = rb * kBlock + (int)threadIdx.x;
+  if (r >= A.n) return 0.0;
+  return ell_row_epi<V, X>(A, x, r, [&](int i, X s) -> double {
+    y[i] = s;
+    return WITH_DOT ? (double)x[i] * (double)s : 0.0;
+  });
 }
 
 // ---------------------------------------------------------------------------

@@ -456,8 +456,7 @@ hipError_t matvec_crs_t(const Crs<V> &A, const X *x, X *y, hipStream_t s) {
 template <typename V, typename X>
 hipError_t matvec_ell_t(const Ell<V> &A, const X *x, X *y, hipStream_t s) {
   if (A.n == 0) return hipSuccess;
-  const int nrb = (A.n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((k_matvec_ell<V, X>), dim3(spmv_grid(nrb)), dim3(kBlock), 0, s, A, x, y);
+  hipLaunchKernelGGL((k_matvec_ell<V, X>), dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A, x, y);
   return hipGetLastError();
 }
 
@@ -529,8 +528,7 @@ hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, 
 
 hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
                               SolveState *st, hipStream_t s) {
-  const int nrb = (A.n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_pcg_matvec_dot_ell, dim3(spmv_grid(nrb)), dim3(kBlock), 0, s, A, p, q,
+  hipLaunchKernelGGL(k_pcg_matvec_dot_ell, dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A, p, q,
                      pq, st);
   return hipGetLastError();
 }

@@ -4,14 +4,17 @@
 // Clark et al., "Solving lattice QCD systems of equations using mixed
 // precision solvers on GPUs", Comput. Phys. Commun. 181, 2010).
 //
-// The inner iteration is the fused three-kernel PCG of kernels.hip on FP32
-// data: the FP32 copy of the CRS values (8 bytes per nonzero with the int32
-// index instead of 12), FP32 vectors (half the vector traffic).  Every dot
-// product is accumulated in FP64 from the products (double)a * (double)b,
+// The inner iteration is the fused three-kernel CG of kernels.hip on FP32
+// data.  With the Jacobi or polynomial preconditioner it runs on the scaled
+// operator A_hat = D^-1/2 A D^-1/2 whose unit diagonal is implicit, so a
+// CG step on A_hat is a Jacobi-PCG step on A without any preconditioner
+// vector in the loop: the SpMV streams 8 bytes per off-diagonal entry
+// (FP32 value + int32 index) and the vector kernels only y, r, p, q.  Every
+// dot product is accumulated in FP64 from the products (double)a * (double)b,
 // which are exact for float operands, so only the summation rounds, in a
-// fixed tree order.  alpha and beta are computed in FP64 and rounded to
-// FP32 for the vector updates.  The solution and the true residual live in
-// FP64: at each reliable update x64 += x32 and r = k - A x64 is recomputed
+// fixed tree order.  alpha and beta are computed in FP64 and rounded to FP32
+// for the vector updates.  The solution and the true residual live in FP64:
+// at each reliable update x64 += s o y32 and r = k - A x64 is recomputed
 // with the FP64 matrix; the search direction p32 is kept, so no Krylov
 // information is lost as it would be by restarting.  The stop test is the
 // FP64 true relative residual.
@@ -36,11 +39,92 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const double *__r
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = (float)x[i];
 }
 
+// s[i] = 1 / sqrt(a_ii); bad[0] = 1 for a row without exactly one diagonal
+// entry or with a_ii <= 0 (every writer stores the same value).
+__global__ __launch_bounds__(kBlock) void k_scaled_diag(Crs<double> A, double *__restrict__ s,
+                                                        int *__restrict__ bad) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < A.n; i += T) {
+    double d = 0.0;
+    int cnt = 0;
+    for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++)
+      if (A.index[e] == (int)i) {
+        d = A.value[e];
+        cnt++;
+      }
+    if (cnt != 1 || !(d > 0.0)) {
+      bad[0] = 1;
+      s[i] = 0.0;
+    } else {
+      s[i] = 1.0 / sqrt(d);
+    }
+  }
+}
+
+// Off-diagonal entries of row i, in stored order, to offset ptr[i] - i
+// (row i has exactly one diagonal entry, checked by k_scaled_diag).
+__global__ __launch_bounds__(kBlock) void k_scaled_copy(Crs<double> A, const double *__restrict__ s,
+                                                        int *__restrict__ index_nd,
+                                                        float *__restrict__ value_nd) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < A.n; i += T) {
+    int k = A.ptr[i] - (int)i;
+    const double si = s[i];
+    for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
+      const int c = A.index[e];
+      if (c == (int)i) continue;
+      index_nd[k] = c;
+      value_nd[k] = (float)((A.value[e] * si) * s[c]);
+      k++;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const double *__restrict__ s,
+                                                       int *__restrict__ length,
+                                                       int *__restrict__ index,
+                                                       float *__restrict__ data) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  const int64_t n = A.n;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
+    int64_t k = 0;
+    const double si = s[i];
+    for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
+      const int c = A.index[e];
+      if (c == (int)i) continue;
+      index[k * n + i] = c;
+      data[k * n + i] = (float)((A.value[e] * si) * s[c]);
+      k++;
+    }
+    length[i] = (int)k;
+  }
+}
+
+// Row-block loop of the FP32 operator: calls epi(i, t) with the sum t of
+// the stored products of row i (from 0, in stored order) for every row the
+// thread owns; returns the sum of the epilogue's dot contributions.
+template <bool ELL, typename Epi>
+__device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Epi epi) {
+  double tpart = 0.0;
+  if (ELL) {
+    const int nrb = ell_chunks(A.ell.n);
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+      const int r = rb * kBlock + (int)threadIdx.x;
+      if (r < A.ell.n) tpart += ell_row_epi<float, float>(A.ell, x, r, epi);
+    }
+  } else {
+    for (int rb = blockIdx.x; rb < A.crs.nrb; rb += gridDim.x)
+      tpart += crs_rowblock_epi<float, float>(A.crs, x, rb, prod, epi);
+  }
+  return tpart;
+}
+
 __global__ __launch_bounds__(kBlock) void k_mp_accum(int n, double *__restrict__ x64,
-                                                     const float *__restrict__ x32) {
+                                                     const float *__restrict__ y32,
+                                                     const double *__restrict__ scale) {
   const int64_t T = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T)
-    x64[i] = x64[i] + (double)x32[i];
+    x64[i] = x64[i] + (scale ? scale[i] * (double)y32[i] : (double)y32[i]);
 }
 
 // r64 = k - q64 and the partials of ||r64||^2 (pairs order).
@@ -61,35 +145,32 @@ __global__ __launch_bounds__(kBlock) void k_mp_residual(int n, const double *__r
   if (threadIdx.x == 0) pa.a[blockIdx.x] = bp;
 }
 
-// r32 = (float)r64, x32 = 0, z = C32 r32, p32 = z when first; partials of
-// <r32,r32> and <r32,z> (quads order).
-template <bool JAC>
+__device__ inline float restart_value(const double *r64, const double *scale, int64_t e) {
+  return scale ? (float)(scale[e] * r64[e]) : (float)r64[e];
+}
+
+// r32 = (float)(s o r64), y32 = 0, p32 = r32 when first; partials of
+// <r32,r32> (quads order).
 __global__ __launch_bounds__(kBlock) void k_mp_restart(int n, const double *__restrict__ r64,
+                                                       const double *__restrict__ scale,
                                                        float *__restrict__ r32,
-                                                       float *__restrict__ x32,
-                                                       float *__restrict__ p32,
-                                                       const float *__restrict__ C32, int first,
+                                                       float *__restrict__ y32,
+                                                       float *__restrict__ p32, int first,
                                                        Partials pb) {
   __shared__ double lds[4];
   Quads Q(n);
-  double s_rr = 0.0, s_rz = 0.0;
+  double s_rr = 0.0;
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      const float r = (float)r64[e];
+      const float r = restart_value(r64, scale, e);
       r32[e] = r;
-      x32[e] = 0.0f;
-      const float z = JAC ? C32[e] * r : r;
-      if (first) p32[e] = z;
+      y32[e] = 0.0f;
+      if (first) p32[e] = r;
       s_rr += (double)r * (double)r;
-      s_rz += (double)r * (double)z;
     }
   }
   const double b0 = block_fold(s_rr, lds);
-  const double b1 = block_fold(s_rz, lds);
-  if (threadIdx.x == 0) {
-    pb.a[blockIdx.x] = b0;
-    pb.b[blockIdx.x] = b1;
-  }
+  if (threadIdx.x == 0) pb.a[blockIdx.x] = b0;
 }
 
 // One block: the true residual norm, rho and r2 of the restarted inner
@@ -105,17 +186,17 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_fold(Partials prt, int gt
     MixedScalars s = st->next;
     if (first) {
       st->nrm2_0 = sqrt(rt2);
+      st->inner0 = sqrt(rr);
       st->updates = 0;
       s.iteration = 0;
     } else {
       st->updates += 1;
     }
-    const double nrm0 = st->nrm2_0;
-    st->true_res = sqrt(rt2) / nrm0;
+    st->true_res = sqrt(rt2) / st->nrm2_0;
     s.rho = rz;
     s.r2 = rr;
     s.r2_ref = rr;
-    s.residual = sqrt(rr) / nrm0;
+    s.residual = sqrt(rr) / st->inner0;
     s.done = st->true_res <= st->tol ? 1 : 0;
     st->next = s;
     st->cur = s;
@@ -137,38 +218,37 @@ __device__ inline bool mp_begin(MixedState *st) {
   return !stop;
 }
 
-// q32 = A32 p32 (rows summed left to right in FP32) and partials of <p,q>.
-__global__ __launch_bounds__(kBlock) void k_mp_matvec_dot_crs(Crs<float> A, const float *__restrict__ p,
-                                                              float *__restrict__ q, Partials pq,
-                                                              MixedState *st) {
+// q32 = A32 p32 (row sums in stored order in FP32; + p_i with the implicit
+// unit diagonal) and partials of <p,q>.
+template <bool ELL>
+__global__ __launch_bounds__(kBlock) void k_mp_matvec_dot(Op32 A, const float *__restrict__ p,
+                                                          float *__restrict__ q, Partials pq,
+                                                          MixedState *st) {
   __shared__ float prod[kNnzCap];
   __shared__ double lds[4];
   if (!mp_begin(st)) return;
-  double tpart = 0.0;
-  for (int rb = blockIdx.x; rb < A.nrb; rb += gridDim.x)
-    tpart += crs_rowblock<float, float, true>(A, p, q, rb, prod);
+  const int unit = ELL ? A.ell.unit : A.crs.unit;
+  const double tpart = op32_rows<ELL>(A, p, prod, [&](int i, float t) -> double {
+    const float qi = unit ? p[i] + t : t;
+    q[i] = qi;
+    return (double)p[i] * (double)qi;
+  });
   const double bp = block_fold(tpart, lds);
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
 
-template <bool JAC>
-__device__ inline void mp_update_elem(float &x, float &r, float p, float q, float c, float a,
-                                      float na, double &s_rr, double &s_rz) {
-  x = x + a * p;
+__device__ inline void mp_update_elem(float &y, float &r, float p, float q, float a, float na,
+                                      double &s_rr) {
+  y = y + a * p;
   r = r + na * q;
-  const float z = JAC ? c * r : r;
   s_rr += (double)r * (double)r;
-  s_rz += (double)r * (double)z;
 }
 
-// alpha = rho / <p,q>; x32 += alpha p32; r32 -= alpha q32; partials of
-// <r,r> and <r,z>, z = C32 r32.
-template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__restrict__ x,
+// alpha = rho / <p,q>; y32 += alpha p32; r32 -= alpha q32; partials of <r,r>.
+__global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__restrict__ y,
                                                       float *__restrict__ r,
                                                       const float *__restrict__ p,
-                                                      const float *__restrict__ q,
-                                                      const float *__restrict__ C, Partials ppq,
+                                                      const float *__restrict__ q, Partials ppq,
                                                       Partials prr, const MixedState *st,
                                                       History h) {
   __shared__ double lds[4];
@@ -182,111 +262,100 @@ __global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__r
   }
   const float a = (float)alpha, na = (float)(-alpha);
   Quads Q(n);
-  double s_rr = 0.0, s_rz = 0.0;
+  double s_rr = 0.0;
   for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 xv[kQuadUnroll], rv[kQuadUnroll], pv[kQuadUnroll], qv[kQuadUnroll], cv[kQuadUnroll];
+    float4 yv[kQuadUnroll], rv[kQuadUnroll], pv[kQuadUnroll], qv[kQuadUnroll];
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
-      xv[u] = v4(x)[j];
+      yv[u] = v4(y)[j];
       rv[u] = v4(r)[j];
       pv[u] = v4(p)[j];
       qv[u] = v4(q)[j];
-      if (JAC) cv[u] = v4(C)[j];
     }
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
-      mp_update_elem<JAC>(xv[u].x, rv[u].x, pv[u].x, qv[u].x, JAC ? cv[u].x : 0.f, a, na, s_rr, s_rz);
-      mp_update_elem<JAC>(xv[u].y, rv[u].y, pv[u].y, qv[u].y, JAC ? cv[u].y : 0.f, a, na, s_rr, s_rz);
-      mp_update_elem<JAC>(xv[u].z, rv[u].z, pv[u].z, qv[u].z, JAC ? cv[u].z : 0.f, a, na, s_rr, s_rz);
-      mp_update_elem<JAC>(xv[u].w, rv[u].w, pv[u].w, qv[u].w, JAC ? cv[u].w : 0.f, a, na, s_rr, s_rz);
-      v4(x)[j] = xv[u];
+      mp_update_elem(yv[u].x, rv[u].x, pv[u].x, qv[u].x, a, na, s_rr);
+      mp_update_elem(yv[u].y, rv[u].y, pv[u].y, qv[u].y, a, na, s_rr);
+      mp_update_elem(yv[u].z, rv[u].z, pv[u].z, qv[u].z, a, na, s_rr);
+      mp_update_elem(yv[u].w, rv[u].w, pv[u].w, qv[u].w, a, na, s_rr);
+      v4(y)[j] = yv[u];
       v4(r)[j] = rv[u];
     }
   }
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      float xe = x[e], re = r[e];
-      mp_update_elem<JAC>(xe, re, p[e], q[e], JAC ? C[e] : 0.f, a, na, s_rr, s_rz);
-      x[e] = xe;
+      float ye = y[e], re = r[e];
+      mp_update_elem(ye, re, p[e], q[e], a, na, s_rr);
+      y[e] = ye;
       r[e] = re;
     }
   }
   const double b0 = block_fold(s_rr, lds);
-  const double b1 = block_fold(s_rz, lds);
-  if (threadIdx.x == 0) {
-    prr.a[blockIdx.x] = b0;
-    prr.b[blockIdx.x] = b1;
-  }
+  if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
 }
 
-// Recursive residual test and reliable-update trigger; beta = rho_new / rho;
-// p32 = z + beta p32 (always: a reliable update keeps p).  Writes `next`.
-template <bool JAC>
+// Writes `next`: iteration count, rho = rz, r2 = rr, the inner residual and
+// the reliable-update trigger; the history entries.
+__device__ inline void mp_advance(MixedState *st, const MixedScalars &c, double rr, double rz,
+                                  double beta, History h) {
+  const int it = c.iteration;
+  MixedScalars s = c;
+  s.iteration = it + 1;
+  s.rho = rz;
+  s.r2 = rr;
+  s.residual = sqrt(rr) / st->inner0;
+  if (s.residual <= st->tol || rr <= st->delta2 * c.r2_ref) s.done = 2;
+  st->next = s;
+  if (h.r2) h.r2[it + 1] = rr;
+  if (h.rho) h.rho[it + 1] = rz;
+  if (h.beta) h.beta[it] = beta;
+}
+
+// beta = <r,r>_new / <r,r>; p32 = r32 + beta p32 (always: a reliable update
+// keeps p).  Writes `next`.
 __global__ __launch_bounds__(kBlock) void k_mp_xpay(int n, float *__restrict__ p,
-                                                    const float *__restrict__ r,
-                                                    const float *__restrict__ C, Partials prr,
+                                                    const float *__restrict__ r, Partials prr,
                                                     MixedState *st, History h) {
   __shared__ double lds[4];
   const MixedScalars c = st->cur;
   if (c.done) return;
-  const int G = gridDim.x;  // the update kernel ran on the same grid
-  const double rr = fold_partials(prr.a, G, lds);
-  const double rz = fold_partials(prr.b, G, lds);
-  const double res = sqrt(rr) / st->nrm2_0;
-  const double beta = rz / c.rho;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const int it = c.iteration;
-    MixedScalars s = c;
-    s.iteration = it + 1;
-    s.rho = rz;
-    s.r2 = rr;
-    s.residual = res;
-    if (res <= st->tol || rr <= st->delta2 * c.r2_ref) s.done = 2;
-    st->next = s;
-    if (h.r2) h.r2[it + 1] = rr;
-    if (h.rho) h.rho[it + 1] = rz;
-    if (h.beta) h.beta[it] = beta;
-  }
+  const double rr = fold_partials(prr.a, gridDim.x, lds);  // the update ran on this grid
+  const double beta = rr / c.rho;
+  if (blockIdx.x == 0 && threadIdx.x == 0) mp_advance(st, c, rr, rr, beta, h);
   const float b = (float)beta;
   Quads Q(n);
   for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 pv[kQuadUnroll], rv[kQuadUnroll], cv[kQuadUnroll];
+    float4 pv[kQuadUnroll], rv[kQuadUnroll];
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
       pv[u] = v4(p)[j];
       rv[u] = v4(r)[j];
-      if (JAC) cv[u] = v4(C)[j];
     }
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       float4 o;
-      o.x = (JAC ? cv[u].x * rv[u].x : rv[u].x) + b * pv[u].x;
-      o.y = (JAC ? cv[u].y * rv[u].y : rv[u].y) + b * pv[u].y;
-      o.z = (JAC ? cv[u].z * rv[u].z : rv[u].z) + b * pv[u].z;
-      o.w = (JAC ? cv[u].w * rv[u].w : rv[u].w) + b * pv[u].w;
+      o.x = rv[u].x + b * pv[u].x;
+      o.y = rv[u].y + b * pv[u].y;
+      o.z = rv[u].z + b * pv[u].z;
+      o.w = rv[u].w + b * pv[u].w;
       v4(p)[Q.q + u * Q.T] = o;
     }
   }
-  for (; Q.q < Q.nquad; Q.q += Q.T) {
-    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      const float z = JAC ? C[e] * r[e] : r[e];
-      p[e] = z + b * p[e];
-    }
-  }
+  for (; Q.q < Q.nquad; Q.q += Q.T)
+    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) p[e] = r[e] + b * p[e];
 }
 
 // ---------------------------------------------------------------------------
 // Polynomial preconditioner (Clenshaw steps fused into FP32 SpMVs).
 
-// x32 += alpha p32; r32 -= alpha q32; B_k = c_k (C32 r32); partials of <r,r>.
-__global__ __launch_bounds__(kBlock) void k_mp_update_poly(int n, int gpq, float *__restrict__ x,
+// y32 += alpha p32; r32 -= alpha q32; B_k = c_k r32; partials of <r,r>.
+__global__ __launch_bounds__(kBlock) void k_mp_update_poly(int n, int gpq, float *__restrict__ y,
                                                            float *__restrict__ r,
                                                            const float *__restrict__ p,
                                                            const float *__restrict__ q,
-                                                           const float *__restrict__ C,
                                                            const PolyParams *poly,
                                                            float *__restrict__ bk, Partials ppq,
                                                            Partials prr, const MixedState *st,
@@ -304,25 +373,52 @@ __global__ __launch_bounds__(kBlock) void k_mp_update_poly(int n, int gpq, float
   const float ck = poly->c[poly->degree];
   Quads Q(n);
   double s_rr = 0.0;
+  for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
+    float4 yv[kQuadUnroll], rv[kQuadUnroll], pv[kQuadUnroll], qv[kQuadUnroll];
+#pragma unroll
+    for (int u = 0; u < kQuadUnroll; u++) {
+      const int64_t j = Q.q + u * Q.T;
+      yv[u] = v4(y)[j];
+      rv[u] = v4(r)[j];
+      pv[u] = v4(p)[j];
+      qv[u] = v4(q)[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kQuadUnroll; u++) {
+      const int64_t j = Q.q + u * Q.T;
+      mp_update_elem(yv[u].x, rv[u].x, pv[u].x, qv[u].x, a, na, s_rr);
+      mp_update_elem(yv[u].y, rv[u].y, pv[u].y, qv[u].y, a, na, s_rr);
+      mp_update_elem(yv[u].z, rv[u].z, pv[u].z, qv[u].z, a, na, s_rr);
+      mp_update_elem(yv[u].w, rv[u].w, pv[u].w, qv[u].w, a, na, s_rr);
+      float4 bv;
+      bv.x = ck * rv[u].x;
+      bv.y = ck * rv[u].y;
+      bv.z = ck * rv[u].z;
+      bv.w = ck * rv[u].w;
+      v4(y)[j] = yv[u];
+      v4(r)[j] = rv[u];
+      v4(bk)[j] = bv;
+    }
+  }
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      x[e] = x[e] + a * p[e];
-      const float re = r[e] + na * q[e];
+      float ye = y[e], re = r[e];
+      mp_update_elem(ye, re, p[e], q[e], a, na, s_rr);
+      y[e] = ye;
       r[e] = re;
-      bk[e] = ck * (C[e] * re);
-      s_rr += (double)re * (double)re;
+      bk[e] = ck * re;
     }
   }
   const double b0 = block_fold(s_rr, lds);
   if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
 }
 
-// Restart of the polynomial variant: r32 = (float)r64, x32 = 0,
-// B_k = c_k (C32 r32), partials of <r,r>.
+// Restart of the polynomial variant: r32 = (float)(s o r64), y32 = 0,
+// B_k = c_k r32, partials of <r,r>.
 __global__ __launch_bounds__(kBlock) void k_mp_restart_poly(int n, const double *__restrict__ r64,
+                                                            const double *__restrict__ scale,
                                                             float *__restrict__ r32,
-                                                            float *__restrict__ x32,
-                                                            const float *__restrict__ C,
+                                                            float *__restrict__ y32,
                                                             const PolyParams *poly,
                                                             float *__restrict__ bk, Partials prr) {
   __shared__ double lds[4];
@@ -331,10 +427,10 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_poly(int n, const double 
   double s_rr = 0.0;
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      const float re = (float)r64[e];
+      const float re = restart_value(r64, scale, e);
       r32[e] = re;
-      x32[e] = 0.0f;
-      bk[e] = ck * (C[e] * re);
+      y32[e] = 0.0f;
+      bk[e] = ck * re;
       s_rr += (double)re * (double)re;
     }
   }
@@ -342,18 +438,17 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_poly(int n, const double 
   if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
 }
 
-// Clenshaw step j with one SpMV t = A B_{j+1}:
-//   s = (2/b) (C t) - B_{j+1}
-//   j > 0: B_j = (c_j u + 2 s) - B_{j+2}
-//   j = 0: z   = (c_0 u + s) - B_2, and the partials of <r, z>
-// with u = C r (B_{j+2} = 0 when it does not exist).  st == nullptr runs the
-// step unconditionally (restart); otherwise it is skipped once the
-// iteration is off.
-template <bool FINAL>
-__global__ __launch_bounds__(kBlock) void k_poly_step(Crs<float> A, const float *__restrict__ b1,
+// Clenshaw step j with one SpMV t = A_hat B_{j+1}:
+//   s = (2/b) t - B_{j+1}
+//   j > 0: B_j = (c_j r + 2 s) - B_{j+2}
+//   j = 0: z   = (c_0 r + s) - B_2, and the partials of <r, z>
+// (B_{j+2} = 0 when it does not exist).  st == nullptr runs the step
+// unconditionally (restart); otherwise it is skipped once the iteration is
+// off.
+template <bool FINAL, bool ELL>
+__global__ __launch_bounds__(kBlock) void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
                                                       const float *__restrict__ r,
-                                                      const float *__restrict__ C,
                                                       const PolyParams *poly, int j,
                                                       float *__restrict__ b0,
                                                       float *__restrict__ pcopy, Partials prz,
@@ -363,23 +458,22 @@ __global__ __launch_bounds__(kBlock) void k_poly_step(Crs<float> A, const float 
   if (st && st->cur.done) return;
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
-  double tpart = 0.0;
-  for (int rb = blockIdx.x; rb < A.nrb; rb += gridDim.x) {
-    tpart += crs_rowblock_epi<float, float>(A, b1, rb, prod, [&](int i, float t) -> double {
-      const float ci = C[i];
-      const float u = ci * r[i];
-      const float sv = tb * (ci * t) - b1[i];
-      const float bb = b2 ? b2[i] : 0.0f;
-      if (FINAL) {
-        const float z = (cj * u + sv) - bb;
-        b0[i] = z;
-        if (pcopy) pcopy[i] = z;
-        return (double)r[i] * (double)z;
-      }
-      b0[i] = (cj * u + 2.0f * sv) - bb;
-      return 0.0;
-    });
-  }
+  const int unit = ELL ? A.ell.unit : A.crs.unit;
+  const double tpart = op32_rows<ELL>(A, b1, prod, [&](int i, float t0) -> double {
+    const float bi = b1[i];
+    const float t = unit ? bi + t0 : t0;
+    const float sv = tb * t - bi;
+    const float bb = b2 ? b2[i] : 0.0f;
+    const float ri = r[i];
+    if (FINAL) {
+      const float z = (cj * ri + sv) - bb;
+      b0[i] = z;
+      if (pcopy) pcopy[i] = z;
+      return (double)ri * (double)z;
+    }
+    b0[i] = (cj * ri + 2.0f * sv) - bb;
+    return 0.0;
+  });
   if (FINAL) {
     const double bp = block_fold(tpart, lds);
     if (threadIdx.x == 0) prz.a[blockIdx.x] = bp;
@@ -387,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_poly_step(Crs<float> A, const float 
 }
 
 // beta = <r,z>_new / <r,z>; p32 = z + beta p32; stop test / reliable-update
-// trigger on the recursive residual.  Writes `next`.
+// trigger on the inner residual.  Writes `next`.
 __global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restrict__ p,
                                                          const float *__restrict__ z, Partials prr,
                                                          Partials prz, int gz, MixedState *st,
@@ -397,21 +491,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restric
   if (c.done) return;
   const double rr = fold_partials(prr.a, gridDim.x, lds);
   const double rz = fold_partials(prz.a, gz, lds);
-  const double res = sqrt(rr) / st->nrm2_0;
   const double beta = rz / c.rho;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const int it = c.iteration;
-    MixedScalars s = c;
-    s.iteration = it + 1;
-    s.rho = rz;
-    s.r2 = rr;
-    s.residual = res;
-    if (res <= st->tol || rr <= st->delta2 * c.r2_ref) s.done = 2;
-    st->next = s;
-    if (h.r2) h.r2[it + 1] = rr;
-    if (h.rho) h.rho[it + 1] = rz;
-    if (h.beta) h.beta[it] = beta;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) mp_advance(st, c, rr, rz, beta, h);
   const float b = (float)beta;
   Quads Q(n);
   for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
@@ -435,6 +516,13 @@ __global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restric
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) p[e] = z[e] + b * p[e];
 }
 
+int row_grid(int n) {
+  int64_t g = ((int64_t)n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
 }  // namespace
 
 hipError_t convert_f64_f32(int64_t n, const double *x, float *y, hipStream_t s) {
@@ -445,8 +533,27 @@ hipError_t convert_f64_f32(int64_t n, const double *x, float *y, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t mp_accumulate(int n, double *x64, const float *x32, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_accum, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x64, x32);
+hipError_t scaled_diag(const Crs<double> &A, double *s, int *bad, hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_diag, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, bad);
+  return hipGetLastError();
+}
+
+hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, float *value_nd,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_copy, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, index_nd,
+                     value_nd);
+  return hipGetLastError();
+}
+
+hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, int *index, float *data,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_ell, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, length, index,
+                     data);
+  return hipGetLastError();
+}
+
+hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_accum, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x64, y32, scale);
   return hipGetLastError();
 }
 
@@ -456,14 +563,10 @@ hipError_t mp_residual(int n, const double *k, const double *q64, double *r64, P
   return hipGetLastError();
 }
 
-hipError_t mp_restart(int n, const double *r64, float *r32, float *x32, float *p32,
-                      const float *C32, bool first, Partials pb, hipStream_t s) {
-  if (C32)
-    hipLaunchKernelGGL(k_mp_restart<true>, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, r64, r32, x32,
-                       p32, C32, (int)first, pb);
-  else
-    hipLaunchKernelGGL(k_mp_restart<false>, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, r64, r32,
-                       x32, p32, C32, (int)first, pb);
+hipError_t mp_restart(int n, const double *r64, const double *scale, float *r32, float *y32,
+                      float *p32, bool first, Partials pb, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_restart, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, r64, scale, r32, y32,
+                     p32, (int)first, pb);
   return hipGetLastError();
 }
 
@@ -474,33 +577,43 @@ hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, 
   return hipGetLastError();
 }
 
-hipError_t mp_restart_poly(int n, const double *r64, float *r32, float *x32, const float *C32,
+hipError_t mp_restart_poly(int n, const double *r64, const double *scale, float *r32, float *y32,
                            const PolyParams *poly, float *bk, Partials prr, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_restart_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, r64, r32, x32,
-                     C32, poly, bk, prr);
+  hipLaunchKernelGGL(k_mp_restart_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, r64, scale, r32,
+                     y32, poly, bk, prr);
   return hipGetLastError();
 }
 
-hipError_t mp_poly_apply(const Crs<float> &A, const float *r32, const float *C32,
-                         const PolyParams *poly, int degree, float *const w[3], float *pcopy,
-                         Partials prz, const MixedState *st, hipStream_t s) {
-  const dim3 grid(spmv_grid(A.nrb)), block(kBlock);
+template <bool ELL>
+static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
+                         float *const w[3], float *pcopy, Partials prz, const MixedState *st,
+                         hipStream_t s) {
+  const dim3 grid(op32_grid(A)), block(kBlock);
   for (int j = degree - 1; j >= 1; j--) {
     const float *b2 = (j + 2 <= degree) ? w[(j + 2) % 3] : nullptr;
-    hipLaunchKernelGGL(k_poly_step<false>, grid, block, 0, s, A, w[(j + 1) % 3], b2, r32, C32, poly,
-                       j, w[j % 3], (float *)nullptr, prz, st);
+    hipLaunchKernelGGL((k_poly_step<false, ELL>), grid, block, 0, s, A, w[(j + 1) % 3], b2, r32,
+                       poly, j, w[j % 3], (float *)nullptr, prz, st);
   }
   const float *b2 = degree >= 2 ? w[2] : nullptr;
-  hipLaunchKernelGGL(k_poly_step<true>, grid, block, 0, s, A, w[1], b2, r32, C32, poly, 0, w[0],
+  hipLaunchKernelGGL((k_poly_step<true, ELL>), grid, block, 0, s, A, w[1], b2, r32, poly, 0, w[0],
                      pcopy, prz, st);
+}
+
+hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
+                         float *const w[3], float *pcopy, Partials prz, const MixedState *st,
+                         hipStream_t s) {
+  if (A.is_ell)
+    poly_apply_t<true>(A, r32, poly, degree, w, pcopy, prz, st, s);
+  else
+    poly_apply_t<false>(A, r32, poly, degree, w, pcopy, prz, st, s);
   return hipGetLastError();
 }
 
-hipError_t mp_update_poly(int n, int gpq, float *x32, float *r32, const float *p32,
-                          const float *q32, const float *C32, const PolyParams *poly, float *bk,
-                          Partials pq, Partials prr, MixedState *st, History h, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_update_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, x32, r32,
-                     p32, q32, C32, poly, bk, pq, prr, st, h);
+hipError_t mp_update_poly(int n, int gpq, float *y32, float *r32, const float *p32,
+                          const float *q32, const PolyParams *poly, float *bk, Partials pq,
+                          Partials prr, MixedState *st, History h, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_update_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, y32, r32,
+                     p32, q32, poly, bk, pq, prr, st, h);
   return hipGetLastError();
 }
 
@@ -511,33 +624,27 @@ hipError_t mp_xpay_poly(int n, float *p32, const float *z32, Partials prr, Parti
   return hipGetLastError();
 }
 
-hipError_t mp_matvec_dot_crs(const Crs<float> &A, const float *p, float *q, Partials pq,
-                             MixedState *st, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_matvec_dot_crs, dim3(spmv_grid(A.nrb)), dim3(kBlock), 0, s, A, p, q, pq,
-                     st);
+hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, MixedState *st,
+                         hipStream_t s) {
+  if (A.is_ell)
+    hipLaunchKernelGGL(k_mp_matvec_dot<true>, dim3(op32_grid(A)), dim3(kBlock), 0, s, A, p, q, pq,
+                       st);
+  else
+    hipLaunchKernelGGL(k_mp_matvec_dot<false>, dim3(op32_grid(A)), dim3(kBlock), 0, s, A, p, q, pq,
+                       st);
   return hipGetLastError();
 }
 
-hipError_t mp_update(int n, int gpq, float *x32, float *r32, const float *p32, const float *q32,
-                     const float *C32, Partials pq, Partials pr, MixedState *st, History h,
-                     hipStream_t s) {
-  if (C32)
-    hipLaunchKernelGGL(k_mp_update<true>, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, x32, r32,
-                       p32, q32, C32, pq, pr, st, h);
-  else
-    hipLaunchKernelGGL(k_mp_update<false>, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, x32,
-                       r32, p32, q32, C32, pq, pr, st, h);
+hipError_t mp_update(int n, int gpq, float *y32, float *r32, const float *p32, const float *q32,
+                     Partials pq, Partials pr, MixedState *st, History h, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_update, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, y32, r32, p32,
+                     q32, pq, pr, st, h);
   return hipGetLastError();
 }
 
-hipError_t mp_xpay(int n, float *p32, const float *r32, const float *C32, Partials pr,
-                   MixedState *st, History h, hipStream_t s) {
-  if (C32)
-    hipLaunchKernelGGL(k_mp_xpay<true>, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, p32, r32, C32, pr,
-                       st, h);
-  else
-    hipLaunchKernelGGL(k_mp_xpay<false>, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, p32, r32, C32,
-                       pr, st, h);
+hipError_t mp_xpay(int n, float *p32, const float *r32, Partials pr, MixedState *st, History h,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_xpay, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, p32, r32, pr, st, h);
   return hipGetLastError();
 }
 

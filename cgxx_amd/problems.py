@@ -65,6 +65,22 @@ class CRS:
             k[live] += self.value[starts[live] + j]
         return k
 
+    def to_ell(self):
+        """(width, length, index, data) in column-major ELL, slot j*n + i,
+        as Matrix.cpp:339-361 builds it from CRS (padding slots zero)."""
+        n = self.n
+        lens = self.row_lengths().astype(np.int32)
+        w = int(lens.max()) if n else 0
+        idx = np.zeros(n * w, np.int32)
+        dat = np.zeros(n * w, np.float64)
+        starts = self.ptr[:-1].astype(np.int64)
+        rows = np.arange(n, dtype=np.int64)
+        for j in range(w):
+            live = lens > j
+            idx[j * n + rows[live]] = self.index[starts[live] + j]
+            dat[j * n + rows[live]] = self.value[starts[live] + j]
+        return w, lens, idx, dat
+
     def matvec(self, x: np.ndarray) -> np.ndarray:
         """Plain float64 product (any order); for residual checks only."""
         rows = np.repeat(np.arange(self.n, dtype=np.int64), self.row_lengths())

@@ -72,7 +72,7 @@ const char *cghip_last_error(const cghip_ctx *ctx);
 /* Reduction geometry of the kernels (the summation order is a function of
  * these and of the vector length / CRS row pointer only). */
 int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap,
-                   int *spmv_gmax);
+                   int *spmv_gmax, int *ell_gmax);
 
 /* --- data (CGCUDA::doTransferTo / doTransferFrom, cuda/CGCUDA.cu:38-112) - */
 
@@ -140,16 +140,24 @@ int cghip_solve(cghip_ctx *ctx, const cghip_solve_opts *opts,
 
 /* --- mixed precision (no counterpart in the reference, whose arithmetic is
  * FP64 throughout, def.h:6) ------------------------------------------------
- * An FP32 inner PCG (FP32 matrix values and vectors, int32 indices, every
+ * An FP32 inner CG (FP32 matrix values and vectors, int32 indices, every
  * dot accumulated in FP64) with FP64 reliable updates: whenever the inner
  * recursive residual has dropped by `delta` (or reached `tol`), the FP32
  * correction is added to the FP64 solution X, the true residual K - A X is
  * recomputed in FP64 with the FP64 matrix, and the inner iteration goes on
  * from it with its search direction kept.  The stop test is the FP64 true
- * relative residual ||K - A X|| / ||K - A X0|| <= tol.  CRS only. */
+ * relative residual ||K - A X|| / ||K - A X0|| <= tol.  CRS only.
+ * With the Jacobi (or polynomial) preconditioner the inner iteration runs
+ * on the scaled operator D^-1/2 A D^-1/2, D = diag(A), stored in FP32
+ * without its unit diagonal: Jacobi-PCG applied as a symmetric scaling
+ * (the same iterates in exact arithmetic, no preconditioner vector in the
+ * inner loop).  PRECOND_NONE runs on the FP32 copy of A. */
 
-/* Builds the FP32 copy of the CRS values and of the Jacobi diagonal and the
- * FP32 work vectors (after cghip_upload_crs / cghip_upload_jacobi). */
+/* Builds the FP32 copy of the CRS values, the scaled operator (when every
+ * row has exactly one diagonal entry and it is positive; otherwise only
+ * PRECOND_NONE is available to cghip_solve_mixed) and the FP32 work
+ * vectors.  Call after cghip_upload_crs; cghip_upload_jacobi is not
+ * needed (the scaling comes from the matrix's own diagonal). */
 int cghip_prepare_fp32(cghip_ctx *ctx);
 
 /* Polynomial preconditioner B = p(D^-1 A) D^-1 with
@@ -157,8 +165,10 @@ int cghip_prepare_fp32(cghip_ctx *ctx);
  * on [0, b]; b must bound the spectrum of D^-1 A from above (e.g. its
  * Gershgorin bound).  B is SPD when p > 0 on the spectrum: the call rejects
  * (CGHIP_ERR_ARG) coefficients with p <= 0 anywhere on a 4096-point grid of
- * [0, b].  1 <= degree <= 16.  Applied by Clenshaw's recurrence, one FP32
- * SpMV per degree, fused into the SpMV kernels. */
+ * [0, b].  1 <= degree <= 16.  Applied in scaled form, z = p(A_hat) r with
+ * A_hat = D^-1/2 A D^-1/2, by Clenshaw's recurrence: one FP32 SpMV per
+ * degree with the recurrence fused into the SpMV kernel.  Needs the scaled
+ * operator of cghip_prepare_fp32; dropped by the next matrix upload. */
 int cghip_set_poly(cghip_ctx *ctx, int degree, const double *coeffs, double b);
 
 typedef struct {
@@ -176,8 +186,14 @@ typedef struct {
   double true_residual;    /* final FP64 ||K - A X|| / ||K - A X0|| */
   double inner_residual;   /* final FP32 recursive relative residual */
   double wall_s;
-  double ms_matvec, ms_update, ms_xpay, ms_refresh;  /* profile: summed kernel ms */
-  int n_matvec, n_update, n_xpay, n_refresh;
+  /* profile != NONE: summed kernel time (ms) and launch counts of the SpMV +
+   * <p,q>, the x/r update, the p update, the reliable updates and (the
+   * polynomial) the Clenshaw SpMV steps */
+  double ms_matvec, ms_update, ms_xpay, ms_refresh, ms_poly;
+  int n_matvec, n_update, n_xpay, n_refresh, n_poly;
+  /* reliable updates that also restarted the search direction (the true
+   * residual fell by more than 1e3 since the previous update) */
+  int restarts;
 } cghip_mixed_info;
 
 int cghip_solve_mixed(cghip_ctx *ctx, const cghip_mixed_opts *opts,

@@ -390,8 +390,13 @@ int orc_crs_rowblocks(int N, const int *ptr, int *starts) {
  * running sum from 0.0); then the block and final folds. */
 double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
                              const double *q) {
+  return orc_rowblock_dot_tree_g(nrb, starts, p, q, ORC_SPMV_GMAX);
+}
+
+double orc_rowblock_dot_tree_g(int nrb, const int *starts, const double *p,
+                               const double *q, int gmax) {
   if (nrb <= 0) return 0.0;
-  int G = nrb < ORC_SPMV_GMAX ? nrb : ORC_SPMV_GMAX;
+  int G = nrb < gmax ? nrb : gmax;
   double *partial = (double *)malloc(sizeof(double) * (size_t)G);
   double v[ORC_BLOCK];
   for (int g = 0; g < G; g++) {
@@ -518,7 +523,8 @@ int orc_pcg(const orc_problem *p, const double *k, double *x, orc_result *res,
 
   for (it = 0; it < p->max_iter; it++) {
     spmv(&c, P, Q);
-    dot_pq = fused ? orc_rowblock_dot_tree(c.nrb, c.starts, P, Q)
+    dot_pq = fused ? orc_rowblock_dot_tree_g(c.nrb, c.starts, P, Q,
+                                             p->format == 2 ? ORC_ELL_GMAX : ORC_SPMV_GMAX)
                    : dot(&c, P, Q);
     a = rho / dot_pq;
     if (h_pq) h_pq[it] = dot_pq;
@@ -588,51 +594,127 @@ typedef struct {
   int iteration, done;
 } mp_scalars;
 
-/* Clenshaw's recurrence for z = p(M) u, M = D^-1 A, u = C32 r32, in FP32 with
- * the expression order of the HIP kernels' epilogues: B_j lives in
- * w[j % 3]; on entry w[degree % 3] holds B_degree = c_degree u; for
- * j = degree-1 .. 0 one FP32 row-ordered SpMV t = A32 B_{j+1}, then
- * s = (2/b) (C t) - B_{j+1}, B_j = (c_j u + 2 s) - B_{j+2} (j > 0) or
- * z = (c_0 u + s) - B_2 (j = 0), B_{j+2} = 0 when j + 2 > degree.
- * z ends in w[0]. */
-static void poly_clenshaw(int N, const int *ptr, const int *index,
-                          const float *v32, const float *C32,
-                          const float *r32, const orc_poly *pl, float *w[3]) {
+/* The FP32 operator of the inner iteration.
+ * scaled = 0: A itself, value32 = (float)a_ij, every stored entry.
+ * scaled = 1: A_hat = D^-1/2 A D^-1/2 with its diagonal taken as exactly 1
+ *   and not stored: s_i = 1 / sqrt(a_ii) (FP64), off-diagonal values
+ *   (float)((a_ij * s_i) * s_j) in stored order, diagonal entries dropped.
+ * y = op x: t = 0; t += v * x[c] over the stored entries in order (FP32);
+ * y_i = scaled ? x_i + t : t. */
+typedef struct {
+  int N, unit, nrb;
+  int *ptr, *index, *starts;
+  float *value;
+  double *s; /* scaled: s_i, else NULL */
+} op32;
+
+static void op32_free(op32 *o) {
+  free(o->ptr), free(o->index), free(o->starts), free(o->value), free(o->s);
+}
+
+/* Returns 0, or -3 if scaled and a row has no positive diagonal entry. */
+static int op32_build(const orc_problem *p, int scaled, op32 *o) {
+  int N = p->N, nz = p->ptr[N];
+  memset(o, 0, sizeof(*o));
+  o->N = N;
+  o->unit = scaled;
+  size_t nb = (size_t)(N > 0 ? N : 1);
+  o->ptr = (int *)malloc(sizeof(int) * (nb + 1));
+  o->index = (int *)malloc(sizeof(int) * (size_t)(nz > 0 ? nz : 1));
+  o->value = (float *)malloc(sizeof(float) * (size_t)(nz > 0 ? nz : 1));
+  o->starts = (int *)malloc(sizeof(int) * (nb + 1));
+  if (scaled) {
+    o->s = (double *)malloc(sizeof(double) * nb);
+    for (int i = 0; i < N; i++) {
+      double d = 0.0;
+      int has = 0;
+      for (int e = p->ptr[i]; e < p->ptr[i + 1]; e++)
+        if (p->index[e] == i) d = p->value[e], has = 1;
+      if (!has || !(d > 0.0)) return -3;
+      o->s[i] = 1.0 / sqrt(d);
+    }
+  }
+  int k = 0;
+  for (int i = 0; i < N; i++) {
+    o->ptr[i] = k;
+    for (int e = p->ptr[i]; e < p->ptr[i + 1]; e++) {
+      int c = p->index[e];
+      if (scaled && c == i) continue;
+      o->index[k] = c;
+      o->value[k] = scaled ? (float)((p->value[e] * o->s[i]) * o->s[c]) : (float)p->value[e];
+      k++;
+    }
+  }
+  o->ptr[N] = k;
+  o->nrb = orc_crs_rowblocks(N, o->ptr, o->starts);
+  if (scaled) {
+    /* the library stores the scaled operator as ELL (256-row chunks) when
+     * W * N <= 2 (nnz + N), W the longest row: same row sums, the dot tree
+     * over uniform 256-row blocks */
+    int64_t w = 0;
+    for (int i = 0; i < N; i++)
+      if (o->ptr[i + 1] - o->ptr[i] > w) w = o->ptr[i + 1] - o->ptr[i];
+    if (w * (int64_t)N <= 2 * ((int64_t)k + N)) {
+      o->nrb = (N + ORC_ROWS_MAX - 1) / ORC_ROWS_MAX;
+      for (int b = 0; b <= o->nrb; b++)
+        o->starts[b] = b * ORC_ROWS_MAX < N ? b * ORC_ROWS_MAX : N;
+    }
+  }
+  return 0;
+}
+
+static float op32_row(const op32 *o, const float *x, int i) {
+  float t = 0.0f;
+  for (int e = o->ptr[i]; e < o->ptr[i + 1]; e++) t += o->value[e] * x[o->index[e]];
+  return o->unit ? x[i] + t : t;
+}
+
+/* Clenshaw's recurrence for z = p(op) u in FP32 with the expression order
+ * of the HIP kernels' epilogues: B_j lives in w[j % 3]; on entry
+ * w[degree % 3] holds B_degree = c_degree u; for j = degree-1 .. 0 one
+ * operator application t = op B_{j+1}, then s = (2/b) t - B_{j+1},
+ * B_j = (c_j u + 2 s) - B_{j+2} (j > 0) or z = (c_0 u + s) - B_2 (j = 0),
+ * B_{j+2} = 0 when j + 2 > degree.  z ends in w[0]. */
+static void poly_clenshaw(const op32 *o, const float *u, const orc_poly *pl,
+                          float *w[3]) {
   const float tb = pl->two_over_b;
   for (int j = pl->degree - 1; j >= 0; j--) {
     const float *b1 = w[(j + 1) % 3];
     const float *b2 = (j + 2 <= pl->degree) ? w[(j + 2) % 3] : NULL;
     float *b0 = w[j % 3];
     const float cj = pl->c[j];
-    for (int i = 0; i < N; i++) {
-      float t = 0.0f;
-      for (int e = ptr[i]; e < ptr[i + 1]; e++) t += v32[e] * b1[index[e]];
-      const float u = C32[i] * r32[i];
-      const float sv = tb * (C32[i] * t) - b1[i];
+    for (int i = 0; i < o->N; i++) {
+      const float t = op32_row(o, b1, i);
+      const float sv = tb * t - b1[i];
       const float bb = b2 ? b2[i] : 0.0f;
-      b0[i] = j > 0 ? (cj * u + 2.0f * sv) - bb : (cj * u + sv) - bb;
+      b0[i] = j > 0 ? (cj * u[i] + 2.0f * sv) - bb : (cj * u[i] + sv) - bb;
     }
   }
 }
 
 int orc_poly_apply(int N, const int *ptr, const int *index,
-                   const double *value, const double *C, const float *r32,
-                   const orc_poly *pl, float *z) {
+                   const double *value, const float *r32, const orc_poly *pl,
+                   float *z) {
   if (!pl || pl->degree < 1 || pl->degree > ORC_POLY_MAX) return -2;
+  orc_problem p;
+  memset(&p, 0, sizeof(p));
+  p.N = N;
+  p.ptr = ptr, p.index = index, p.value = value;
+  op32 o;
+  int rc = op32_build(&p, 1, &o);
+  if (rc) {
+    op32_free(&o);
+    return rc;
+  }
   size_t nb = (size_t)(N > 0 ? N : 1);
-  int nz = ptr[N];
-  float *v32 = (float *)malloc(sizeof(float) * (size_t)(nz > 0 ? nz : 1));
-  float *C32 = (float *)malloc(sizeof(float) * nb);
   float *w[3];
   for (int j = 0; j < 3; j++) w[j] = (float *)calloc(nb, sizeof(float));
-  for (int e = 0; e < nz; e++) v32[e] = (float)value[e];
-  for (int i = 0; i < N; i++) C32[i] = (float)C[i];
   const float ck = pl->c[pl->degree];
-  for (int i = 0; i < N; i++) w[pl->degree % 3][i] = ck * (C32[i] * r32[i]);
-  poly_clenshaw(N, ptr, index, v32, C32, r32, pl, w);
+  for (int i = 0; i < N; i++) w[pl->degree % 3][i] = ck * r32[i];
+  poly_clenshaw(&o, r32, pl, w);
   memcpy(z, w[0], sizeof(float) * (size_t)N);
-  free(v32), free(C32);
   for (int j = 0; j < 3; j++) free(w[j]);
+  op32_free(&o);
   return 0;
 }
 
@@ -651,13 +733,17 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
   if (p->format != 1) return -1;
   if (p->precond == 2 && (!pl || pl->degree < 1 || pl->degree > ORC_POLY_MAX))
     return -2;
-  int N = p->N, nz = p->ptr[N];
-  int jac = p->precond == 1;
+  int N = p->N;
   int poly = p->precond == 2;
+  int scaled = p->precond != 0;
+  op32 o;
+  int rc = op32_build(p, scaled, &o);
+  if (rc) {
+    op32_free(&o);
+    return rc;
+  }
   size_t nb = (size_t)(N > 0 ? N : 1);
-  float *v32 = (float *)malloc(sizeof(float) * (size_t)(nz > 0 ? nz : 1));
-  float *C32 = (float *)malloc(sizeof(float) * nb);
-  float *x32 = (float *)calloc(nb, sizeof(float)), *r32 = (float *)malloc(sizeof(float) * nb),
+  float *y32 = (float *)calloc(nb, sizeof(float)), *r32 = (float *)malloc(sizeof(float) * nb),
         *p32 = (float *)malloc(sizeof(float) * nb), *q32 = (float *)malloc(sizeof(float) * nb);
   double *q64 = (double *)malloc(sizeof(double) * nb), *r64 = (double *)malloc(sizeof(double) * nb),
          *f0 = (double *)malloc(sizeof(double) * nb), *f1 = (double *)malloc(sizeof(double) * nb);
@@ -666,56 +752,42 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
     for (int j = 0; j < 3; j++) w[j] = (float *)calloc(nb, sizeof(float));
   const int deg = poly ? pl->degree : 0;
   const float ck = poly ? pl->c[deg] : 0.0f;
-  int nrb = 0;
-  int *starts = (int *)malloc(sizeof(int) * (nb + 1));
-  nrb = orc_crs_rowblocks(N, p->ptr, starts);
-  for (int e = 0; e < nz; e++) v32[e] = (float)p->value[e];
-  for (int i = 0; i < N; i++) C32[i] = (jac || poly) ? (float)p->C[i] : 1.0f;
   double delta2 = (delta > 0 ? delta : 0.1) * (delta > 0 ? delta : 0.1);
-  double nrm2_0 = 0, true_res = 0;
+  double nrm2_0 = 0, inner0 = 0, true_res = 0;
   int updates = 0;
   mp_scalars s = {0, 0, 0, 0, 0, 0};
 
-  /* refresh: x64 += x32 (unless first); r64 = k - A x64; restart r32. */
+  /* Reliable update: x64 += s o y32 (y32 unscaled when not scaled) unless
+   * first; r64 = k - A x64 (FP64); r32 = (float)(s o r64); y32 = 0. */
 #define REFRESH(first)                                                        \
   do {                                                                        \
     if (!(first))                                                             \
-      for (int i = 0; i < N; i++) x[i] = x[i] + (double)x32[i];               \
+      for (int i = 0; i < N; i++)                                             \
+        x[i] = x[i] + (scaled ? o.s[i] * (double)y32[i] : (double)y32[i]);    \
     orc_spmv_crs(N, p->ptr, p->index, p->value, x, q64);                      \
     for (int i = 0; i < N; i++) r64[i] = k[i] - q64[i];                       \
     double rt2 = orc_dot_tree(N, r64, r64);                                   \
-    double rr, rz;                                                            \
+    for (int i = 0; i < N; i++) {                                             \
+      float r = scaled ? (float)(o.s[i] * r64[i]) : (float)r64[i];            \
+      r32[i] = r;                                                             \
+      y32[i] = 0.0f;                                                          \
+      if (poly) w[deg % 3][i] = ck * r;                                       \
+      else if (first) p32[i] = r;                                             \
+      f0[i] = (double)r * (double)r;                                          \
+    }                                                                         \
+    double rr = quad_tree(N, f0), rz = rr;                                    \
     if (poly) {                                                               \
-      for (int i = 0; i < N; i++) {                                           \
-        float r = (float)r64[i];                                              \
-        r32[i] = r;                                                           \
-        x32[i] = 0.0f;                                                        \
-        w[deg % 3][i] = ck * (C32[i] * r);                                    \
-        f0[i] = (double)r * (double)r;                                        \
-      }                                                                       \
-      rr = quad_tree(N, f0);                                                  \
-      poly_clenshaw(N, p->ptr, p->index, v32, C32, r32, pl, w);               \
+      poly_clenshaw(&o, r32, pl, w);                                          \
       for (int i = 0; i < N; i++) {                                           \
         if (first) p32[i] = w[0][i];                                          \
         f0[i] = (double)r32[i];                                               \
         f1[i] = (double)w[0][i];                                              \
       }                                                                       \
-      rz = orc_rowblock_dot_tree(nrb, starts, f0, f1);                        \
-    } else {                                                                  \
-      for (int i = 0; i < N; i++) {                                           \
-        float r = (float)r64[i];                                              \
-        r32[i] = r;                                                           \
-        x32[i] = 0.0f;                                                        \
-        float z = jac ? C32[i] * r : r;                                       \
-        if (first) p32[i] = z;                                                \
-        f0[i] = (double)r * (double)r;                                        \
-        f1[i] = (double)r * (double)z;                                        \
-      }                                                                       \
-      rr = quad_tree(N, f0);                                                  \
-      rz = quad_tree(N, f1);                                                  \
+      rz = orc_rowblock_dot_tree(o.nrb, o.starts, f0, f1);                    \
     }                                                                         \
     if (first) {                                                              \
       nrm2_0 = sqrt(rt2);                                                     \
+      inner0 = sqrt(rr);                                                      \
       updates = 0;                                                            \
       s.iteration = 0;                                                        \
     } else {                                                                  \
@@ -725,7 +797,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
     s.rho = rz;                                                               \
     s.r2 = rr;                                                                \
     s.r2_ref = rr;                                                            \
-    s.residual = sqrt(rr) / nrm2_0;                                           \
+    s.residual = sqrt(rr) / inner0;                                           \
     s.done = true_res <= p->tol ? 1 : 0;                                      \
     if ((first) && h_r2) h_r2[0] = rr;                                        \
     if ((first) && h_rho) h_rho[0] = rz;                                      \
@@ -735,7 +807,14 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
   for (;;) {
     if (s.done == 1) break;
     if (s.done == 2) {
+      /* A phase that reduced the true residual by more than 1e3 (e.g. one
+       * step that reached FP32 resolution) leaves a search direction that
+       * carries no accurate information about the new residual: restart
+       * it, p = z. */
+      double prev = true_res;
       REFRESH(0);
+      if (true_res < ORC_RESET_RATIO * prev)
+        for (int i = 0; i < N; i++) p32[i] = poly ? w[0][i] : r32[i];
       continue;
     }
     if (s.iteration >= p->max_iter) {
@@ -743,48 +822,31 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
       break;
     }
     int it = s.iteration;
-    /* q32 = A32 p32, rows in order in FP32 */
-    for (int i = 0; i < N; i++) {
-      float t = 0.0f;
-      for (int j = p->ptr[i]; j < p->ptr[i + 1]; j++) t += v32[j] * p32[p->index[j]];
-      q32[i] = t;
-    }
+    for (int i = 0; i < N; i++) q32[i] = op32_row(&o, p32, i);
     for (int i = 0; i < N; i++) f0[i] = (double)p32[i];
     for (int i = 0; i < N; i++) f1[i] = (double)q32[i];
-    double pq = orc_rowblock_dot_tree(nrb, starts, f0, f1);
+    double pq = orc_rowblock_dot_tree(o.nrb, o.starts, f0, f1);
     double alpha = s.rho / pq;
     if (h_pq) h_pq[it] = pq;
     if (h_alpha) h_alpha[it] = alpha;
     float a = (float)alpha, na = (float)(-alpha);
-    double rr, rz;
+    for (int i = 0; i < N; i++) {
+      y32[i] = y32[i] + a * p32[i];
+      float r = r32[i] + na * q32[i];
+      r32[i] = r;
+      if (poly) w[deg % 3][i] = ck * r;
+      f0[i] = (double)r * (double)r;
+    }
+    double rr = quad_tree(N, f0), rz = rr;
     if (poly) {
-      for (int i = 0; i < N; i++) {
-        x32[i] = x32[i] + a * p32[i];
-        float r = r32[i] + na * q32[i];
-        r32[i] = r;
-        w[deg % 3][i] = ck * (C32[i] * r);
-        f0[i] = (double)r * (double)r;
-      }
-      rr = quad_tree(N, f0);
-      poly_clenshaw(N, p->ptr, p->index, v32, C32, r32, pl, w);
+      poly_clenshaw(&o, r32, pl, w);
       for (int i = 0; i < N; i++) {
         f0[i] = (double)r32[i];
         f1[i] = (double)w[0][i];
       }
-      rz = orc_rowblock_dot_tree(nrb, starts, f0, f1);
-    } else {
-      for (int i = 0; i < N; i++) {
-        x32[i] = x32[i] + a * p32[i];
-        float r = r32[i] + na * q32[i];
-        r32[i] = r;
-        float z = jac ? C32[i] * r : r;
-        f0[i] = (double)r * (double)r;
-        f1[i] = (double)r * (double)z;
-      }
-      rr = quad_tree(N, f0);
-      rz = quad_tree(N, f1);
+      rz = orc_rowblock_dot_tree(o.nrb, o.starts, f0, f1);
     }
-    double resid = sqrt(rr) / nrm2_0;
+    double resid = sqrt(rr) / inner0;
     double beta = rz / s.rho;
     mp_scalars n2 = s;
     n2.iteration = it + 1;
@@ -797,7 +859,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
     if (h_beta) h_beta[it] = beta;
     float b = (float)beta;
     for (int i = 0; i < N; i++) {
-      float z = poly ? w[0][i] : jac ? C32[i] * r32[i] : r32[i];
+      float z = poly ? w[0][i] : r32[i];
       p32[i] = z + b * p32[i];
     }
     s = n2;
@@ -807,8 +869,9 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
   res->updates = updates;
   res->true_residual = true_res;
   res->inner_residual = s.residual;
-  free(v32), free(C32), free(x32), free(r32), free(p32), free(q32);
-  free(q64), free(r64), free(f0), free(f1), free(starts);
+  free(y32), free(r32), free(p32), free(q32);
+  free(q64), free(r64), free(f0), free(f1);
   for (int j = 0; j < 3; j++) free(w[j]);
+  op32_free(&o);
   return 0;
 }

@@ -45,7 +45,8 @@ extern "C" {
 #define ORC_VEC_GMAX 2048 /* max blocks of the grid-stride vector kernels */
 #define ORC_ROWS_MAX 256  /* max rows of one SpMV row block (one per thread) */
 #define ORC_NNZ_CAP 2048  /* max nonzeros staged per CRS row block */
-#define ORC_SPMV_GMAX 2048 /* max blocks of the SpMV kernels */
+#define ORC_SPMV_GMAX 2048 /* max blocks of the CRS SpMV and FP32 ELL kernels */
+#define ORC_ELL_GMAX 8192  /* max blocks of the FP64 ELL SpMV kernels */
 
 /* Matrix Market COO, after the reader's symmetric mirroring. */
 typedef struct {
@@ -92,6 +93,9 @@ double orc_dot_tree(int N, const double *a, const double *b);
  * (CRS: orc_crs_rowblocks; ELL: 256-row chunks). */
 double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
                              const double *q);
+/* The same with at most gmax blocks (G = min(nrb, gmax)). */
+double orc_rowblock_dot_tree_g(int nrb, const int *starts, const double *p,
+                               const double *q, int gmax);
 
 /* PCG (CG.cpp:284-385).
  * order 0: every dot left to right (SerialCG.cpp:145-155);
@@ -135,15 +139,26 @@ int orc_pcg(const orc_problem *p, const double *k, double *x, orc_result *res,
 int orc_vec_grid4(int N);
 
 /* The FP32-inner / FP64 reliable-update PCG of the HIP library (no
- * reference counterpart), restated kernel by kernel with the same casts:
- * FP32 matrix values (float)value, FP32 vectors, FP32 row sums in row order,
- * FP64 dots of exact float products in the kernels' tree orders, alpha and
- * beta in FP64 rounded to FP32 for the updates, x64 += x32 and
- * r = k - A x64 in FP64 at every reliable update, stop on the FP64 true
- * relative residual.  CRS only; precond 0 none, 1 Jacobi (C32 = (float)C),
- * 2 polynomial (orc_mixed_pcg_poly).
+ * reference counterpart), restated kernel by kernel with the same casts.
+ * precond 0: CG on the FP32 copy of A ((float)a_ij, every stored entry).
+ * precond 1 (Jacobi) and 2 (polynomial): CG on the FP32 scaled operator
+ *   A_hat = D^-1/2 A D^-1/2 with s_i = 1 / sqrt(a_ii) in FP64, off-diagonal
+ *   values (float)((a_ij * s_i) * s_j) and the unit diagonal implicit
+ *   (y_i = x_i + sum of the off-diagonal products), i.e. Jacobi-PCG applied
+ *   as a symmetric scaling; the inner residual is (float)(s_i * r_i) and the
+ *   correction x += s_i * y_i.  Polynomial: z = p(A_hat) r_hat (the matrix
+ *   p(D^-1 A) D^-1 in scaled form).
+ * FP32 row sums in stored order, FP64 dots of exact float products in the
+ * kernels' tree orders, alpha and beta in FP64 rounded to FP32 for the
+ * updates, x64 and r = k - A x64 in FP64 at every reliable update, stop on
+ * the FP64 true relative residual.  The reliable-update trigger compares
+ * the inner residual with its value at the first update; an update whose
+ * true residual fell by more than 1 / ORC_RESET_RATIO since the previous
+ * one also restarts the search direction.  CRS only.
  * x holds x0 on entry and the solution on return.  History (nullable,
- * max_iter + 1 entries) as for orc_pcg, of the inner recursion. */
+ * max_iter + 1 entries) as for orc_pcg, of the inner recursion.
+ * Returns 0, -1 (not CRS), -2 (bad polynomial), -3 (a row without a
+ * positive diagonal, scaled operator). */
 typedef struct {
   int iterations;
   int updates;
@@ -156,23 +171,26 @@ int orc_mixed_pcg(const orc_problem *p, const double *k, double *x,
                   double *h_rho, double *h_pq, double *h_alpha,
                   double *h_beta);
 
-/* Polynomial preconditioner of the mixed solver (precond 2): z = p(M) D^-1 r,
- * M = D^-1 A, p(l) = sum_j c[j] T_j(2 l / b - 1), FP32 coefficients and
- * two_over_b = (float)(2 / b) as the library rounds them.  C (the Jacobi
- * array) holds 1 / A_ii.  The rho = <r, z> sums are in the SpMV row-block
- * tree order (z comes out of the last Clenshaw SpMV). */
+/* Polynomial preconditioner of the mixed solver (precond 2):
+ * p(l) = sum_j c[j] T_j(2 l / b - 1), FP32 coefficients and
+ * two_over_b = (float)(2 / b) as the library rounds them.  The rho = <r, z>
+ * sums are in the SpMV row-block tree order (z comes out of the last
+ * Clenshaw SpMV). */
 #define ORC_POLY_MAX 16
+/* A reliable update whose true residual fell below this fraction of the
+ * previous one restarts the search direction (p = z). */
+#define ORC_RESET_RATIO 1e-3
 typedef struct {
   int degree;
   float c[ORC_POLY_MAX + 1];
   float two_over_b;
 } orc_poly;
 
-/* z = p(D^-1 A) D^-1 r in FP32 with the library's expression order
- * (values and C rounded to float as the library does). */
+/* z = p(A_hat) r in FP32 on the scaled operator A_hat = D^-1/2 A D^-1/2
+ * (built as in orc_mixed_pcg_poly), with the library's expression order. */
 int orc_poly_apply(int N, const int *ptr, const int *index,
-                   const double *value, const double *C, const float *r32,
-                   const orc_poly *pl, float *z);
+                   const double *value, const float *r32, const orc_poly *pl,
+                   float *z);
 
 int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
                        const double *k, double *x, double delta,

@@ -96,7 +96,8 @@ def lib() -> C.CDLL:
     return _lib
 
 
-GEOMETRY = {"block": 256, "vec_gmax": 2048, "rows_max": 256, "nnz_cap": 2048, "spmv_gmax": 2048}
+GEOMETRY = {"block": 256, "vec_gmax": 2048, "rows_max": 256, "nnz_cap": 2048, "spmv_gmax": 2048,
+            "ell_gmax": 8192}
 
 
 class MMFile:
@@ -228,8 +229,10 @@ def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max
               delta=0.1, history=True, poly=None):
     """The oracle's FP32-inner / FP64 reliable-update PCG (CRS).
 
-    precond "poly" needs poly=(coeffs, b) (Chebyshev coefficients on [0, b])
-    and C_jac.  Returns (x, info dict, history dict)."""
+    precond "jacobi" and "poly" run on the scaled operator
+    D^-1/2 A D^-1/2 built from the matrix's own diagonal (C_jac is not
+    used); "poly" needs poly=(coeffs, b) (Chebyshev coefficients on [0, b]).
+    Returns (x, info dict, history dict)."""
     L = lib()
     if not hasattr(L, "_mixed_bound"):
         L.orc_mixed_pcg_poly.argtypes = [C.POINTER(Problem), C.c_void_p, _f64p, _f64p, C.c_double,
@@ -251,10 +254,6 @@ def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max
     p.ptr, p.index, p.value = ptr.ctypes.data, idx.ctypes.data, val.ctypes.data
     p.nz = int(ptr[-1])
     keep = [ptr, idx, val]
-    if precond in ("jacobi", "poly"):
-        cj = np.ascontiguousarray(C_jac, dtype=np.float64)
-        keep.append(cj)
-        p.C = cj.ctypes.data
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
     k = np.ascontiguousarray(k, dtype=np.float64)
     hist = {key: np.full(max_iter + 1, np.nan) for key in ("r2", "rho", "pq", "alpha", "beta")}
@@ -272,21 +271,20 @@ def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max
     return x, info, h
 
 
-def poly_apply(crs, C_jac, r32, coeffs, b):
-    """z = p(D^-1 A) D^-1 r in FP32, the library's expression order."""
+def poly_apply(crs, r32, coeffs, b):
+    """z = p(A_hat) r in FP32 on the scaled operator A_hat = D^-1/2 A D^-1/2,
+    the library's expression order."""
     L = lib()
     ptr = np.ascontiguousarray(crs[0], dtype=np.int32)
     idx = np.ascontiguousarray(crs[1], dtype=np.int32)
     val = np.ascontiguousarray(crs[2], dtype=np.float64)
     n = ptr.size - 1
-    cj = np.ascontiguousarray(C_jac, dtype=np.float64)
     r = np.ascontiguousarray(r32, dtype=np.float32)
     z = np.zeros(n, dtype=np.float32)
     pl = make_poly(coeffs, b)
-    L.orc_poly_apply.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                 C.c_void_p, C.c_void_p, C.c_void_p]
-    rc = L.orc_poly_apply(n, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, cj.ctypes.data,
-                          r.ctypes.data, C.cast(C.byref(pl), C.c_void_p), z.ctypes.data)
+    L.orc_poly_apply.argtypes = [C.c_int] + [C.c_void_p] * 6
+    rc = L.orc_poly_apply(n, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, r.ctypes.data,
+                          C.cast(C.byref(pl), C.c_void_p), z.ctypes.data)
     if rc:
         raise ValueError(f"orc_poly_apply failed with code {rc}")
     return z

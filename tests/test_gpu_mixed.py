@@ -1,4 +1,5 @@
-"""The FP32-inner / FP64 reliable-update solver on the GPU.
+"""The FP32-inner / FP64 reliable-update solver on the GPU (Jacobi as the
+scaled operator D^-1/2 A D^-1/2, no preconditioner on the FP32 copy of A).
 
 No reference counterpart (the reference is FP64 throughout), so it is pinned
 by (1) bit-exactness against the oracle's kernel-by-kernel restatement with
@@ -36,7 +37,10 @@ def _setup(ctx, ptr, idx, val, k, C):
     ctx.upload("X", np.zeros(ptr.size - 1))
 
 
-CASES = ["lap2d_16", "lap2d_64", "rand_sym_40", "rand_gen_300", "diff3d_24", "lap3d_16", "lap2d_256"]
+# the scaled operator is stored as ELL for all but zipf_3000 (rows up to 202
+# entries: CRS row blocks), so both layouts and dot trees are covered
+CASES = ["lap2d_16", "lap2d_64", "rand_sym_40", "rand_gen_300", "diff3d_24", "lap3d_16", "lap2d_256",
+         "zipf_3000"]
 
 
 @pytest.mark.parametrize("precond", ["jacobi", "none"])

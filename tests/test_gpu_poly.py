@@ -41,7 +41,7 @@ def _coeffs(ptr, idx, val, degree, ratio=100.0):
 
 
 @pytest.mark.parametrize("degree", [1, 2, 3, 4, 8, 16])
-@pytest.mark.parametrize("name", ["lap2d_64", "diff3d_24", "rand_gen_300"])
+@pytest.mark.parametrize("name", ["lap2d_64", "diff3d_24", "rand_gen_300", "zipf_3000"])
 def test_poly_bitwise_vs_oracle(ctx, name, degree, tmp_path):
     fx = [f for f in load_golden() if f["case"] == name][0]
     case = Case(fx, str(tmp_path))

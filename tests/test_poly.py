@@ -8,8 +8,10 @@ Pins (none of these has a reference counterpart):
   - Chebyshev residual polynomial: max |1 - l p(l)| on [a, b] equals the
     closed form 1 / T_{k+1}((b + a) / (b - a)); p > 0 on [0, b].
   - The oracle's FP32 Clenshaw application against the dense FP64 operator
-    p(D^-1 A) D^-1 built by the three-term recurrence T_{j+1} = 2 Y T_j - T_{j-1}
-    (brute force), and its symmetry <B u, v> = <u, B v>.
+    p(A_hat), A_hat = D^-1/2 A D^-1/2 (the preconditioner p(D^-1 A) D^-1 in
+    scaled form), built by the three-term recurrence
+    T_{j+1} = 2 Y T_j - T_{j-1} (brute force), and its symmetry
+    <B u, v> = <u, B v>.
   - The oracle's mixed solver with the polynomial reaches the FP64 true
     residual and x = 1.
 """
@@ -39,16 +41,20 @@ def dense(m: CRS) -> np.ndarray:
     return A
 
 
-def dense_poly_operator(A, C, coeffs, b):
-    """p(D^-1 A) D^-1 by the Chebyshev three-term recurrence, FP64."""
+def dense_poly_operator(A, coeffs, b):
+    """p(A_hat), A_hat = D^-1/2 A D^-1/2, by the Chebyshev three-term
+    recurrence T_{j+1} = 2 Y T_j - T_{j-1} with Y = 2 A_hat / b - I, FP64.
+    (p(A_hat) = D^1/2 p(D^-1 A) D^-1 D^1/2: the preconditioner in scaled form.)"""
     n = A.shape[0]
-    Y = 2.0 * (np.diag(C) @ A) / b - np.eye(n)
+    s = 1.0 / np.sqrt(np.diag(A))
+    Ah = s[:, None] * A * s[None, :]
+    Y = 2.0 * Ah / b - np.eye(n)
     T0, T1 = np.eye(n), Y
     P = coeffs[0] * T0 + coeffs[1] * T1
     for j in range(2, len(coeffs)):
         T0, T1 = T1, 2.0 * Y @ T1 - T0
         P = P + coeffs[j] * T1
-    return P @ np.diag(C)
+    return P
 
 
 def test_ritz_closed_form_laplace1d():
@@ -102,37 +108,43 @@ def test_gershgorin_bound():
 @pytest.mark.parametrize("name", ["lap2d", "spd"])
 def test_oracle_poly_apply_brute_force(name, degree):
     m = problems.laplace2d(7) if name == "lap2d" else problems.random_spd_dense(40, 0.3, 11)
-    C = 1.0 / m.diagonal()
     b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
     c = spectrum.cheb_residual_coeffs(b / 50, b, degree)
     c32 = c.astype(np.float32).astype(np.float64)
     b32 = 2.0 / float(np.float32(2.0 / b))
-    B = dense_poly_operator(dense(m), C, c32, b32)
+    B = dense_poly_operator(dense(m), c32, b32)
+    crs = (m.ptr, m.index, m.value)
     rng = np.random.default_rng(degree)
     for _ in range(3):
         r = rng.standard_normal(m.n).astype(np.float32)
-        z = oracle.poly_apply((m.ptr, m.index, m.value), C, r, c, b)
+        z = oracle.poly_apply(crs, r, c, b)
         zr = B @ r.astype(np.float64)
         # FP32 Clenshaw: error grows with the degree and with sum |c_j|
-        tol = 2e-6 * (degree + 1) * np.abs(c32).sum() * np.abs(C).max() * np.abs(r).max()
+        tol = 2e-6 * (degree + 1) * np.abs(c32).sum() * np.abs(r).max()
         assert np.abs(z - zr).max() <= tol
     # symmetry of the operator (FP32 application)
     u = rng.standard_normal(m.n).astype(np.float32)
     v = rng.standard_normal(m.n).astype(np.float32)
-    bu = oracle.poly_apply((m.ptr, m.index, m.value), C, u, c, b).astype(np.float64)
-    bv = oracle.poly_apply((m.ptr, m.index, m.value), C, v, c, b).astype(np.float64)
-    lhs, rhs = bu @ v, u @ bv
-    assert abs(lhs - rhs) <= 1e-5 * np.abs(B).max() * m.n
+    bu = oracle.poly_apply(crs, u, c, b).astype(np.float64)
+    bv = oracle.poly_apply(crs, v, c, b).astype(np.float64)
+    assert abs(bu @ v - u @ bv) <= 1e-5 * np.abs(B).max() * m.n
     np.testing.assert_allclose(B, B.T, rtol=0, atol=1e-10 * np.abs(B).max())
 
 
-def test_oracle_poly_is_textbook_jacobi_for_constant():
-    """degree 1 with c = (c0, 0): z = c0 D^-1 r exactly (the Jacobi step)."""
+def test_oracle_poly_constant_is_identity_step():
+    """degree 1 with c = (1, 0): z = r exactly (the Jacobi step in scaled form)."""
     m = problems.laplace2d(6)
-    C = 1.0 / m.diagonal()
     r = np.random.default_rng(0).standard_normal(m.n).astype(np.float32)
-    z = oracle.poly_apply((m.ptr, m.index, m.value), C, r, [1.0, 0.0], 2.0)
-    np.testing.assert_array_equal(z, np.float32(1.0) * (C.astype(np.float32) * r))
+    z = oracle.poly_apply((m.ptr, m.index, m.value), r, [1.0, 0.0], 2.0)
+    np.testing.assert_array_equal(z, r)
+
+
+def test_oracle_scaled_operator_rejects_missing_diagonal():
+    ptr = np.array([0, 2, 3], np.int32)
+    idx = np.array([0, 1, 0], np.int32)
+    val = np.array([2.0, -1.0, -1.0])
+    with pytest.raises(ValueError, match="-3"):
+        oracle.mixed_pcg(n=2, crs=(ptr, idx, val), k=np.ones(2), precond="jacobi")
 
 
 @pytest.mark.parametrize("degree", [2, 8])
