@@ -48,14 +48,15 @@ __device__ inline double fold_partials(const double *partial, int G, double *lds
 // ---------------------------------------------------------------------------
 // SpMV.
 
-// CSR-stream of row block rb.  Returns the thread's product x[r] * y[r] for
-// the row it owns (0 if none) when WITH_DOT.
-// CSR-stream of row block rb with an epilogue: the thread owning row r calls
-// epi(r, s) with the row's sum s (products added left to right from 0) and
-// gets back its contribution to a fused dot (0 for threads without a row).
-template <typename V, typename X, typename Epi>
-__device__ inline double crs_rowblock_epi(const Crs<V> &A, const X *__restrict__ x, int rb,
-                                          X *__restrict__ prod, Epi epi) {
+// CSR-stream of row block rb with an epilogue: the thread owning row r
+// calls pv = pre(r) as soon as it knows its row (before the products, so
+// the epilogue's own loads are in flight with the matrix stream), then
+// epi(r, s, pv) with the row's sum s (products added left to right from 0)
+// and gets back its contribution to a fused dot (0 for threads without a
+// row).
+template <typename V, typename X, typename Pre, typename Epi>
+__device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restrict__ x, int rb,
+                                              X *__restrict__ prod, Pre pre, Epi epi) {
   const int t = threadIdx.x;
   const int r0 = A.rb_start[rb], r1 = A.rb_start[rb + 1];
   const int e0 = A.ptr[r0], e1 = A.ptr[r1];
@@ -74,9 +75,11 @@ __device__ inline double crs_rowblock_epi(const Crs<V> &A, const X *__restrict__
       }
     }
     int rb0 = 0, re0 = 0;
+    decltype(pre(0)) pv{};
     if (own) {
       rb0 = A.ptr[r];
       re0 = A.ptr[r + 1];
+      pv = pre(r);
     }
 #pragma unroll
     for (int u = 0; u < kRounds; u++) {
@@ -87,12 +90,14 @@ __device__ inline double crs_rowblock_epi(const Crs<V> &A, const X *__restrict__
     if (own) {
       X s = X(0);
       for (int j = rb0; j < re0; j++) s += prod[j - e0];
-      dp = epi(r, s);
+      dp = epi(r, s, pv);
     }
     __syncthreads();
   } else {
     // One row longer than kNnzCap: chunks through LDS, thread 0 adds in order.
     X s = X(0);
+    decltype(pre(0)) pv{};
+    if (t == 0) pv = pre(r0);
     for (int c0 = e0; c0 < e1; c0 += kNnzCap) {
       const int ce = min(c0 + kNnzCap, e1);
       for (int e = c0 + t; e < ce; e += kBlock) prod[e - c0] = X(A.value[e]) * x[A.index[e]];
@@ -101,9 +106,17 @@ __device__ inline double crs_rowblock_epi(const Crs<V> &A, const X *__restrict__
         for (int j = 0; j < ce - c0; j++) s += prod[j];
       __syncthreads();
     }
-    if (t == 0) dp = epi(r0, s);
+    if (t == 0) dp = epi(r0, s, pv);
   }
   return dp;
+}
+
+// The same without early loads: epi(r, s).
+template <typename V, typename X, typename Epi>
+__device__ inline double crs_rowblock_epi(const Crs<V> &A, const X *__restrict__ x, int rb,
+                                          X *__restrict__ prod, Epi epi) {
+  return crs_rowblock_pre_epi<V, X>(
+      A, x, rb, prod, [](int) { return 0; }, [&](int r, X s, int) { return epi(r, s); });
 }
 
 // y = A x over row block rb; with WITH_DOT, returns the owner's x[r] * y[r]
@@ -111,20 +124,25 @@ __device__ inline double crs_rowblock_epi(const Crs<V> &A, const X *__restrict__
 template <typename V, typename X, bool WITH_DOT>
 __device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, X *__restrict__ y,
                                       int rb, X *__restrict__ prod) {
-  return crs_rowblock_epi<V, X>(A, x, rb, prod, [&](int r, X s) -> double {
-    y[r] = s;
-    return WITH_DOT ? (double)x[r] * (double)s : 0.0;
-  });
+  return crs_rowblock_pre_epi<V, X>(
+      A, x, rb, prod, [&](int r) { return WITH_DOT ? x[r] : X(0); },
+      [&](int r, X s, X xr) -> double {
+        y[r] = s;
+        return WITH_DOT ? (double)xr * (double)s : 0.0;
+      });
 }
 
-// ELL row r (column-major slots j * n + r), the thread adds its live slots
-// left to right from 0 and calls epi(r, s).  The matrix streams (length,
-// index, data) are read once, so they are loaded non-temporally and leave
-// the caches to the gathered x.
-template <typename V, typename X, typename Epi>
-__device__ inline double ell_row_epi(const Ell<V> &A, const X *__restrict__ x, int r, Epi epi) {
+// ELL row r (column-major slots j * n + r): pv = pre(r) first (the
+// epilogue's own loads in flight with the slots), the thread adds its live
+// slots left to right from 0 and calls epi(r, s, pv).  The matrix streams
+// (length, index, data) are read once, so they are loaded non-temporally
+// and leave the caches to the gathered x.
+template <typename V, typename X, typename Pre, typename Epi>
+__device__ inline double ell_row_pre_epi(const Ell<V> &A, const X *__restrict__ x, int r, Pre pre,
+                                         Epi epi) {
   const int64_t n = A.n;
   const int len = __builtin_nontemporal_load(A.length + r);
+  const auto pv = pre(r);
   X s = X(0);
   for (int j0 = 0; j0 < len; j0 += kEllChunk) {
     int c[kEllChunk];
@@ -141,7 +159,13 @@ __device__ inline double ell_row_epi(const Ell<V> &A, const X *__restrict__ x, i
     for (int u = 0; u < kEllChunk; u++)
       if (j0 + u < len) s += X(v[u]) * x[c[u]];
   }
-  return epi(r, s);
+  return epi(r, s, pv);
+}
+
+template <typename V, typename X, typename Epi>
+__device__ inline double ell_row_epi(const Ell<V> &A, const X *__restrict__ x, int r, Epi epi) {
+  return ell_row_pre_epi<V, X>(
+      A, x, r, [](int) { return 0; }, [&](int i, X s, int) { return epi(i, s); });
 }
 
 // ELL rows rb*256 + t: y = A x; with WITH_DOT, returns the owner's
@@ -151,10 +175,12 @@ __device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, 
                                       int rb) {
   const int r = rb * kBlock + (int)threadIdx.x;
   if (r >= A.n) return 0.0;
-  return ell_row_epi<V, X>(A, x, r, [&](int i, X s) -> double {
-    y[i] = s;
-    return WITH_DOT ? (double)x[i] * (double)s : 0.0;
-  });
+  return ell_row_pre_epi<V, X>(
+      A, x, r, [&](int i) { return WITH_DOT ? x[i] : X(0); },
+      [&](int i, X s, X xi) -> double {
+        y[i] = s;
+        return WITH_DOT ? (double)xi * (double)s : 0.0;
+      });
 }
 
 // ---------------------------------------------------------------------------

@@ -100,21 +100,23 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
   }
 }
 
-// Row-block loop of the FP32 operator: calls epi(i, t) with the sum t of
-// the stored products of row i (from 0, in stored order) for every row the
-// thread owns; returns the sum of the epilogue's dot contributions.
-template <bool ELL, typename Epi>
-__device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Epi epi) {
+// Row-block loop of the FP32 operator: for every row i the thread owns,
+// pv = pre(i) early, then epi(i, t, pv) with the sum t of the stored
+// products of row i (from 0, in stored order); returns the sum of the
+// epilogue's dot contributions.
+template <bool ELL, typename Pre, typename Epi>
+__device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
+                                   Epi epi) {
   double tpart = 0.0;
   if (ELL) {
     const int nrb = ell_chunks(A.ell.n);
     for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
       const int r = rb * kBlock + (int)threadIdx.x;
-      if (r < A.ell.n) tpart += ell_row_epi<float, float>(A.ell, x, r, epi);
+      if (r < A.ell.n) tpart += ell_row_pre_epi<float, float>(A.ell, x, r, pre, epi);
     }
   } else {
     for (int rb = blockIdx.x; rb < A.crs.nrb; rb += gridDim.x)
-      tpart += crs_rowblock_epi<float, float>(A.crs, x, rb, prod, epi);
+      tpart += crs_rowblock_pre_epi<float, float>(A.crs, x, rb, prod, pre, epi);
   }
   return tpart;
 }
@@ -228,11 +230,13 @@ __global__ __launch_bounds__(kBlock) void k_mp_matvec_dot(Op32 A, const float *_
   __shared__ double lds[4];
   if (!mp_begin(st)) return;
   const int unit = ELL ? A.ell.unit : A.crs.unit;
-  const double tpart = op32_rows<ELL>(A, p, prod, [&](int i, float t) -> double {
-    const float qi = unit ? p[i] + t : t;
-    q[i] = qi;
-    return (double)p[i] * (double)qi;
-  });
+  const double tpart = op32_rows<ELL>(
+      A, p, prod, [&](int i) { return p[i]; },
+      [&](int i, float t, float pi) -> double {
+        const float qi = unit ? pi + t : t;
+        q[i] = qi;
+        return (double)pi * (double)qi;
+      });
   const double bp = block_fold(tpart, lds);
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
@@ -459,21 +463,21 @@ __global__ __launch_bounds__(kBlock) void k_poly_step(Op32 A, const float *__res
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
   const int unit = ELL ? A.ell.unit : A.crs.unit;
-  const double tpart = op32_rows<ELL>(A, b1, prod, [&](int i, float t0) -> double {
-    const float bi = b1[i];
-    const float t = unit ? bi + t0 : t0;
-    const float sv = tb * t - bi;
-    const float bb = b2 ? b2[i] : 0.0f;
-    const float ri = r[i];
-    if (FINAL) {
-      const float z = (cj * ri + sv) - bb;
-      b0[i] = z;
-      if (pcopy) pcopy[i] = z;
-      return (double)ri * (double)z;
-    }
-    b0[i] = (cj * ri + 2.0f * sv) - bb;
-    return 0.0;
-  });
+  const double tpart = op32_rows<ELL>(
+      A, b1, prod, [&](int i) { return float4{b1[i], r[i], b2 ? b2[i] : 0.0f, 0.0f}; },
+      [&](int i, float t0, float4 pv) -> double {
+        const float bi = pv.x, ri = pv.y, bb = pv.z;
+        const float t = unit ? bi + t0 : t0;
+        const float sv = tb * t - bi;
+        if (FINAL) {
+          const float z = (cj * ri + sv) - bb;
+          b0[i] = z;
+          if (pcopy) pcopy[i] = z;
+          return (double)ri * (double)z;
+        }
+        b0[i] = (cj * ri + 2.0f * sv) - bb;
+        return 0.0;
+      });
   if (FINAL) {
     const double bp = block_fold(tpart, lds);
     if (threadIdx.x == 0) prz.a[blockIdx.x] = bp;

@@ -11,10 +11,13 @@ Features (42):
   37..41  from the first 10 steps of FP64 Jacobi-PCG (the reference's loop,
           CG.cpp:284-385): log of the smallest and largest Ritz values after
           5 and 10 steps, and log10(r2_10 / r2_0) / 10
-Outputs (9): log p at the 9 Chebyshev points of [0, b], b = 1 + max Gershgorin
-radius (an upper bound of the spectrum of D^-1 A).  The coefficients follow
-by the discrete Chebyshev transform, so p > 0 at those points by
-construction; the library checks p > 0 on all of [0, b] before use.
+Outputs (11): log p at the 9 Chebyshev points of [0, b], b = 1 + max
+Gershgorin radius (an upper bound of the spectrum of D^-1 A), then the log of
+the iteration counts predicted for Jacobi-PCG and for the polynomial PCG.
+The coefficients follow by the discrete Chebyshev transform, so p > 0 at
+those points by construction; the library checks p > 0 on all of [0, b]
+before use.  The iteration predictions, with the per-iteration costs timed
+on the device, decide whether the polynomial is used at all.
 
 Labels come from the spectrum the FP64 solver itself reveals: the Lanczos
 tridiagonal of a converged Jacobi-PCG run (its alpha/beta) gives Ritz nodes
@@ -38,7 +41,9 @@ from . import spectrum
 DEGREE = 8
 N_STATS = 36
 N_FEATURES = N_STATS + 1 + 5
-DIMS = (N_FEATURES, 128, 128, DEGREE + 1)
+# outputs: log p at the DEGREE + 1 Chebyshev points, then the log of the
+# predicted iteration counts of Jacobi-PCG and of the polynomial PCG
+DIMS = (N_FEATURES, 128, 128, DEGREE + 3)
 RITZ_STEPS = 10
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "poly_mlp.npz")
 # lower ends a = theta_1 * 2^(s/2), s = 0..A_STEPS-1, theta_1 = smallest Ritz value
@@ -209,14 +214,58 @@ def load_model(path=MODEL_PATH) -> dict:
 
 
 def normalize(f, model) -> np.ndarray:
-    return ((np.atleast_2d(f) - model["feat_mean"]) / model["feat_std"]).astype(np.float32)
+    """Standardised features, first clamped to the range seen in training
+    (a system outside the family, e.g. a larger grid, is mapped to its
+    nearest training edge instead of being extrapolated)."""
+    f = np.atleast_2d(np.asarray(f, np.float64))
+    if "feat_lo" in model:
+        f = np.clip(f, model["feat_lo"], model["feat_hi"])
+    return ((f - model["feat_mean"]) / model["feat_std"]).astype(np.float32)
+
+
+def predict(mlp, model, f, b: float) -> dict:
+    """Network output for features f (on the GPU): the coefficients on
+    [0, b] and, for a model with 11 outputs, the predicted Jacobi and
+    polynomial iteration counts."""
+    yn = mlp.forward(normalize(f, model))[0].astype(np.float64)
+    y = yn * model["out_std"] + model["out_mean"]
+    k = int(model["degree"])
+    out = {"coeffs": decode_coeffs(y[: k + 1], b)}
+    if y.size >= k + 3:
+        out["it_jacobi"] = float(np.exp(y[k + 1]))
+        out["it_poly"] = float(np.exp(y[k + 2]))
+    return out
 
 
 def predict_coeffs(mlp, model, f, b: float) -> np.ndarray:
     """Coefficients on [0, b] from features f, with the network on the GPU."""
-    yn = mlp.forward(normalize(f, model))[0].astype(np.float64)
-    y = yn * model["out_std"] + model["out_mean"]
-    return decode_coeffs(y, b)
+    return predict(mlp, model, f, b)["coeffs"]
+
+
+def iteration_costs(ctx, coeffs, b: float) -> tuple[float, float] | None:
+    """Device time (ms) of one mixed-precision iteration with Jacobi and with
+    the polynomial on the uploaded system, from two short kernel-timed
+    solves; cached on the context per system and degree.  None if the
+    library refuses the polynomial."""
+    from .cghip import CGHIPError
+
+    key = (ctx.n, len(coeffs) - 1)
+    cache = getattr(ctx, "_iteration_costs", {})
+    if key in cache:
+        return cache[key]
+    try:
+        ctx.set_poly(coeffs, b)
+    except CGHIPError:
+        return None
+    ctx.fill("X", 0.0)
+    j = ctx.solve_mixed(precond="jacobi", max_iter=3, tol=1e-300, profile=1)
+    ctx.fill("X", 0.0)
+    p = ctx.solve_mixed(precond="poly", max_iter=2, tol=1e-300, profile=1)
+    tj = (j["ms_matvec"] + j["ms_update"] + j["ms_xpay"]) / max(j["n_matvec"], 1)
+    tp = (p["ms_matvec"] + p["ms_update"] + p["ms_poly"] + p["ms_xpay"]) / max(p["n_matvec"], 1)
+    cache[key] = (tj, tp)
+    ctx._iteration_costs = cache
+    return cache[key]
 
 
 def fixed_rule_coeffs(theta10: float, b: float, rho: float) -> np.ndarray:
@@ -271,13 +320,41 @@ def solve_poly_with_fallback(ctx, coeffs, b: float, tol: float = 1e-9, max_iter:
 
 def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 100000,
                   budget: int | None = None) -> dict:
-    """Features on the device, coefficients from the network (on the GPU),
-    then solve_poly_with_fallback.  The wall time includes the features."""
+    """Features on the device, the network on the GPU, then the solve.
+
+    With iteration predictions (11-output model) the predicted costs decide:
+    the polynomial runs only if (predicted polynomial iterations) x (its
+    per-iteration time) is below the same product for Jacobi, and then with
+    an SpMV budget worth 1.25x Jacobi's predicted time; past it, or if the
+    library refuses the polynomial, Jacobi takes over
+    (solve_poly_with_fallback).  Otherwise Jacobi runs from the start.
+    The per-iteration times are timed once per system and context
+    (iteration_costs).  The wall time includes features and inference."""
     t0 = time.perf_counter()
     f, b, _ = system_features(ctx, n)
-    c = predict_coeffs(mlp, model, f, b)
-    res = solve_poly_with_fallback(ctx, c, b, tol, max_iter, budget)
+    pr = predict(mlp, model, f, b)
+    c = pr["coeffs"]
+    k = len(c) - 1
+    decision = None
+    if "it_jacobi" in pr and budget is None:
+        costs = iteration_costs(ctx, c, b)
+        if costs is not None:
+            tj, tp = costs
+            cost_j, cost_p = tj * pr["it_jacobi"], tp * pr["it_poly"]
+            decision = {"it_jacobi_pred": pr["it_jacobi"], "it_poly_pred": pr["it_poly"],
+                        "ms_jacobi_iter": tj, "ms_poly_iter": tp,
+                        "use_poly": cost_p < cost_j}
+            budget = int(np.ceil(1.25 * cost_j / tp)) * (k + 1)
+    if decision is not None and not decision["use_poly"]:
+        ctx.fill("X", 0.0)
+        info = ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=tol)
+        res = {"method": "jacobi-selected", "poly_iterations": 0,
+               "jacobi_iterations": info["iterations"], "updates": info["updates"],
+               "true_residual": info["true_residual"], "spmvs": info["iterations"]}
+    else:
+        res = solve_poly_with_fallback(ctx, c, b, tol, max_iter, budget)
     res["wall_s"] = time.perf_counter() - t0
     res["coeffs"] = c
     res["b"] = b
+    res["decision"] = decision
     return res

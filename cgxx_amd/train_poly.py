@@ -27,6 +27,17 @@ def load(path):
         return {k: z[k] for k in z.files}
 
 
+def targets(d) -> np.ndarray:
+    """log p at the Chebyshev points of the best Chebyshev polynomial, the
+    log of the FP64 Jacobi-PCG iteration count, the log of the best
+    simulated polynomial iteration count."""
+    it = d["sim_iters"].astype(np.float64)
+    it[it < 0] = np.inf
+    return np.concatenate([d["y"].astype(np.float64),
+                           np.log(d["fp64_iterations"].astype(np.float64))[:, None],
+                           np.log(it.min(1))[:, None]], axis=1)
+
+
 def best_rho(d) -> float:
     it = d["rho_iters"].astype(np.float64)
     it[it < 0] = np.inf
@@ -45,10 +56,11 @@ def main() -> int:
     args = ap.parse_args()
 
     parts = [load(p) for p in args.data]
-    d = {k: np.concatenate([p[k] for p in parts]) for k in ("features", "y", "rho_iters")}
+    d = {k: np.concatenate([p[k] for p in parts])
+         for k in ("features", "y", "rho_iters", "fp64_iterations", "sim_iters")}
     d["rho"] = parts[0]["rho"]
     X = d["features"].astype(np.float64)
-    Y = d["y"].astype(np.float64)
+    Y = targets(d)
     fm, fs = X.mean(0), X.std(0)
     fs = np.where(fs < 1e-6, 1.0, fs)
     om, os_ = Y.mean(0), Y.std(0)
@@ -68,20 +80,26 @@ def main() -> int:
     params = mlp.get_params()
     train_s = time.perf_counter() - t0
     pred = mlp.forward(Xn).astype(np.float64) * os_ + om
-    train_rmse = float(np.sqrt(np.mean((pred - Y) ** 2)))
+    train_rmse = float(np.sqrt(np.mean((pred[:, :learned.DEGREE + 1] - Y[:, :learned.DEGREE + 1]) ** 2)))
     summary = {"n_train": int(X.shape[0]), "steps": args.steps, "lr": args.lr,
                "train_seconds": train_s, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                "train_rmse_log_p": train_rmse, "rho_star": best_rho(d)}
     if args.heldout:
         hp = [load(p) for p in args.heldout]
         Xh = np.concatenate([p["features"] for p in hp]).astype(np.float64)
-        Yh = np.concatenate([p["y"] for p in hp]).astype(np.float64)
-        ph = mlp.forward(((Xh - fm) / fs).astype(np.float32)).astype(np.float64) * os_ + om
+        Yh = targets({k: np.concatenate([p[k] for p in hp])
+                      for k in ("y", "fp64_iterations", "sim_iters")})
+        Xc = np.clip(Xh, X.min(0), X.max(0))  # as inference clamps to the training range
+        ph = mlp.forward(((Xc - fm) / fs).astype(np.float32)).astype(np.float64) * os_ + om
         summary["n_heldout"] = int(Xh.shape[0])
-        summary["heldout_rmse_log_p"] = float(np.sqrt(np.mean((ph - Yh) ** 2)))
-        summary["heldout_rmse_mean_predictor"] = float(np.sqrt(np.mean((om - Yh) ** 2)))
+        k = learned.DEGREE + 1
+        summary["heldout_rmse_log_p"] = float(np.sqrt(np.mean((ph[:, :k] - Yh[:, :k]) ** 2)))
+        summary["heldout_rmse_mean_predictor"] = float(np.sqrt(np.mean((om[:k] - Yh[:, :k]) ** 2)))
+        summary["heldout_iterations_rel_err"] = [
+            float(np.mean(np.abs(np.exp(ph[:, k + i] - Yh[:, k + i]) - 1))) for i in (0, 1)]
     mlp.close()
     learned.save_model(args.out, params, fm, fs, om, os_, rho_star=summary["rho_star"],
+                       feat_lo=X.min(0), feat_hi=X.max(0),
                        loss_history=hist[:: max(1, len(hist) // 200)])
     print(json.dumps(summary), flush=True)
     return 0
