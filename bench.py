@@ -287,9 +287,10 @@ def main() -> int:
                         "(log-normal kappa, sigma=1, corr=8, seed 20261202), k=A*1, x0=0, "
                         "FP64 true relative residual 1e-9",
             "method": "mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 (ELL, FP32 values, int32 "
-                      "indices) with a degree-8 polynomial preconditioner whose coefficients come "
-                      "from the trained network, FP64 reliable updates (delta 0.1); features, "
-                      "inference and fallback inside the timed step",
+                      "indices), FP64 reliable updates (delta 0.1); the trained network gives the "
+                      "degree-8 polynomial's coefficients and predicted iteration counts, and the "
+                      "polynomial runs only when its predicted cost beats Jacobi's (else Jacobi); "
+                      "features, inference, choice and fallback inside the timed step",
             "n": n,
             "nnz": nnz,
             "format": "CRS upload, scaled FP32 operator in ELL",

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 
 LIB_SOURCES = ["cghip.hip", "kernels.hip", "kernels_mixed.hip", "cghip_mlp.hip", "kernels_mlp.hip",
                "kernels_features.hip"]
-HOST_SOURCES = ["CG.cpp", "Matrix.cpp", "Preconditioner.cpp", "CGHIP.cpp", "main.cpp"]
+HOST_SOURCES = ["CG.cpp", "Matrix.cpp", "Preconditioner.cpp", "CGHIP.cpp", "LearnedPoly.cpp", "main.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -449,14 +449,18 @@ static inline dim3 vgrid(int n) { return dim3(vec_grid(n)); }
 template <typename V, typename X>
 hipError_t matvec_crs_t(const Crs<V> &A, const X *x, X *y, hipStream_t s) {
   if (A.n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_matvec_crs<V, X>), dim3(spmv_grid(A.nrb)), dim3(kBlock), 0, s, A, x, y);
+  // no reduction: one block per row block (more waves in flight than the
+  // persistent grid of the fused kernels)
+  hipLaunchKernelGGL((k_matvec_crs<V, X>), dim3(A.nrb > 0 ? A.nrb : 1), dim3(kBlock), 0, s, A, x,
+                     y);
   return hipGetLastError();
 }
 
 template <typename V, typename X>
 hipError_t matvec_ell_t(const Ell<V> &A, const X *x, X *y, hipStream_t s) {
   if (A.n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_matvec_ell<V, X>), dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A, x, y);
+  hipLaunchKernelGGL((k_matvec_ell<V, X>), dim3(ell_chunks(A.n) > 0 ? ell_chunks(A.n) : 1),
+                     dim3(kBlock), 0, s, A, x, y);
   return hipGetLastError();
 }
 

@@ -23,13 +23,28 @@
 //                   print as 0).  Default: one iteration at a time with every
 //                   kernel timed by HIP events (one host round trip per
 //                   iteration).
+//   CG_HIP_MIXED    1: FP32-inner CG with FP64 reliable updates (CRS only;
+//                   with Jacobi on the scaled operator D^-1/2 A D^-1/2); the
+//                   reported residual is the FP64 true relative residual.
+//   CG_HIP_POLY     none (default) | learned | fixed: with CG_HIP_MIXED=1 and
+//                   Jacobi, the degree-8 polynomial preconditioner from the
+//                   trained network (with its Jacobi choice and fallback) or
+//                   from the fixed Chebyshev rule.
+//   CG_HIP_MODEL    model file for CG_HIP_POLY (default: ../data/poly_mlp.bin
+//                   next to this executable).
 #include <cerrno>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <string>
 
+#include <limits.h>
+#include <unistd.h>
+
+#include <memory>
+
 #include "CG.h"
+#include "LearnedPoly.h"
 #include "cghip.h"
 
 namespace {
@@ -60,6 +75,12 @@ class CGHIP : public CG {
   bool fused = true;
   bool graph = false;
   int dotMode = CGHIP_DOT_TREE;
+  bool mixed = false;
+  std::string polyMode = "none";
+  std::string modelPath;
+  std::unique_ptr<LearnedPoly> learned;
+
+  bool solveMixed(int maxIterations, floatType tolerance, FusedResult &result);
 
   void chk(int rc, const char *what) {
     if (rc != CGHIP_OK) hip_fail(ctx, what);
@@ -103,6 +124,21 @@ public:
     device = env_int("CG_HIP_DEVICE", 0);
     fused = env_int("CG_HIP_FUSED", 1) != 0;
     graph = env_int("CG_HIP_GRAPH", 0) != 0;
+    mixed = env_int("CG_HIP_MIXED", 0) != 0;
+    if (const char *v = std::getenv("CG_HIP_POLY")) {
+      const std::string s(v);
+      if (s == "none" || s == "learned" || s == "fixed")
+        polyMode = s;
+      else if (!s.empty()) {
+        std::cerr << "Invalid value for CG_HIP_POLY! (none, learned, or fixed)" << std::endl;
+        std::exit(1);
+      }
+    }
+    if (const char *v = std::getenv("CG_HIP_MODEL")) modelPath = v;
+    if (polyMode != "none" && !mixed) {
+      std::cerr << "CG_HIP_POLY needs CG_HIP_MIXED=1!" << std::endl;
+      std::exit(1);
+    }
     if (const char *v = std::getenv("CG_HIP_DOT")) {
       const std::string s(v);
       if (s == "tree")
@@ -139,11 +175,73 @@ void CGHIP::doTransferTo() {
   if (preconditioner == PreconditionerJacobi) chk(cghip_upload_jacobi(ctx, jacobi->C), "upload_jacobi");
   chk(cghip_upload_vector(ctx, CGHIP_VEC_K, k), "upload k");
   chk(cghip_upload_vector(ctx, CGHIP_VEC_X, x), "upload x");
+  if (mixed) {
+    if (matrixFormat != MatrixFormatCRS) {
+      std::cerr << "CG_HIP_MIXED needs CG_MATRIX_FORMAT=CRS!" << std::endl;
+      std::exit(1);
+    }
+    chk(cghip_prepare_fp32(ctx), "prepare_fp32");
+    if (polyMode != "none" && !learned) {
+      if (preconditioner != PreconditionerJacobi) {
+        std::cerr << "CG_HIP_POLY needs CG_PRECONDITIONER=jacobi!" << std::endl;
+        std::exit(1);
+      }
+      std::string path = modelPath;
+      if (path.empty()) {
+        char exe[PATH_MAX];
+        const ssize_t len = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+        std::string dir = len > 0 ? std::string(exe, (size_t)len) : std::string(".");
+        dir = dir.substr(0, dir.find_last_of('/'));
+        path = dir + "/../data/poly_mlp.bin";
+      }
+      PolyModel model;
+      std::string err;
+      if (!model.load(path, err)) {
+        std::cerr << "HIP error: " << err << std::endl;
+        std::exit(1);
+      }
+      learned.reset(new LearnedPoly(ctx, device, model, polyMode));
+    }
+  }
+}
+
+bool CGHIP::solveMixed(int maxIterations, floatType tolerance, FusedResult &result) {
+  if (learned) {
+    LearnedResult r;
+    std::string err;
+    if (!learned->solve(N, tolerance, maxIterations, r, err)) {
+      std::cerr << "HIP error: " << err << std::endl;
+      std::exit(1);
+    }
+    std::cout << "HIP mixed precision, polynomial (" << polyMode << "): " << r.method << ", "
+              << r.polyIterations << " polynomial + " << r.jacobiIterations
+              << " Jacobi inner iterations" << std::endl;
+    result.iterations = r.polyIterations + r.jacobiIterations;
+    result.residual = r.trueResidual;
+    return true;
+  }
+  cghip_mixed_opts o{};
+  o.precond = preconditioner == PreconditionerJacobi ? CGHIP_PRECOND_JACOBI : CGHIP_PRECOND_NONE;
+  o.max_iter = maxIterations;
+  o.tol = tolerance;
+  o.delta = 0.0;
+  o.batch = 0;
+  o.profile = graph ? CGHIP_PROFILE_NONE : CGHIP_PROFILE_KERNELS;
+  cghip_mixed_info info{};
+  chk(cghip_solve_mixed(ctx, &o, &info), "solve_mixed");
+  std::cout << "HIP mixed precision: " << info.updates << " reliable updates" << std::endl;
+  result.iterations = info.iterations;
+  result.residual = info.true_residual;
+  result.matvec = (info.ms_matvec + info.ms_refresh) * 1e-3;
+  result.axpy = info.ms_update * 1e-3;
+  result.xpay = info.ms_xpay * 1e-3;
+  return true;
 }
 
 void CGHIP::doTransferFrom() { chk(cghip_download_vector(ctx, CGHIP_VEC_X, x), "download x"); }
 
 bool CGHIP::solveFused(int maxIterations, floatType tolerance, FusedResult &result) {
+  if (mixed) return solveMixed(maxIterations, tolerance, result);
   if (!fused) return false;
   cghip_solve_opts o{};
   o.precond = preconditioner == PreconditionerJacobi ? CGHIP_PRECOND_JACOBI : CGHIP_PRECOND_NONE;

@@ -1,0 +1,64 @@
+// Mixed-precision solve with the learned polynomial preconditioner, driven
+// from C++ through the C ABI (include/cghip.h, include/cghip_mlp.h).
+//
+// The same sequence as cgxx_amd/learned.py (solve_learned): device matrix
+// statistics and 10 FP64 Jacobi-PCG steps for the 42 features, the network
+// on the GPU for log p at the Chebyshev points of [0, b] and the predicted
+// iteration counts, the discrete Chebyshev transform to coefficients, a
+// choice between the polynomial and Jacobi from the predicted iterations
+// times the per-iteration times (timed once on the device), and the Jacobi
+// fallback judged by the FP64 true residual.  No reference counterpart.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "cghip.h"
+#include "cghip_mlp.h"
+
+struct PolyModel {
+  int in = 0, h1 = 0, h2 = 0, out = 0, degree = 0;
+  double rho_star = 16.0;
+  std::vector<double> feat_mean, feat_std, feat_lo, feat_hi, out_mean, out_std;
+  std::vector<float> params;
+  // Reads the binary written by cgxx_amd.learned.export_bin; false + err on
+  // a malformed file.
+  bool load(const std::string &path, std::string &err);
+};
+
+struct LearnedResult {
+  std::string method;      // poly, poly+jacobi, jacobi (refused), jacobi-selected
+  int polyIterations = 0;  // inner iterations with the polynomial
+  int jacobiIterations = 0;
+  double trueResidual = 0;  // FP64 ||k - A x|| / ||k||
+  double msMatvec = 0, msUpdate = 0, msXpay = 0;
+};
+
+class LearnedPoly {
+ public:
+  // mode "learned": network coefficients and cost choice; "fixed":
+  // Chebyshev on [theta_min(10 steps) / rho*, b], always the polynomial.
+  LearnedPoly(cghip_ctx *ctx, int device, const PolyModel &model, const std::string &mode);
+  ~LearnedPoly();
+  // Solves from X = 0 (the context holds the CRS matrix, the Jacobi
+  // diagonal, K, and the FP32 data of cghip_prepare_fp32).  Returns false
+  // with err set when a library call fails.
+  bool solve(int n, double tol, int maxIter, LearnedResult &res, std::string &err);
+
+ private:
+  cghip_ctx *ctx_;
+  int device_;
+  PolyModel model_;
+  std::string mode_;
+  cghip_mlp *mlp_ = nullptr;
+  bool costsKnown_ = false;
+  double msJacobiIter_ = 0, msPolyIter_ = 0;
+};
+
+// Extreme eigenvalues of the Lanczos tridiagonal of m PCG steps
+// (T_jj = 1/a_j + b_{j-1}/a_{j-1}, T_j,j+1 = sqrt(b_j)/a_j), by Sturm-sequence
+// bisection.
+void ritzExtremes(const double *alpha, const double *beta, int m, double &lo, double &hi);
+// Chebyshev coefficients on [0, b] of the residual polynomial of degree + 1
+// on [a, b] (the Chebyshev preconditioner).
+std::vector<double> chebResidualCoeffs(double a, double b, int degree);

@@ -358,3 +358,24 @@ def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 10
     res["b"] = b
     res["decision"] = decision
     return res
+
+
+BIN_MAGIC = b"CGHIPMLP"
+
+
+def export_bin(model: dict, path: str) -> None:
+    """Writes the model for the C++ host (cgxx_amd/host/LearnedPoly.cpp), all
+    little-endian: magic "CGHIPMLP", int32 version 1, int32 in, h1, h2, out,
+    degree, float64 rho_star, float64 feat_mean[in], feat_std[in],
+    feat_lo[in], feat_hi[in], out_mean[out], out_std[out], int32 nparams,
+    float32 params[nparams]."""
+    dims = [int(v) for v in model["dims"]]
+    with open(path, "wb") as f:
+        f.write(BIN_MAGIC)
+        np.array([1, *dims, int(model["degree"])], "<i4").tofile(f)
+        np.array([float(model["rho_star"])], "<f8").tofile(f)
+        for key in ("feat_mean", "feat_std", "feat_lo", "feat_hi", "out_mean", "out_std"):
+            np.asarray(model[key], "<f8").tofile(f)
+        params = np.asarray(model["params"], "<f4")
+        np.array([params.size], "<i4").tofile(f)
+        params.tofile(f)

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -101,6 +102,7 @@ def main() -> int:
     learned.save_model(args.out, params, fm, fs, om, os_, rho_star=summary["rho_star"],
                        feat_lo=X.min(0), feat_hi=X.max(0),
                        loss_history=hist[:: max(1, len(hist) // 200)])
+    learned.export_bin(learned.load_model(args.out), os.path.splitext(args.out)[0] + ".bin")
     print(json.dumps(summary), flush=True)
     return 0
 

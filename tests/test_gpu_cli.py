@@ -95,3 +95,32 @@ def test_repetitions_and_strict_check(mtx):
     assert bad.returncode == 2 and "elements are incorrect!" in bad.stdout
     lenient = run(EXE, mtx["lap2d_256"], CG_MAX_ITER=5)
     assert lenient.returncode == 0 and "elements are incorrect!" in lenient.stdout
+
+
+@pytest.mark.parametrize("poly", ["none", "learned", "fixed"])
+@pytest.mark.parametrize("name", ["lap2d_256", "diff3d_24"])
+def test_mixed_precision_cli(mtx, name, poly):
+    """CG_HIP_MIXED=1: the FP32-inner solver reaches the FP64 true residual
+    (the printed residual) and passes the reference's check; CG_HIP_POLY adds
+    the polynomial preconditioner (learned: the committed network)."""
+    env = dict(CG_MATRIX_FORMAT="CRS", CG_PRECONDITIONER="jacobi", CG_MAX_ITER=20000,
+               CG_HIP_MIXED=1, CG_HIP_POLY=poly, CG_STRICT_CHECK=1)
+    ours = run(EXE, mtx[name], **env)
+    assert ours.returncode == 0, ours.stderr + ours.stdout
+    assert "Solution is correct!" in ours.stdout
+    res = [ln.split(":", 1)[1].strip() for ln in ours.stdout.splitlines() if ln.startswith("Residual")]
+    assert float(res[0]) <= 1e-9
+    if poly != "none":
+        assert "HIP mixed precision, polynomial (" + poly + ")" in ours.stdout
+
+
+def test_mixed_precision_cli_rejects(mtx):
+    ell = run(EXE, mtx["lap2d_256"], CG_MATRIX_FORMAT="ELL", CG_HIP_MIXED=1)
+    assert ell.returncode == 1 and "CG_HIP_MIXED needs CG_MATRIX_FORMAT=CRS" in ell.stderr
+    nomix = run(EXE, mtx["lap2d_256"], CG_HIP_POLY="learned")
+    assert nomix.returncode == 1 and "needs CG_HIP_MIXED=1" in nomix.stderr
+    bad = run(EXE, mtx["lap2d_256"], CG_HIP_MIXED=1, CG_HIP_POLY="cubic")
+    assert bad.returncode == 1 and "Invalid value for CG_HIP_POLY" in bad.stderr
+    nomodel = run(EXE, mtx["lap2d_256"], CG_MATRIX_FORMAT="CRS", CG_HIP_MIXED=1,
+                  CG_HIP_POLY="learned", CG_HIP_MODEL="/nonexistent/model.bin")
+    assert nomodel.returncode == 1 and "cannot open model file" in nomodel.stderr
