@@ -27,7 +27,7 @@ SYMBOLS = [
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
-    "cghip_set_poly", "cghip_matrix_stats",
+    "cghip_set_poly", "cghip_matrix_stats", "cghip_time_poly_step",
     "cghip_mlp_param_count", "cghip_mlp_create", "cghip_mlp_destroy", "cghip_mlp_last_error",
     "cghip_mlp_set_params", "cghip_mlp_get_params", "cghip_mlp_forward", "cghip_mlp_gradient",
     "cghip_mlp_train",
@@ -115,6 +115,7 @@ def load() -> C.CDLL:
     L.cghip_prepare_fp32.argtypes = [vp]
     L.cghip_set_poly.argtypes = [vp, C.c_int, _f64p, C.c_double]
     L.cghip_matrix_stats.argtypes = [vp, _f64p, C.c_int]
+    L.cghip_time_poly_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.cghip_mlp_param_count.argtypes = [C.c_int] * 4
     L.cghip_mlp_create.argtypes = [C.c_int] * 6 + [C.POINTER(vp)]
@@ -249,6 +250,12 @@ class Context:
         """Polynomial preconditioner p(D^-1 A) D^-1, p = sum_j coeffs[j] T_j(2 l / b - 1)."""
         c = np.ascontiguousarray(coeffs, dtype=np.float64)
         self._chk(self.L.cghip_set_poly(self._h, int(c.size - 1), c, float(b)), "cghip_set_poly")
+
+    def time_poly_step(self, reps: int = 5) -> float:
+        """Average ms of one Clenshaw step (back-to-back applications)."""
+        ms = C.c_double()
+        self._chk(self.L.cghip_time_poly_step(self._h, reps, C.byref(ms)), "cghip_time_poly_step")
+        return ms.value
 
     def matrix_stats(self) -> np.ndarray:
         """36 statistics of the CRS matrix (include/cghip.h, cghip_matrix_stats)."""

@@ -1116,6 +1116,33 @@ int cghip_matrix_stats(cghip_ctx *c, double *stats, int cap) {
   return CGHIP_OK;
 }
 
+int cghip_time_poly_step(cghip_ctx *c, int reps, double *avg_ms) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (reps <= 0 || !avg_ms) return fail(c, CGHIP_ERR_ARG, "invalid arguments");
+  if (!c->has_poly || !c->has_scaled) return fail(c, CGHIP_ERR_STATE, "call cghip_set_poly first");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int rc = ensure_events(c, 2)) return rc;
+  Op32 A{};
+  A.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, c->n, c->nrb_nd, 1};
+  A.ell = Ell<float>{c->ell_len_nd, c->ell_idx_nd, c->ell_val_nd, c->n, c->ell_w_nd, 1};
+  A.is_ell = c->ell_nd ? 1 : 0;
+  float *const w[3] = {c->w32[0], c->w32[1], c->w32[2]};
+  const Partials prz{pr_of(c).b, nullptr};
+  // defined inputs: r32 = 1, B_k = c_k r32 is rebuilt by each application
+  HIPCHK(c, fill32(c->n, 1.0f, c->v32[1], c->stream));
+  for (auto &v : c->w32) HIPCHK(c, fill32(c->n, 0.0f, v, c->stream));
+  HIPCHK(c, mp_poly_apply(A, c->v32[1], c->d_poly, c->poly_degree, w, nullptr, prz, nullptr,
+                          c->stream));  // warm-up
+  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
+  for (int i = 0; i < reps; i++)
+    HIPCHK(c, mp_poly_apply(A, c->v32[1], c->d_poly, c->poly_degree, w, nullptr, prz, nullptr,
+                            c->stream));
+  HIPCHK(c, hipEventRecord(c->events[1], c->stream));
+  HIPCHK(c, hipEventSynchronize(c->events[1]));
+  *avg_ms = elapsed(c->events[0], c->events[1]) / ((double)reps * c->poly_degree);
+  return CGHIP_OK;
+}
+
 int cghip_time_matvec(cghip_ctx *c, int x, int y, int reps, double *avg_ms) {
   if (int rc = check_ready(c)) return rc;
   if (!valid_vec(x) || !valid_vec(y) || x == y || reps <= 0 || !avg_ms)

@@ -199,6 +199,8 @@ inline bool ell_pays(int64_t width, int64_t n, int64_t nnz) { return width * n <
 
 // y = (float)x
 hipError_t convert_f64_f32(int64_t n, const double *x, float *y, hipStream_t s);
+// y[i] = v
+hipError_t fill32(int64_t n, float v, float *y, hipStream_t s);
 // Scaled operator set-up.  scaled_diag: s[i] = 1 / sqrt(a_ii) and
 // bad[0] = 1 if a row has no or more than one diagonal entry or a_ii <= 0.
 // scaled_copy: off-diagonal entries of row i to offset ptr[i] - i.

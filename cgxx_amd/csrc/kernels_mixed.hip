@@ -39,6 +39,11 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const double *__r
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = (float)x[i];
 }
 
+__global__ __launch_bounds__(kBlock) void k_fill32(int64_t n, float v, float *__restrict__ y) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) y[i] = v;
+}
+
 // s[i] = 1 / sqrt(a_ii); bad[0] = 1 for a row without exactly one diagonal
 // entry or with a_ii <= 0 (every writer stores the same value).
 __global__ __launch_bounds__(kBlock) void k_scaled_diag(Crs<double> A, double *__restrict__ s,
@@ -534,6 +539,14 @@ hipError_t convert_f64_f32(int64_t n, const double *x, float *y, hipStream_t s) 
   if (g > 8192) g = 8192;
   if (g < 1) g = 1;
   hipLaunchKernelGGL(k_convert, dim3((unsigned)g), dim3(kBlock), 0, s, n, x, y);
+  return hipGetLastError();
+}
+
+hipError_t fill32(int64_t n, float v, float *y, hipStream_t s) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_fill32, dim3((unsigned)g), dim3(kBlock), 0, s, n, v, y);
   return hipGetLastError();
 }
 

@@ -230,6 +230,12 @@ int cghip_matrix_stats(cghip_ctx *ctx, double *stats, int cap);
  * x -> y, timed with HIP events on the context's stream. */
 int cghip_time_matvec(cghip_ctx *ctx, int x, int y, int reps, double *avg_ms);
 
+/* Average duration (ms) of one Clenshaw step of the polynomial set by
+ * cghip_set_poly on the scaled FP32 operator: `reps` applications of the
+ * whole recurrence back to back on the context's stream between two HIP
+ * events, divided by reps x degree.  Overwrites the FP32 work vectors. */
+int cghip_time_poly_step(cghip_ctx *ctx, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

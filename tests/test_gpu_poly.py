@@ -123,3 +123,21 @@ def test_poly_full_size_c2(ctx):
     ctx.fill("X", 0.0)
     infoj = ctx.solve_mixed(precond="jacobi")
     assert info["iterations"] < infoj["iterations"] / 3
+
+
+def test_time_poly_step(ctx, tmp_path):
+    from cgxx_amd.cghip import CGHIPError
+
+    fx = [f for f in load_golden() if f["case"] == "diff3d_24"][0]
+    case = Case(fx, str(tmp_path))
+    _setup(ctx, *case.crs, case.k, case.C)
+    with pytest.raises(CGHIPError):
+        ctx.time_poly_step()
+    ctx.set_poly(*_coeffs(*case.crs, 8))
+    assert ctx.time_poly_step(reps=3) > 0
+    # the solver still runs bit-exactly after the timing overwrote its buffers
+    info = ctx.solve_mixed(precond="poly")
+    ox, oinfo, _ = oracle.mixed_pcg(n=case.n, crs=case.crs, k=case.k, precond="poly",
+                                    poly=_coeffs(*case.crs, 8))
+    assert info["iterations"] == oinfo["iterations"]
+    assert_bitwise(ctx.download("X"), ox, "x after timing")
