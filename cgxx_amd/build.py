@@ -8,7 +8,9 @@ Outputs (git-ignored, shipped with the tree):
 """
 from __future__ import annotations
 
+import json
 import os
+import re
 import subprocess
 import sys
 
@@ -44,11 +46,59 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
+RESOURCES = os.path.join(HERE, "lib", "kernel_resources.json")
+_REMARK_KEYS = {"TotalSGPRs": "sgpr", "VGPRs": "vgpr", "AGPRs": "agpr",
+                "ScratchSize [bytes/lane]": "scratch", "Occupancy [waves/SIMD]": "occupancy",
+                "LDS Size [bytes/block]": "lds"}
+
+
+def parse_resource_remarks(text: str) -> dict[str, dict[str, int]]:
+    """Per-kernel register, scratch, LDS and occupancy figures from the
+    compiler's -Rpass-analysis=kernel-resource-usage remarks."""
+    kernels: dict[str, dict[str, int]] = {}
+    cur = None
+    for line in text.splitlines():
+        if "remark:" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split(" [-Rpass", 1)[0].strip()
+        if body.startswith("Function Name:"):
+            cur = kernels.setdefault(body.split(":", 1)[1].strip(), {})
+            continue
+        key, _, val = body.rpartition(":")
+        if cur is not None and key.strip() in _REMARK_KEYS:
+            try:
+                cur[_REMARK_KEYS[key.strip()]] = int(val.strip())
+            except ValueError:
+                pass
+    names = list(kernels)
+    try:
+        r = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True,
+                           check=True)
+        demangled = r.stdout.splitlines()
+        if len(demangled) == len(names):
+            return {d: kernels[m] for m, d in zip(names, demangled)}
+    except (OSError, subprocess.CalledProcessError):
+        pass
+    return kernels
+
+
 def build_library() -> str:
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    cmd = [HIPCC, *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-shared", "-o", tmp, *srcs]
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # drop the remarks and the source lines the compiler echoes under them
+    rest = "\n".join(l for l in r.stdout.splitlines()
+                     if "packed-fp32-ops" not in l and "remark:" not in l
+                     and not re.match(r"^\s*\d*\s*\|", l) and l.strip())
+    if rest.strip():
+        print(rest, flush=True)
+    if r.returncode:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+    with open(RESOURCES, "w") as f:
+        json.dump(parse_resource_remarks(r.stdout), f, indent=1, sort_keys=True)
     os.replace(tmp, LIB)
     return LIB
 

@@ -9,7 +9,7 @@
 
 #include "../../include/cghip.h"
 #include "cghip_kernels.h"
-#include "features.h"
+#include "matrix_stats.h"
 
 #include <algorithm>
 #include <chrono>
@@ -79,8 +79,8 @@ struct cghip_ctx {
   // the scaled operator in column-major ELL when its padding is small
   bool ell_nd = false;
   int ell_w_nd = 0;
-  int *ell_len_nd = nullptr, *ell_idx_nd = nullptr;
-  float *ell_val_nd = nullptr;
+  int *ell_len_nd = nullptr;
+  uint64_t *ell_slot_nd = nullptr;  // packed (index | value bits << 32)
   float *v32[4] = {};
   MixedState *d_mstate = nullptr;
   MixedState *h_mstate = nullptr;  // pinned
@@ -161,8 +161,7 @@ void free_fp32(cghip_ctx *c) {
   dfree(c->value_nd);
   dfree(c->scale64);
   dfree(c->ell_len_nd);
-  dfree(c->ell_idx_nd);
-  dfree(c->ell_val_nd);
+  dfree(c->ell_slot_nd);
   c->ell_nd = false;
   c->ell_w_nd = 0;
   for (auto &v : c->v32) dfree(v);
@@ -754,14 +753,12 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     if (ell_pays(w, n, c->nnz_nd)) {
       const size_t slots = (size_t)w * (size_t)n;
       rc = dalloc(c, &c->ell_len_nd, (size_t)n);
-      if (!rc) rc = dalloc(c, &c->ell_idx_nd, slots);
-      if (!rc) rc = dalloc(c, &c->ell_val_nd, slots);
+      if (!rc) rc = dalloc(c, &c->ell_slot_nd, slots);
       if (rc) {
         free_fp32(c);
         return rc;
       }
-      HIPCHK(c, scaled_ell(crs_of(c), c->scale64, c->ell_len_nd, c->ell_idx_nd, c->ell_val_nd,
-                           c->stream));
+      HIPCHK(c, scaled_ell(crs_of(c), c->scale64, c->ell_len_nd, c->ell_slot_nd, c->stream));
       c->ell_nd = true;
       c->ell_w_nd = w;
     }
@@ -874,7 +871,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   Op32 A32{};
   if (scaled) {
     A32.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, n, c->nrb_nd, 1};
-    A32.ell = Ell<float>{c->ell_len_nd, c->ell_idx_nd, c->ell_val_nd, n, c->ell_w_nd, 1};
+    A32.ell = EllPacked{c->ell_len_nd, c->ell_slot_nd, n, c->ell_w_nd, 1};
     A32.is_ell = c->ell_nd ? 1 : 0;
   } else {
     A32.crs = Crs<float>{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb, 0};
@@ -1124,7 +1121,7 @@ int cghip_time_poly_step(cghip_ctx *c, int reps, double *avg_ms) {
   if (int rc = ensure_events(c, 2)) return rc;
   Op32 A{};
   A.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, c->n, c->nrb_nd, 1};
-  A.ell = Ell<float>{c->ell_len_nd, c->ell_idx_nd, c->ell_val_nd, c->n, c->ell_w_nd, 1};
+  A.ell = EllPacked{c->ell_len_nd, c->ell_slot_nd, c->n, c->ell_w_nd, 1};
   A.is_ell = c->ell_nd ? 1 : 0;
   float *const w[3] = {c->w32[0], c->w32[1], c->w32[2]};
   const Partials prz{pr_of(c).b, nullptr};

@@ -136,13 +136,19 @@ __device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, 
 // epilogue's own loads in flight with the slots), the thread adds its live
 // slots left to right from 0 and calls epi(r, s, pv).  The matrix streams
 // (length, index, data) are read once, so they are loaded non-temporally
-// and leave the caches to the gathered x.
-template <typename V, typename X, typename Pre, typename Epi>
+// and leave the caches to the gathered x.  With O32 (allowed when
+// width * n * sizeof(V) <= 2^32, see ell_o32) the slots are addressed by
+// 32-bit byte offsets from the two array bases, which keeps the FP64
+// kernels at 58 VGPRs (8 waves per SIMD) instead of 76 with 64-bit
+// addresses per slot.
+template <typename V, typename X, bool O32, typename Pre, typename Epi>
 __device__ inline double ell_row_pre_epi(const Ell<V> &A, const X *__restrict__ x, int r, Pre pre,
                                          Epi epi) {
   const int64_t n = A.n;
   const int len = __builtin_nontemporal_load(A.length + r);
   const auto pv = pre(r);
+  const char *ib = reinterpret_cast<const char *>(A.index);
+  const char *vb = reinterpret_cast<const char *>(A.data);
   X s = X(0);
   for (int j0 = 0; j0 < len; j0 += kEllChunk) {
     int c[kEllChunk];
@@ -150,9 +156,16 @@ __device__ inline double ell_row_pre_epi(const Ell<V> &A, const X *__restrict__ 
 #pragma unroll
     for (int u = 0; u < kEllChunk; u++) {
       if (j0 + u < len) {
-        const int64_t k = (int64_t)(j0 + u) * n + r;
-        c[u] = __builtin_nontemporal_load(A.index + k);
-        v[u] = __builtin_nontemporal_load(A.data + k);
+        if (O32) {
+          const uint32_t k = (uint32_t)(j0 + u) * (uint32_t)n + (uint32_t)r;
+          c[u] = __builtin_nontemporal_load(reinterpret_cast<const int *>(ib + k * 4u));
+          v[u] = __builtin_nontemporal_load(
+              reinterpret_cast<const V *>(vb + k * (uint32_t)sizeof(V)));
+        } else {
+          const int64_t k = (int64_t)(j0 + u) * n + r;
+          c[u] = __builtin_nontemporal_load(A.index + k);
+          v[u] = __builtin_nontemporal_load(A.data + k);
+        }
       }
     }
 #pragma unroll
@@ -162,20 +175,35 @@ __device__ inline double ell_row_pre_epi(const Ell<V> &A, const X *__restrict__ 
   return epi(r, s, pv);
 }
 
-template <typename V, typename X, typename Epi>
-__device__ inline double ell_row_epi(const Ell<V> &A, const X *__restrict__ x, int r, Epi epi) {
-  return ell_row_pre_epi<V, X>(
-      A, x, r, [](int) { return 0; }, [&](int i, X s, int) { return epi(i, s); });
+// The same over packed FP32 slots (index | value bits << 32): one 8-byte
+// non-temporal load per slot.
+template <typename Pre, typename Epi>
+__device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__restrict__ x, int r,
+                                         Pre pre, Epi epi) {
+  const int64_t n = A.n;
+  const int len = __builtin_nontemporal_load(A.length + r);
+  const auto pv = pre(r);
+  float s = 0.0f;
+  for (int j0 = 0; j0 < len; j0 += kEllChunk) {
+    uint64_t e[kEllChunk];
+#pragma unroll
+    for (int u = 0; u < kEllChunk; u++)
+      if (j0 + u < len) e[u] = __builtin_nontemporal_load(A.slot + ((int64_t)(j0 + u) * n + r));
+#pragma unroll
+    for (int u = 0; u < kEllChunk; u++)
+      if (j0 + u < len) s += __uint_as_float((uint32_t)(e[u] >> 32)) * x[(int)(uint32_t)e[u]];
+  }
+  return epi(r, s, pv);
 }
 
 // ELL rows rb*256 + t: y = A x; with WITH_DOT, returns the owner's
 // x[r] * y[r] (in FP64) for the fused <x, A x>.
-template <typename V, typename X, bool WITH_DOT>
+template <typename V, typename X, bool WITH_DOT, bool O32>
 __device__ inline double ell_rowblock(const Ell<V> &A, const X *__restrict__ x, X *__restrict__ y,
                                       int rb) {
   const int r = rb * kBlock + (int)threadIdx.x;
   if (r >= A.n) return 0.0;
-  return ell_row_pre_epi<V, X>(
+  return ell_row_pre_epi<V, X, O32>(
       A, x, r, [&](int i) { return WITH_DOT ? x[i] : X(0); },
       [&](int i, X s, X xi) -> double {
         y[i] = s;

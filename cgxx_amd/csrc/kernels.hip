@@ -42,11 +42,11 @@ __global__ __launch_bounds__(kBlock) void k_matvec_crs(Crs<V> A, const X *__rest
   for (int rb = blockIdx.x; rb < A.nrb; rb += gridDim.x) crs_rowblock<V, X, false>(A, x, y, rb, prod);
 }
 
-template <typename V, typename X>
+template <typename V, typename X, bool O32>
 __global__ __launch_bounds__(kBlock) void k_matvec_ell(Ell<V> A, const X *__restrict__ x,
                                                        X *__restrict__ y) {
   const int nrb = (A.n + kBlock - 1) / kBlock;
-  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) ell_rowblock<V, X, false>(A, x, y, rb);
+  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) ell_rowblock<V, X, false, O32>(A, x, y, rb);
 }
 
 // First kernel of an iteration: copies the loop scalars `next` -> `cur`
@@ -77,6 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_crs(Crs<double> A, co
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
 
+template <bool O32>
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_ell(Ell<double> A, const double *__restrict__ p,
                                                                double *__restrict__ q, Partials pq,
                                                                SolveState *st) {
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_ell(Ell<double> A, co
   const int nrb = (A.n + kBlock - 1) / kBlock;
   double tpart = 0.0;
   for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x)
-    tpart += ell_rowblock<double, double, true>(A, p, q, rb);
+    tpart += ell_rowblock<double, double, true, O32>(A, p, q, rb);
   const double bp = block_fold(tpart, lds);
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
@@ -459,8 +460,11 @@ hipError_t matvec_crs_t(const Crs<V> &A, const X *x, X *y, hipStream_t s) {
 template <typename V, typename X>
 hipError_t matvec_ell_t(const Ell<V> &A, const X *x, X *y, hipStream_t s) {
   if (A.n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_matvec_ell<V, X>), dim3(ell_chunks(A.n) > 0 ? ell_chunks(A.n) : 1),
-                     dim3(kBlock), 0, s, A, x, y);
+  const dim3 grid(ell_chunks(A.n) > 0 ? ell_chunks(A.n) : 1);
+  if (ell_o32(A))
+    hipLaunchKernelGGL((k_matvec_ell<V, X, true>), grid, dim3(kBlock), 0, s, A, x, y);
+  else
+    hipLaunchKernelGGL((k_matvec_ell<V, X, false>), grid, dim3(kBlock), 0, s, A, x, y);
   return hipGetLastError();
 }
 
@@ -532,8 +536,12 @@ hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, 
 
 hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
                               SolveState *st, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_matvec_dot_ell, dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A, p, q,
-                     pq, st);
+  if (ell_o32(A))
+    hipLaunchKernelGGL(k_pcg_matvec_dot_ell<true>, dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A,
+                       p, q, pq, st);
+  else
+    hipLaunchKernelGGL(k_pcg_matvec_dot_ell<false>, dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A,
+                       p, q, pq, st);
   return hipGetLastError();
 }
 

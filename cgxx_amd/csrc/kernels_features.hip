@@ -14,7 +14,7 @@
 // blocks in ascending order.
 
 #include "device_common.h"
-#include "features.h"
+#include "matrix_stats.h"
 
 #pragma clang fp contract(off)
 

@@ -87,8 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_scaled_copy(Crs<double> A, const dou
 
 __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const double *__restrict__ s,
                                                        int *__restrict__ length,
-                                                       int *__restrict__ index,
-                                                       float *__restrict__ data) {
+                                                       uint64_t *__restrict__ slot) {
   const int64_t T = (int64_t)gridDim.x * kBlock;
   const int64_t n = A.n;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
@@ -97,8 +96,8 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
     for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
       const int c = A.index[e];
       if (c == (int)i) continue;
-      index[k * n + i] = c;
-      data[k * n + i] = (float)((A.value[e] * si) * s[c]);
+      const float v = (float)((A.value[e] * si) * s[c]);
+      slot[k * n + i] = (uint64_t)(uint32_t)c | ((uint64_t)__float_as_uint(v) << 32);
       k++;
     }
     length[i] = (int)k;
@@ -117,7 +116,7 @@ __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, f
     const int nrb = ell_chunks(A.ell.n);
     for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
       const int r = rb * kBlock + (int)threadIdx.x;
-      if (r < A.ell.n) tpart += ell_row_pre_epi<float, float>(A.ell, x, r, pre, epi);
+      if (r < A.ell.n) tpart += ell_row_pre_epi(A.ell, x, r, pre, epi);
     }
   } else {
     for (int rb = blockIdx.x; rb < A.crs.nrb; rb += gridDim.x)
@@ -453,7 +452,11 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_poly(int n, const double 
 //   j = 0: z   = (c_0 r + s) - B_2, and the partials of <r, z>
 // (B_{j+2} = 0 when it does not exist).  st == nullptr runs the step
 // unconditionally (restart); otherwise it is skipped once the iteration is
-// off.
+// off.  A_hat is the scaled operator: its unit diagonal is implicit (the
+// launcher refuses any other).  The kernel keeps at most 80 SGPRs: above
+// that, fewer than 8 blocks of 256 fit on a CU of gfx950 and the grid of
+// 2048 blocks runs in two rounds (295 us instead of 200 us per step at
+// 256^3).
 template <bool FINAL, bool ELL>
 __global__ __launch_bounds__(kBlock) void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
@@ -467,12 +470,11 @@ __global__ __launch_bounds__(kBlock) void k_poly_step(Op32 A, const float *__res
   if (st && st->cur.done) return;
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
-  const int unit = ELL ? A.ell.unit : A.crs.unit;
   const double tpart = op32_rows<ELL>(
       A, b1, prod, [&](int i) { return float4{b1[i], r[i], b2 ? b2[i] : 0.0f, 0.0f}; },
       [&](int i, float t0, float4 pv) -> double {
         const float bi = pv.x, ri = pv.y, bb = pv.z;
-        const float t = unit ? bi + t0 : t0;
+        const float t = bi + t0;
         const float sv = tb * t - bi;
         if (FINAL) {
           const float z = (cj * ri + sv) - bb;
@@ -562,10 +564,9 @@ hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, flo
   return hipGetLastError();
 }
 
-hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, int *index, float *data,
+hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64_t *slot,
                       hipStream_t st) {
-  hipLaunchKernelGGL(k_scaled_ell, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, length, index,
-                     data);
+  hipLaunchKernelGGL(k_scaled_ell, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, length, slot);
   return hipGetLastError();
 }
 
@@ -619,6 +620,7 @@ static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
+  if ((A.is_ell ? A.ell.unit : A.crs.unit) != 1) return hipErrorInvalidValue;
   if (A.is_ell)
     poly_apply_t<true>(A, r32, poly, degree, w, pcopy, prz, st, s);
   else
