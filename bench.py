@@ -135,7 +135,17 @@ def main() -> int:
     ctx.prepare_fp32()
     ctx.upload("K", k)
     nnz_nd = nnz - n  # the scaled operator stores no diagonal
-    log("uploaded FP64 CRS + ELL, Jacobi, FP32 copy and scaled FP32 operator")
+    layout = ctx.fp32_layout()
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(m.ptr))
+    offs = np.unique(m.index.astype(np.int64) - rows)
+    ndiag = int(offs.size - (offs == 0).sum())
+    del rows
+    # operator stream of one FP32 SpMV of the scaled operator (without x, y)
+    op_bytes = {"DIA": 4 * ndiag * n + n, "ELL": 8 * nnz_nd + 4 * n,
+                "CRS": 8 * nnz_nd + 4 * (n + 1)}[layout]
+    kind = {"CRS": 0, "ELL": 1, "DIA": 2}[layout]
+    log(f"uploaded FP64 CRS + ELL, Jacobi, FP32 copy and scaled FP32 operator ({layout}"
+        + (f", {ndiag} diagonals)" if layout == "DIA" else ")"))
 
     model = learned.load_model()
     mlp = Mlp(tuple(int(v) for v in model["dims"]), 1)
@@ -215,7 +225,7 @@ def main() -> int:
 
     secmj, infomj = timed(mixed, "mixed jacobi")
     profmj = mixed(profile=1)
-    spmv_nd_bytes = 8 * nnz_nd + 4 * n + 8 * n  # FP32 ELL scaled operator, x and y
+    spmv_nd_bytes = op_bytes + 8 * n  # FP32 scaled operator, x and y
     mixed_j = {
         "seconds": secmj, "iterations": infomj["iterations"],
         "reliable_updates": infomj["updates"], "true_relative_residual": infomj["true_residual"],
@@ -248,7 +258,7 @@ def main() -> int:
     step_ms = ctx.time_poly_step(reps=5)  # steady state: applications back to back
     # Clenshaw step j: the operator stream, B_{j+1} gathered, r read, B_j
     # written, B_{j+2} read in all but the first step of the recurrence
-    step_bytes = 8 * nnz_nd + 4 * n + 4 * n + 4 * n + 4 * n + (deg - 1) / deg * 4 * n
+    step_bytes = op_bytes + 4 * n + 4 * n + 4 * n + (deg - 1) / deg * 4 * n
     step_gbps = step_bytes / (step_ms * 1e-3) / 1e9
     log(f"mixed+learned: method {infol['method']}, {infol['poly_iterations']} poly + "
         f"{infol['jacobi_iterations']} Jacobi iterations ({infol['spmvs']} FP32 SpMVs), true residual "
@@ -287,14 +297,15 @@ def main() -> int:
             "workload": f"3D 7-point variable-coefficient diffusion {args.side}^3 "
                         "(log-normal kappa, sigma=1, corr=8, seed 20261202), k=A*1, x0=0, "
                         "FP64 true relative residual 1e-9",
-            "method": "mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 (ELL, FP32 values, int32 "
-                      "indices), FP64 reliable updates (delta 0.1); the trained network gives the "
+            "method": f"mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 ({layout}, FP32 values), "
+                      "FP64 reliable updates (delta 0.1); the trained network gives the "
                       "degree-8 polynomial's coefficients and predicted iteration counts, and the "
                       "polynomial runs only when its predicted cost beats Jacobi's (else Jacobi); "
                       "features, inference, choice and fallback inside the timed step",
             "n": n,
             "nnz": nnz,
-            "format": "CRS upload, scaled FP32 operator in ELL",
+            "format": f"CRS upload, scaled FP32 operator in {layout}"
+                      + (f" ({ndiag} diagonals)" if layout == "DIA" else ""),
         },
         "iterations": infol["poly_iterations"] + infol["jacobi_iterations"],
         "fp32_spmvs": infol["spmvs"],
@@ -320,8 +331,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": step_gbps / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("k_poly_step<false, true>"),
-            "kernel": "k_poly_step<false, true> (Clenshaw step, FP32 ELL scaled operator)",
+            "traffic": traffic_from_profiles(f"k_poly_step<false, {kind}>"),
+            "kernel": f"k_poly_step<false, {kind}> (Clenshaw step, FP32 {layout} scaled operator)",
             "polynomial": prof_src,
             "algorithmic_bytes_per_launch": step_bytes,
         },

@@ -27,7 +27,7 @@ SYMBOLS = [
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
-    "cghip_set_poly", "cghip_matrix_stats", "cghip_time_poly_step",
+    "cghip_set_poly", "cghip_matrix_stats", "cghip_time_poly_step", "cghip_fp32_layout",
     "cghip_mlp_param_count", "cghip_mlp_create", "cghip_mlp_destroy", "cghip_mlp_last_error",
     "cghip_mlp_set_params", "cghip_mlp_get_params", "cghip_mlp_forward", "cghip_mlp_gradient",
     "cghip_mlp_train",
@@ -92,7 +92,7 @@ def load() -> C.CDLL:
     L.cghip_destroy.argtypes = [vp]
     L.cghip_last_error.argtypes = [vp]
     L.cghip_last_error.restype = C.c_char_p
-    L.cghip_geometry.argtypes = [C.POINTER(C.c_int)] * 6
+    L.cghip_geometry.argtypes = [C.POINTER(C.c_int)] * 7
     L.cghip_set_size.argtypes = [vp, C.c_int]
     L.cghip_upload_crs.argtypes = [vp, C.c_int, C.c_int, _i32p, _i32p, _f64p]
     L.cghip_upload_ell.argtypes = [vp, C.c_int, C.c_int, _i32p, _i32p, _f64p]
@@ -116,6 +116,7 @@ def load() -> C.CDLL:
     L.cghip_set_poly.argtypes = [vp, C.c_int, _f64p, C.c_double]
     L.cghip_matrix_stats.argtypes = [vp, _f64p, C.c_int]
     L.cghip_time_poly_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.cghip_fp32_layout.argtypes = [vp, C.POINTER(C.c_int)]
     _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.cghip_mlp_param_count.argtypes = [C.c_int] * 4
     L.cghip_mlp_create.argtypes = [C.c_int] * 6 + [C.POINTER(vp)]
@@ -133,10 +134,10 @@ def load() -> C.CDLL:
 
 
 def geometry() -> dict:
-    vals = [C.c_int() for _ in range(6)]
+    vals = [C.c_int() for _ in range(7)]
     load().cghip_geometry(*[C.byref(v) for v in vals])
-    return dict(zip(["block", "vec_gmax", "rows_max", "nnz_cap", "spmv_gmax", "ell_gmax"],
-                    [v.value for v in vals]))
+    return dict(zip(["block", "vec_gmax", "rows_max", "nnz_cap", "spmv_gmax", "ell_gmax",
+                     "dia_max"], [v.value for v in vals]))
 
 
 class Context:
@@ -245,6 +246,12 @@ class Context:
 
     def prepare_fp32(self):
         self._chk(self.L.cghip_prepare_fp32(self._h), "cghip_prepare_fp32")
+
+    def fp32_layout(self) -> str | None:
+        """Layout of the scaled FP32 operator: "CRS", "ELL", "DIA" or None."""
+        v = C.c_int()
+        self._chk(self.L.cghip_fp32_layout(self._h, C.byref(v)), "cghip_fp32_layout")
+        return {0: "CRS", 1: "ELL", 2: "DIA"}.get(v.value)
 
     def set_poly(self, coeffs, b: float):
         """Polynomial preconditioner p(D^-1 A) D^-1, p = sum_j coeffs[j] T_j(2 l / b - 1)."""

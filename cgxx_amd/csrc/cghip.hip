@@ -81,6 +81,14 @@ struct cghip_ctx {
   int ell_w_nd = 0;
   int *ell_len_nd = nullptr;
   uint64_t *ell_slot_nd = nullptr;  // packed (index | value bits << 32)
+  // ... or in DIA when its off-diagonal entries lie on few diagonals
+  // (found at upload: dia_ndiag = -1 if a row's columns are not ascending or
+  // there are more than kDiaMax distinct offsets)
+  int dia_ndiag = -1;
+  int dia_off[kDiaMax] = {};
+  bool dia_nd = false;
+  float *dia_val = nullptr;
+  uint8_t *dia_mask = nullptr;
   float *v32[4] = {};
   MixedState *d_mstate = nullptr;
   MixedState *h_mstate = nullptr;  // pinned
@@ -162,6 +170,9 @@ void free_fp32(cghip_ctx *c) {
   dfree(c->scale64);
   dfree(c->ell_len_nd);
   dfree(c->ell_slot_nd);
+  dfree(c->dia_val);
+  dfree(c->dia_mask);
+  c->dia_nd = false;
   c->ell_nd = false;
   c->ell_w_nd = 0;
   for (auto &v : c->v32) dfree(v);
@@ -210,6 +221,30 @@ Ell<double> ell_of(cghip_ctx *c) {
   return Ell<double>{c->ell_length, c->ell_index, c->ell_data, c->n, c->ell_width};
 }
 
+Dia dia_of(cghip_ctx *c) {
+  Dia d{c->dia_val, c->dia_mask, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1};
+  for (int k = 0; k < kDiaMax; k++) d.off[k] = c->dia_off[k];
+  return d;
+}
+
+// The FP32 operator of the mixed inner loop: the scaled operator (unit
+// diagonal implicit) in its DIA, ELL or CRS layout, or the unscaled FP32
+// copy of A in CRS.
+Op32 op32_of(cghip_ctx *c, bool scaled) {
+  Op32 A{};
+  const int n = c->n;
+  if (scaled) {
+    A.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, n, c->nrb_nd, 1};
+    A.ell = EllPacked{c->ell_len_nd, c->ell_slot_nd, n, c->ell_w_nd, 1};
+    A.dia = dia_of(c);
+    A.kind = c->dia_nd ? kOpDia : c->ell_nd ? kOpEll : kOpCrs;
+  } else {
+    A.crs = Crs<float>{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb, 0};
+    A.kind = kOpCrs;
+  }
+  return A;
+}
+
 int check_ready(cghip_ctx *c) {
   if (!c) return CGHIP_ERR_ARG;
   if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no vectors: call cghip_set_size first");
@@ -237,13 +272,14 @@ int sync(cghip_ctx *c) {
 extern "C" {
 
 int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap, int *spmv_gmax,
-                   int *ell_gmax) {
+                   int *ell_gmax, int *dia_max) {
   if (block) *block = kBlock;
   if (vec_gmax) *vec_gmax = kVecGMax;
   if (rows_max) *rows_max = kRowsMax;
   if (nnz_cap) *nnz_cap = kNnzCap;
   if (spmv_gmax) *spmv_gmax = kSpmvGMax;
   if (ell_gmax) *ell_gmax = kEllGMax;
+  if (dia_max) *dia_max = kDiaMax;
   return CGHIP_OK;
 }
 
@@ -346,6 +382,28 @@ int cghip_upload_crs(cghip_ctx *c, int n, int nnz, const int32_t *ptr, const int
   for (int e = 0; e < nnz; e++)
     if (index[e] < 0 || index[e] >= n)
       return fail(c, CGHIP_ERR_ARG, "CRS column index %d out of range at entry %d", index[e], e);
+  // distinct off-diagonal offsets j - i (ascending), for the DIA layout
+  int ndiag = 0;
+  int off[kDiaMax + 1];
+  for (int i = 0; i < n && ndiag >= 0; i++)
+    for (int e = ptr[i]; e < ptr[i + 1]; e++) {
+      if (e > ptr[i] && index[e] <= index[e - 1]) {
+        ndiag = -1;  // columns not ascending: the DIA order would differ
+        break;
+      }
+      const int o = index[e] - i;
+      if (o == 0) continue;
+      int d = 0;
+      while (d < ndiag && off[d] < o) d++;
+      if (d < ndiag && off[d] == o) continue;
+      if (ndiag == kDiaMax) {
+        ndiag = -1;
+        break;
+      }
+      for (int k = ndiag; k > d; k--) off[k] = off[k - 1];
+      off[d] = o;
+      ndiag++;
+    }
   if (c->n != n)
     if (int rc = cghip_set_size(c, n)) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -370,6 +428,8 @@ int cghip_upload_crs(cghip_ctx *c, int n, int nnz, const int32_t *ptr, const int
   HIPCHK(c, hipMemcpy(c->crs_rb, rb.data(), sizeof(int) * ((size_t)nrb + 1), hipMemcpyHostToDevice));
   c->crs_nnz = nnz;
   c->crs_nrb = nrb;
+  c->dia_ndiag = ndiag;
+  for (int d = 0; d < kDiaMax; d++) c->dia_off[d] = d < ndiag ? off[d] : 0;
   c->has_crs = true;
   if (!c->format) c->format = CGHIP_FORMAT_CRS;
   return CGHIP_OK;
@@ -750,7 +810,18 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     HIPCHK(c, scaled_copy(crs_of(c), c->scale64, c->index_nd, c->value_nd, c->stream));
     int w = 0;
     for (int i = 0; i < n; i++) w = std::max(w, ptr[i + 1] - ptr[i]);
-    if (ell_pays(w, n, c->nnz_nd)) {
+    if (c->dia_ndiag > 0 && dia_pays(c->dia_ndiag, n, c->nnz_nd)) {
+      const size_t slots = (size_t)c->dia_ndiag * (size_t)n;
+      rc = dalloc(c, &c->dia_val, slots);
+      if (!rc) rc = dalloc(c, &c->dia_mask, (size_t)n);
+      if (rc) {
+        free_fp32(c);
+        return rc;
+      }
+      HIPCHK(c, hipMemsetAsync(c->dia_val, 0, sizeof(float) * slots, c->stream));
+      HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_mask, c->stream));
+      c->dia_nd = true;
+    } else if (ell_pays(w, n, c->nnz_nd)) {
       const size_t slots = (size_t)w * (size_t)n;
       rc = dalloc(c, &c->ell_len_nd, (size_t)n);
       if (!rc) rc = dalloc(c, &c->ell_slot_nd, slots);
@@ -766,6 +837,15 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     c->has_scaled = true;
   }
   c->has_fp32 = true;
+  return CGHIP_OK;
+}
+
+int cghip_fp32_layout(const cghip_ctx *c, int *layout) {
+  if (!c || !layout) return CGHIP_ERR_ARG;
+  *layout = !c->has_scaled ? -1
+            : c->dia_nd    ? CGHIP_LAYOUT_DIA
+            : c->ell_nd    ? CGHIP_LAYOUT_ELL
+                           : CGHIP_LAYOUT_CRS;
   return CGHIP_OK;
 }
 
@@ -868,14 +948,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   float *y32 = c->v32[0], *r32 = c->v32[1], *p32 = c->v32[2], *q32 = c->v32[3];
   double *x64 = c->vec[CGHIP_VEC_X], *k64 = c->vec[CGHIP_VEC_K], *r64 = c->vec[CGHIP_VEC_R],
          *q64 = c->vec[CGHIP_VEC_Q];
-  Op32 A32{};
-  if (scaled) {
-    A32.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, n, c->nrb_nd, 1};
-    A32.ell = EllPacked{c->ell_len_nd, c->ell_slot_nd, n, c->ell_w_nd, 1};
-    A32.is_ell = c->ell_nd ? 1 : 0;
-  } else {
-    A32.crs = Crs<float>{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb, 0};
-  }
+  const Op32 A32 = op32_of(c, scaled);
   const int gpq = op32_grid(A32);
   // slots: <p,q> (SpMV grid), <r,r> (quads grid), <r,z> (SpMV grid, the
   // polynomial only: otherwise z = r), ||k - A x||^2 (pairs grid)
@@ -1119,10 +1192,7 @@ int cghip_time_poly_step(cghip_ctx *c, int reps, double *avg_ms) {
   if (!c->has_poly || !c->has_scaled) return fail(c, CGHIP_ERR_STATE, "call cghip_set_poly first");
   HIPCHK(c, hipSetDevice(c->device));
   if (int rc = ensure_events(c, 2)) return rc;
-  Op32 A{};
-  A.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, c->n, c->nrb_nd, 1};
-  A.ell = EllPacked{c->ell_len_nd, c->ell_slot_nd, c->n, c->ell_w_nd, 1};
-  A.is_ell = c->ell_nd ? 1 : 0;
+  const Op32 A = op32_of(c, true);
   float *const w[3] = {c->w32[0], c->w32[1], c->w32[2]};
   const Partials prz{pr_of(c).b, nullptr};
   // defined inputs: r32 = 1, B_k = c_k r32 is rebuilt by each application

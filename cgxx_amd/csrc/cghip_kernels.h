@@ -19,6 +19,7 @@ constexpr int kNnzCap = 2048;    // max nonzeros staged in LDS per row block
 constexpr int kSpmvGMax = 2048;  // max blocks of the CRS SpMV and FP32 ELL kernels
 constexpr int kEllGMax = 8192;   // max blocks of the FP64 ELL SpMV kernels
 constexpr int kPartialSlot = 8192;  // entries per partial-sum slot (>= every grid above)
+constexpr int kDiaMax = 8;       // max diagonals of the FP32 DIA operator (one mask byte per row)
 
 // Vector kernels work on element pairs: G = ceil(pairs / 256), at most 2048.
 inline int vec_grid(int n) {
@@ -202,16 +203,43 @@ struct PolyParams {
   int degree;
 };
 
+// FP32 diagonal (DIA) layout of an operator whose off-diagonal entries lie
+// on at most kDiaMax diagonals j - i = off[d] (off ascending): val[d * n + i]
+// holds entry (i, i + off[d]) and bit d of mask[i] says whether that entry
+// is stored (absent entries are skipped, not multiplied by zero).  No
+// column indices: 4 bytes per slot and one byte per row, and x is read at
+// i + off[d], contiguous across a wave.  Slots are addressed by 32-bit byte
+// offsets, so dia_fits requires ndiag * n * 4 <= 2^32.
+struct Dia {
+  const float *val;
+  const uint8_t *mask;
+  int n, ndiag;
+  int off[kDiaMax];
+  int unit;  // 1: implicit unit diagonal
+};
+inline bool dia_fits(int64_t ndiag, int64_t n) {
+  return ndiag >= 1 && ndiag <= kDiaMax && ndiag * n * 4 <= (1ll << 32) && n * 4 <= (1ll << 32);
+}
+// DIA is used when its bytes per SpMV (4 ndiag n + n) are at most those of
+// the packed ELL (8 nnz + 4 n).
+inline bool dia_pays(int64_t ndiag, int64_t n, int64_t nnz) {
+  return dia_fits(ndiag, n) && 4 * ndiag * n + n <= 8 * nnz + 4 * n;
+}
+
 // The FP32 operator of the inner iteration: CRS (row blocks) or, for the
-// scaled operator when the padding is small, ELL (256-row chunks).
+// scaled operator, DIA or ELL (256-row chunks) when they pay.
+enum Op32Kind { kOpCrs = 0, kOpEll = 1, kOpDia = 2 };
 struct Op32 {
   Crs<float> crs;
   EllPacked ell;
-  int is_ell;
+  Dia dia;
+  int kind;
 };
-// Blocks of the SpMV kernels on op (and partials of their dots).
+// Blocks of the SpMV kernels on op (and partials of their dots): DIA with
+// up to 8192 blocks, ELL up to 2048, CRS one per row block up to 2048.
 inline int op32_grid(const Op32 &op) {
-  return op.is_ell ? ell32_grid(op.ell.n) : spmv_grid(op.crs.nrb);
+  if (op.kind == kOpDia) return ell64_grid(op.dia.n);
+  return op.kind == kOpEll ? ell32_grid(op.ell.n) : spmv_grid(op.crs.nrb);
 }
 // The ELL layout is used when W * n <= 2 (nnz + n), W the longest row.
 inline bool ell_pays(int64_t width, int64_t n, int64_t nnz) { return width * n <= 2 * (nnz + n); }
@@ -230,6 +258,9 @@ hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, flo
 // k * n + i, length[i].
 hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64_t *slot,
                       hipStream_t st);
+// The same entries in DIA (Dia): val zeroed by the caller, mask written.
+hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
+                      uint8_t *mask, hipStream_t st);
 // x64 += scale o (double)y32 (scale nullable: plain add)
 hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s);
 // r64 = k - q64; partials.a = ||r64||^2 (pairs order)

@@ -196,6 +196,32 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
   return epi(r, s, pv);
 }
 
+// DIA row r: pv = pre(r) first, then the stored entries d = 0, 1, ... in
+// ascending diagonal order (the CRS order of a row with ascending columns)
+// summed left to right from 0, then epi(r, s, pv).  Values and the mask are
+// read once (non-temporal); x at r + off[d] is contiguous across a wave.
+template <typename Pre, typename Epi>
+__device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__ x, int r, Pre pre,
+                                         Epi epi) {
+  const uint32_t m = __builtin_nontemporal_load(A.mask + r);
+  const auto pv = pre(r);
+  const char *vb = reinterpret_cast<const char *>(A.val);
+  const char *xb = reinterpret_cast<const char *>(x);
+  const uint32_t r4 = (uint32_t)r * 4u, n4 = (uint32_t)A.n * 4u;
+  float v[kDiaMax], xv[kDiaMax];
+#pragma unroll
+  for (int d = 0; d < kDiaMax; d++)
+    if (d < A.ndiag && ((m >> d) & 1u)) {
+      v[d] = __builtin_nontemporal_load(reinterpret_cast<const float *>(vb + (r4 + (uint32_t)d * n4)));
+      xv[d] = *reinterpret_cast<const float *>(xb + (r4 + (uint32_t)A.off[d] * 4u));
+    }
+  float s = 0.0f;
+#pragma unroll
+  for (int d = 0; d < kDiaMax; d++)
+    if (d < A.ndiag && ((m >> d) & 1u)) s += v[d] * xv[d];
+  return epi(r, s, pv);
+}
+
 // ELL rows rb*256 + t: y = A x; with WITH_DOT, returns the owner's
 // x[r] * y[r] (in FP64) for the fused <x, A x>.
 template <typename V, typename X, bool WITH_DOT, bool O32>

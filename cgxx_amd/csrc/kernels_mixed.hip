@@ -104,15 +104,45 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
   }
 }
 
-// Row-block loop of the FP32 operator: for every row i the thread owns,
-// pv = pre(i) early, then epi(i, t, pv) with the sum t of the stored
-// products of row i (from 0, in stored order); returns the sum of the
-// epilogue's dot contributions.
-template <bool ELL, typename Pre, typename Epi>
+// DIA copy of the scaled off-diagonal entries: entry (i, c) goes to
+// diagonal d with D.off[d] == c - i (every off-diagonal offset of the
+// matrix is in D.off, checked on the host); val is zero where no entry is.
+__global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const double *__restrict__ s,
+                                                       Dia D, float *__restrict__ val,
+                                                       uint8_t *__restrict__ mask) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  const int64_t n = A.n;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
+    uint32_t m = 0;
+    const double si = s[i];
+    for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
+      const int c = A.index[e];
+      if (c == (int)i) continue;
+      const int off = c - (int)i;
+      int d = 0;
+      while (d < D.ndiag - 1 && D.off[d] != off) d++;
+      val[(int64_t)d * n + i] = (float)((A.value[e] * si) * s[c]);
+      m |= 1u << d;
+    }
+    mask[i] = (uint8_t)m;
+  }
+}
+
+// Row-block loop of the FP32 operator of kind K: for every row i the
+// thread owns, pv = pre(i) early, then epi(i, t, pv) with the sum t of the
+// stored products of row i (from 0, in stored order); returns the sum of
+// the epilogue's dot contributions.
+template <int K, typename Pre, typename Epi>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
                                    Epi epi) {
   double tpart = 0.0;
-  if (ELL) {
+  if (K == kOpDia) {
+    const int nrb = ell_chunks(A.dia.n);
+    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+      const int r = rb * kBlock + (int)threadIdx.x;
+      if (r < A.dia.n) tpart += dia_row_pre_epi(A.dia, x, r, pre, epi);
+    }
+  } else if (K == kOpEll) {
     const int nrb = ell_chunks(A.ell.n);
     for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
       const int r = rb * kBlock + (int)threadIdx.x;
@@ -124,6 +154,10 @@ __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, f
   }
   return tpart;
 }
+
+// The SGPR cap keeps the DIA instances (8 diagonal offsets in SGPRs) at 8
+// blocks of 256 per CU; the CRS and ELL instances use fewer registers.
+#define CGHIP_SGPR_CAP __attribute__((amdgpu_num_sgpr(74)))
 
 __global__ __launch_bounds__(kBlock) void k_mp_accum(int n, double *__restrict__ x64,
                                                      const float *__restrict__ y32,
@@ -225,16 +259,17 @@ __device__ inline bool mp_begin(MixedState *st) {
 }
 
 // q32 = A32 p32 (row sums in stored order in FP32; + p_i with the implicit
-// unit diagonal) and partials of <p,q>.
-template <bool ELL>
-__global__ __launch_bounds__(kBlock) void k_mp_matvec_dot(Op32 A, const float *__restrict__ p,
+// unit diagonal) and partials of <p,q>.  ELL and DIA hold the scaled
+// operator only (unit diagonal); CRS also the unscaled FP32 copy.
+template <int K>
+__global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_mp_matvec_dot(Op32 A, const float *__restrict__ p,
                                                           float *__restrict__ q, Partials pq,
                                                           MixedState *st) {
   __shared__ float prod[kNnzCap];
   __shared__ double lds[4];
   if (!mp_begin(st)) return;
-  const int unit = ELL ? A.ell.unit : A.crs.unit;
-  const double tpart = op32_rows<ELL>(
+  const int unit = K == kOpCrs ? A.crs.unit : 1;
+  const double tpart = op32_rows<K>(
       A, p, prod, [&](int i) { return p[i]; },
       [&](int i, float t, float pi) -> double {
         const float qi = unit ? pi + t : t;
@@ -457,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_poly(int n, const double 
 // that, fewer than 8 blocks of 256 fit on a CU of gfx950 and the grid of
 // 2048 blocks runs in two rounds (295 us instead of 200 us per step at
 // 256^3).
-template <bool FINAL, bool ELL>
-__global__ __launch_bounds__(kBlock) void k_poly_step(Op32 A, const float *__restrict__ b1,
+template <bool FINAL, int K>
+__global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
                                                       const float *__restrict__ r,
                                                       const PolyParams *poly, int j,
@@ -470,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_poly_step(Op32 A, const float *__res
   if (st && st->cur.done) return;
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
-  const double tpart = op32_rows<ELL>(
+  const double tpart = op32_rows<K>(
       A, b1, prod, [&](int i) { return float4{b1[i], r[i], b2 ? b2[i] : 0.0f, 0.0f}; },
       [&](int i, float t0, float4 pv) -> double {
         const float bi = pv.x, ri = pv.y, bb = pv.z;
@@ -570,6 +605,12 @@ hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64
   return hipGetLastError();
 }
 
+hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
+                      uint8_t *mask, hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_dia, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, D, val, mask);
+  return hipGetLastError();
+}
+
 hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s) {
   hipLaunchKernelGGL(k_mp_accum, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x64, y32, scale);
   return hipGetLastError();
@@ -602,29 +643,37 @@ hipError_t mp_restart_poly(int n, const double *r64, const double *scale, float 
   return hipGetLastError();
 }
 
-template <bool ELL>
+template <int K>
 static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
   const dim3 grid(op32_grid(A)), block(kBlock);
   for (int j = degree - 1; j >= 1; j--) {
     const float *b2 = (j + 2 <= degree) ? w[(j + 2) % 3] : nullptr;
-    hipLaunchKernelGGL((k_poly_step<false, ELL>), grid, block, 0, s, A, w[(j + 1) % 3], b2, r32,
+    hipLaunchKernelGGL((k_poly_step<false, K>), grid, block, 0, s, A, w[(j + 1) % 3], b2, r32,
                        poly, j, w[j % 3], (float *)nullptr, prz, st);
   }
   const float *b2 = degree >= 2 ? w[2] : nullptr;
-  hipLaunchKernelGGL((k_poly_step<true, ELL>), grid, block, 0, s, A, w[1], b2, r32, poly, 0, w[0],
+  hipLaunchKernelGGL((k_poly_step<true, K>), grid, block, 0, s, A, w[1], b2, r32, poly, 0, w[0],
                      pcopy, prz, st);
 }
 
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
-  if ((A.is_ell ? A.ell.unit : A.crs.unit) != 1) return hipErrorInvalidValue;
-  if (A.is_ell)
-    poly_apply_t<true>(A, r32, poly, degree, w, pcopy, prz, st, s);
-  else
-    poly_apply_t<false>(A, r32, poly, degree, w, pcopy, prz, st, s);
+  switch (A.kind) {
+    case kOpDia:
+      if (A.dia.unit != 1) return hipErrorInvalidValue;
+      poly_apply_t<kOpDia>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      break;
+    case kOpEll:
+      if (A.ell.unit != 1) return hipErrorInvalidValue;
+      poly_apply_t<kOpEll>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      break;
+    default:
+      if (A.crs.unit != 1) return hipErrorInvalidValue;
+      poly_apply_t<kOpCrs>(A, r32, poly, degree, w, pcopy, prz, st, s);
+  }
   return hipGetLastError();
 }
 
@@ -645,12 +694,19 @@ hipError_t mp_xpay_poly(int n, float *p32, const float *z32, Partials prr, Parti
 
 hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, MixedState *st,
                          hipStream_t s) {
-  if (A.is_ell)
-    hipLaunchKernelGGL(k_mp_matvec_dot<true>, dim3(op32_grid(A)), dim3(kBlock), 0, s, A, p, q, pq,
-                       st);
-  else
-    hipLaunchKernelGGL(k_mp_matvec_dot<false>, dim3(op32_grid(A)), dim3(kBlock), 0, s, A, p, q, pq,
-                       st);
+  const dim3 grid(op32_grid(A)), block(kBlock);
+  switch (A.kind) {
+    case kOpDia:
+      if (A.dia.unit != 1) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_mp_matvec_dot<kOpDia>, grid, block, 0, s, A, p, q, pq, st);
+      break;
+    case kOpEll:
+      if (A.ell.unit != 1) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_mp_matvec_dot<kOpEll>, grid, block, 0, s, A, p, q, pq, st);
+      break;
+    default:
+      hipLaunchKernelGGL(k_mp_matvec_dot<kOpCrs>, grid, block, 0, s, A, p, q, pq, st);
+  }
   return hipGetLastError();
 }
 

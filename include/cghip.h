@@ -72,7 +72,7 @@ const char *cghip_last_error(const cghip_ctx *ctx);
 /* Reduction geometry of the kernels (the summation order is a function of
  * these and of the vector length / CRS row pointer only). */
 int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap,
-                   int *spmv_gmax, int *ell_gmax);
+                   int *spmv_gmax, int *ell_gmax, int *dia_max);
 
 /* --- data (CGCUDA::doTransferTo / doTransferFrom, cuda/CGCUDA.cu:38-112) - */
 
@@ -157,8 +157,20 @@ int cghip_solve(cghip_ctx *ctx, const cghip_solve_opts *opts,
  * row has exactly one diagonal entry and it is positive; otherwise only
  * PRECOND_NONE is available to cghip_solve_mixed) and the FP32 work
  * vectors.  Call after cghip_upload_crs; cghip_upload_jacobi is not
- * needed (the scaling comes from the matrix's own diagonal). */
+ * needed (the scaling comes from the matrix's own diagonal).  The scaled
+ * operator is stored in DIA when every row's columns are ascending, its
+ * off-diagonal entries lie on at most dia_max (cghip_geometry) diagonals and
+ * 4 ndiag N + N <= 8 nnz_offdiag + 4 N; else in packed ELL when
+ * W N <= 2 (nnz_offdiag + N), W the longest off-diagonal row; else in CRS.
+ * The row sums are the same in every layout (stored order). */
 int cghip_prepare_fp32(cghip_ctx *ctx);
+
+/* Layout of the scaled FP32 operator: 0 CRS, 1 ELL, 2 DIA; -1 when there is
+ * none (cghip_prepare_fp32 not called, or no positive unit scaling). */
+#define CGHIP_LAYOUT_CRS 0
+#define CGHIP_LAYOUT_ELL 1
+#define CGHIP_LAYOUT_DIA 2
+int cghip_fp32_layout(const cghip_ctx *ctx, int *layout);
 
 /* Polynomial preconditioner B = p(D^-1 A) D^-1 with
  * p(l) = sum_{j=0}^{degree} coeffs[j] T_j(2 l / b - 1), the Chebyshev basis

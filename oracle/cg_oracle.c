@@ -602,11 +602,33 @@ typedef struct {
  * y = op x: t = 0; t += v * x[c] over the stored entries in order (FP32);
  * y_i = scaled ? x_i + t : t. */
 typedef struct {
-  int N, unit, nrb;
+  int N, unit, nrb, gmax;
   int *ptr, *index, *starts;
   float *value;
   double *s; /* scaled: s_i, else NULL */
 } op32;
+
+/* The library's choice of the DIA layout for the scaled operator
+ * (cghip_prepare_fp32, include/cghip.h): every row's columns ascending, the
+ * off-diagonal entries on ndiag <= ORC_DIA_MAX distinct diagonals,
+ * ndiag N 4 <= 2^32 and N 4 <= 2^32 (32-bit byte offsets), and
+ * 4 ndiag N + N <= 8 nnz_offdiag + 4 N.  Returns 1 for DIA. */
+static int dia_chosen(const orc_problem *p, int64_t nnz_nd) {
+  int N = p->N, ndiag = 0, off[ORC_DIA_MAX];
+  for (int i = 0; i < N; i++)
+    for (int e = p->ptr[i]; e < p->ptr[i + 1]; e++) {
+      if (e > p->ptr[i] && p->index[e] <= p->index[e - 1]) return 0;
+      int o = p->index[e] - i, d = 0;
+      if (o == 0) continue;
+      while (d < ndiag && off[d] != o) d++;
+      if (d < ndiag) continue;
+      if (ndiag == ORC_DIA_MAX) return 0;
+      off[ndiag++] = o;
+    }
+  int64_t D = ndiag, n = N;
+  return D >= 1 && D * n * 4 <= ((int64_t)1 << 32) && n * 4 <= ((int64_t)1 << 32) &&
+         4 * D * n + n <= 8 * nnz_nd + 4 * n;
+}
 
 static void op32_free(op32 *o) {
   free(o->ptr), free(o->index), free(o->starts), free(o->value), free(o->s);
@@ -647,17 +669,22 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
   }
   o->ptr[N] = k;
   o->nrb = orc_crs_rowblocks(N, o->ptr, o->starts);
+  o->gmax = ORC_SPMV_GMAX;
   if (scaled) {
-    /* the library stores the scaled operator as ELL (256-row chunks) when
-     * W * N <= 2 (nnz + N), W the longest row: same row sums, the dot tree
-     * over uniform 256-row blocks */
+    /* the library stores the scaled operator as DIA (up to 8192 blocks) or
+     * else as ELL (up to 2048 blocks) when W * N <= 2 (nnz + N), W the
+     * longest row, both over 256-row chunks: same row sums (DIA adds a
+     * row's entries by ascending diagonal, i.e. in stored order), the dot
+     * tree over uniform 256-row blocks */
     int64_t w = 0;
     for (int i = 0; i < N; i++)
       if (o->ptr[i + 1] - o->ptr[i] > w) w = o->ptr[i + 1] - o->ptr[i];
-    if (w * (int64_t)N <= 2 * ((int64_t)k + N)) {
+    int dia = dia_chosen(p, k);
+    if (dia || w * (int64_t)N <= 2 * ((int64_t)k + N)) {
       o->nrb = (N + ORC_ROWS_MAX - 1) / ORC_ROWS_MAX;
       for (int b = 0; b <= o->nrb; b++)
         o->starts[b] = b * ORC_ROWS_MAX < N ? b * ORC_ROWS_MAX : N;
+      if (dia) o->gmax = ORC_ELL_GMAX;
     }
   }
   return 0;
@@ -783,7 +810,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
         f0[i] = (double)r32[i];                                               \
         f1[i] = (double)w[0][i];                                              \
       }                                                                       \
-      rz = orc_rowblock_dot_tree(o.nrb, o.starts, f0, f1);                    \
+      rz = orc_rowblock_dot_tree_g(o.nrb, o.starts, f0, f1, o.gmax);                    \
     }                                                                         \
     if (first) {                                                              \
       nrm2_0 = sqrt(rt2);                                                     \
@@ -825,7 +852,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
     for (int i = 0; i < N; i++) q32[i] = op32_row(&o, p32, i);
     for (int i = 0; i < N; i++) f0[i] = (double)p32[i];
     for (int i = 0; i < N; i++) f1[i] = (double)q32[i];
-    double pq = orc_rowblock_dot_tree(o.nrb, o.starts, f0, f1);
+    double pq = orc_rowblock_dot_tree_g(o.nrb, o.starts, f0, f1, o.gmax);
     double alpha = s.rho / pq;
     if (h_pq) h_pq[it] = pq;
     if (h_alpha) h_alpha[it] = alpha;
@@ -844,7 +871,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
         f0[i] = (double)r32[i];
         f1[i] = (double)w[0][i];
       }
-      rz = orc_rowblock_dot_tree(o.nrb, o.starts, f0, f1);
+      rz = orc_rowblock_dot_tree_g(o.nrb, o.starts, f0, f1, o.gmax);
     }
     double resid = sqrt(rr) / inner0;
     double beta = rz / s.rho;

@@ -97,7 +97,7 @@ def lib() -> C.CDLL:
 
 
 GEOMETRY = {"block": 256, "vec_gmax": 2048, "rows_max": 256, "nnz_cap": 2048, "spmv_gmax": 2048,
-            "ell_gmax": 8192}
+            "ell_gmax": 8192, "dia_max": 8}
 
 
 class MMFile:

@@ -37,10 +37,12 @@ def _setup(ctx, ptr, idx, val, k, C):
     ctx.upload("X", np.zeros(ptr.size - 1))
 
 
-# the scaled operator is stored as ELL for all but zipf_3000 (rows up to 202
-# entries: CRS row blocks), so both layouts and dot trees are covered
-CASES = ["lap2d_16", "lap2d_64", "rand_sym_40", "rand_gen_300", "diff3d_24", "lap3d_16", "lap2d_256",
-         "zipf_3000"]
+# layout of the scaled operator: the stencils lie on 4 or 6 diagonals (DIA),
+# the random matrices have short rows on many diagonals (ELL), zipf_3000 has
+# rows of up to 202 entries (CRS row blocks): every layout and dot tree
+LAYOUT = {"lap2d_16": "DIA", "lap2d_64": "DIA", "rand_sym_40": "ELL", "rand_gen_300": "ELL",
+          "diff3d_24": "DIA", "lap3d_16": "DIA", "lap2d_256": "DIA", "zipf_3000": "CRS"}
+CASES = list(LAYOUT)
 
 
 @pytest.mark.parametrize("precond", ["jacobi", "none"])
@@ -49,6 +51,7 @@ def test_mixed_bitwise_vs_oracle(ctx, name, precond, tmp_path):
     fx = [f for f in load_golden() if f["case"] == name][0]
     case = Case(fx, str(tmp_path))
     _setup(ctx, *case.crs, case.k, case.C)
+    assert ctx.fp32_layout() == LAYOUT[name]
     info = ctx.solve_mixed(precond=precond, max_iter=20000, tol=1e-9)
     x = ctx.download("X")
     ox, oinfo, oh = oracle.mixed_pcg(n=case.n, crs=case.crs, k=case.k, precond=precond,
@@ -113,3 +116,33 @@ def test_mixed_full_size_c2(ctx):
     assert info["iterations"] <= 1.3 * info64["iterations"]
     assert np.abs(x_mixed - x64).max() <= 1e-5
     assert np.abs(x_mixed - 1.0).max() <= 1e-5
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "poly"])
+def test_mixed_dia_grid_above_2048_vs_oracle(ctx, precond):
+    """84^3 variable-coefficient diffusion (592,704 rows): the DIA kernels
+    run 2316 blocks, more than the 2048 of the ELL and CRS kernels, so the
+    <p,q> and <r,z> partials use the DIA grid.  Bitwise against the oracle."""
+    from cgxx_amd import spectrum
+
+    m = problems.diffusion3d(84, sigma=1.0, corr=8.0, seed=7)
+    k = m.ones_rhs()
+    C = 1.0 / m.diagonal()
+    _setup(ctx, m.ptr, m.index, m.value, k, C)
+    assert ctx.fp32_layout() == "DIA"
+    poly = None
+    if precond == "poly":
+        b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+        poly = (spectrum.cheb_residual_coeffs(b / 100.0, b, 4), b)
+        ctx.set_poly(*poly)
+    info = ctx.solve_mixed(precond=precond, max_iter=20000, tol=1e-9)
+    x = ctx.download("X")
+    ox, oinfo, oh = oracle.mixed_pcg(n=m.n, crs=(m.ptr, m.index, m.value), k=k, precond=precond,
+                                     C_jac=C, poly=poly, max_iter=20000, tol=1e-9)
+    assert (info["iterations"], info["updates"]) == (oinfo["iterations"], oinfo["updates"])
+    it = info["iterations"]
+    assert_bitwise(ctx.history("r2", it + 1), oh["r2"], "inner r2")
+    assert_bitwise(ctx.history("alpha", it), oh["alpha"], "alpha")
+    assert_bitwise(x, ox, "x")
+    r = k - m.matvec(x)
+    assert np.linalg.norm(r) / np.linalg.norm(k) <= 1.05e-9
