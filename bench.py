@@ -143,6 +143,8 @@ def main() -> int:
     # operator stream of one FP32 SpMV of the scaled operator (without x, y)
     op_bytes = {"DIA": 4 * ndiag * n + n, "ELL": 8 * nnz_nd + 4 * n,
                 "CRS": 8 * nnz_nd + 4 * (n + 1)}[layout]
+    # the Clenshaw steps read the DIA values in FP16
+    poly_op_bytes = 2 * ndiag * n + n if layout == "DIA" else op_bytes
     kind = {"CRS": 0, "ELL": 1, "DIA": 2}[layout]
     log(f"uploaded FP64 CRS + ELL, Jacobi, FP32 copy and scaled FP32 operator ({layout}"
         + (f", {ndiag} diagonals)" if layout == "DIA" else ")"))
@@ -258,7 +260,7 @@ def main() -> int:
     step_ms = ctx.time_poly_step(reps=5)  # steady state: applications back to back
     # Clenshaw step j: the operator stream, B_{j+1} gathered, r read, B_j
     # written, B_{j+2} read in all but the first step of the recurrence
-    step_bytes = op_bytes + 4 * n + 4 * n + 4 * n + (deg - 1) / deg * 4 * n
+    step_bytes = poly_op_bytes + 4 * n + 4 * n + 4 * n + (deg - 1) / deg * 4 * n
     step_gbps = step_bytes / (step_ms * 1e-3) / 1e9
     log(f"mixed+learned: method {infol['method']}, {infol['poly_iterations']} poly + "
         f"{infol['jacobi_iterations']} Jacobi iterations ({infol['spmvs']} FP32 SpMVs), true residual "

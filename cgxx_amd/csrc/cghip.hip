@@ -88,6 +88,7 @@ struct cghip_ctx {
   int dia_off[kDiaMax] = {};
   bool dia_nd = false;
   float *dia_val = nullptr;
+  uint16_t *dia_val16 = nullptr;  // FP16 copy for the polynomial's Clenshaw steps
   uint8_t *dia_mask = nullptr;
   float *v32[4] = {};
   MixedState *d_mstate = nullptr;
@@ -171,6 +172,7 @@ void free_fp32(cghip_ctx *c) {
   dfree(c->ell_len_nd);
   dfree(c->ell_slot_nd);
   dfree(c->dia_val);
+  dfree(c->dia_val16);
   dfree(c->dia_mask);
   c->dia_nd = false;
   c->ell_nd = false;
@@ -222,7 +224,7 @@ Ell<double> ell_of(cghip_ctx *c) {
 }
 
 Dia dia_of(cghip_ctx *c) {
-  Dia d{c->dia_val, c->dia_mask, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1};
+  Dia d{c->dia_val, c->dia_val16, c->dia_mask, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1};
   for (int k = 0; k < kDiaMax; k++) d.off[k] = c->dia_off[k];
   return d;
 }
@@ -813,13 +815,16 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     if (c->dia_ndiag > 0 && dia_pays(c->dia_ndiag, n, c->nnz_nd)) {
       const size_t slots = (size_t)c->dia_ndiag * (size_t)n;
       rc = dalloc(c, &c->dia_val, slots);
+      if (!rc) rc = dalloc(c, &c->dia_val16, slots);
       if (!rc) rc = dalloc(c, &c->dia_mask, (size_t)n);
       if (rc) {
         free_fp32(c);
         return rc;
       }
       HIPCHK(c, hipMemsetAsync(c->dia_val, 0, sizeof(float) * slots, c->stream));
-      HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_mask, c->stream));
+      HIPCHK(c, hipMemsetAsync(c->dia_val16, 0, sizeof(uint16_t) * slots, c->stream));
+      HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_val16,
+                           c->dia_mask, c->stream));
       c->dia_nd = true;
     } else if (ell_pays(w, n, c->nnz_nd)) {
       const size_t slots = (size_t)w * (size_t)n;
@@ -869,10 +874,14 @@ int cghip_set_poly(cghip_ctx *c, int degree, const double *coeffs, double b) {
   pp.two_over_b = (float)(2.0 / b);
   pp.degree = degree;
   // p > 0 on [0, b] (the float-rounded coefficients), evaluated in FP64 by
-  // Clenshaw's recurrence at y = 2 l / b - 1 on a uniform grid.
+  // Clenshaw's recurrence at y = 2 l / b - 1 on a uniform grid.  With the
+  // DIA layout the Clenshaw steps apply the FP16-rounded operator, whose
+  // entries differ by at most 2^-11 relative (|a_ij| <= 1 after scaling), so
+  // its spectrum lies in [-2^-10 b, (1 + 2^-10) b]: p is checked there.
   constexpr int kGrid = 4096;
+  const double ext = c->dia_nd ? 2.0 * std::ldexp(1.0, -10) : 0.0;  // in y units
   for (int i = 0; i < kGrid; i++) {
-    const double y = -1.0 + 2.0 * i / (kGrid - 1);
+    const double y = -1.0 - ext + 2.0 * (1.0 + ext) * i / (kGrid - 1);
     double b1 = 0.0, b2 = 0.0;
     for (int j = degree; j >= 1; j--) {
       const double t = (double)pp.c[j] + 2.0 * y * b1 - b2;

@@ -209,9 +209,13 @@ struct PolyParams {
 // is stored (absent entries are skipped, not multiplied by zero).  No
 // column indices: 4 bytes per slot and one byte per row, and x is read at
 // i + off[d], contiguous across a wave.  Slots are addressed by 32-bit byte
-// offsets, so dia_fits requires ndiag * n * 4 <= 2^32.
+// offsets, so dia_fits requires ndiag * n * 4 <= 2^32.  val16 holds the
+// same values rounded to FP16 (round to nearest even): the polynomial
+// preconditioner's Clenshaw steps read them (2 bytes per slot); the CG
+// operator itself stays FP32.
 struct Dia {
   const float *val;
+  const uint16_t *val16;
   const uint8_t *mask;
   int n, ndiag;
   int off[kDiaMax];
@@ -258,9 +262,10 @@ hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, flo
 // k * n + i, length[i].
 hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64_t *slot,
                       hipStream_t st);
-// The same entries in DIA (Dia): val zeroed by the caller, mask written.
+// The same entries in DIA (Dia): val and val16 zeroed by the caller, mask
+// written.
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
-                      uint8_t *mask, hipStream_t st);
+                      uint16_t *val16, uint8_t *mask, hipStream_t st);
 // x64 += scale o (double)y32 (scale nullable: plain add)
 hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s);
 // r64 = k - q64; partials.a = ||r64||^2 (pairs order)

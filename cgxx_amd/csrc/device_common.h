@@ -200,25 +200,45 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
 // ascending diagonal order (the CRS order of a row with ascending columns)
 // summed left to right from 0, then epi(r, s, pv).  Values and the mask are
 // read once (non-temporal); x at r + off[d] is contiguous across a wave.
-template <typename Pre, typename Epi>
+// HALF: the values come from val16 (FP16, converted exactly to FP32).
+template <bool HALF, typename Pre, typename Epi>
 __device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__ x, int r, Pre pre,
                                          Epi epi) {
   const uint32_t m = __builtin_nontemporal_load(A.mask + r);
   const auto pv = pre(r);
-  const char *vb = reinterpret_cast<const char *>(A.val);
+  const char *vb = HALF ? reinterpret_cast<const char *>(A.val16)
+                        : reinterpret_cast<const char *>(A.val);
   const char *xb = reinterpret_cast<const char *>(x);
-  const uint32_t r4 = (uint32_t)r * 4u, n4 = (uint32_t)A.n * 4u;
-  float v[kDiaMax], xv[kDiaMax];
+  constexpr uint32_t vs = HALF ? 2u : 4u;
+  const uint32_t nv = (uint32_t)A.n * vs, rv = (uint32_t)r * vs;
+  // Every load is issued before any is used and none waits for the mask:
+  // the values of all ndiag diagonals are read (absent ones are zeros of
+  // the same cache lines) and x at the column clamped into [0, n), so the
+  // mask only selects the products.  Raw value bits first: a conversion
+  // next to its load would wait for it.
+  uint32_t v[kDiaMax];
+  float xv[kDiaMax];
+  const int last = A.n - 1;
 #pragma unroll
   for (int d = 0; d < kDiaMax; d++)
-    if (d < A.ndiag && ((m >> d) & 1u)) {
-      v[d] = __builtin_nontemporal_load(reinterpret_cast<const float *>(vb + (r4 + (uint32_t)d * n4)));
-      xv[d] = *reinterpret_cast<const float *>(xb + (r4 + (uint32_t)A.off[d] * 4u));
+    if (d < A.ndiag) {
+      if (HALF)
+        v[d] = __builtin_nontemporal_load(
+            reinterpret_cast<const uint16_t *>(vb + (rv + (uint32_t)d * nv)));
+      else
+        v[d] = __builtin_nontemporal_load(
+            reinterpret_cast<const uint32_t *>(vb + (rv + (uint32_t)d * nv)));
+      const int col = min(max(r + A.off[d], 0), last);
+      xv[d] = *reinterpret_cast<const float *>(xb + (uint32_t)col * 4u);
     }
   float s = 0.0f;
 #pragma unroll
   for (int d = 0; d < kDiaMax; d++)
-    if (d < A.ndiag && ((m >> d) & 1u)) s += v[d] * xv[d];
+    if (d < A.ndiag && ((m >> d) & 1u)) {
+      const float a = HALF ? (float)__builtin_bit_cast(_Float16, (uint16_t)v[d])
+                           : __builtin_bit_cast(float, v[d]);
+      s += a * xv[d];
+    }
   return epi(r, s, pv);
 }
 

@@ -109,6 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
 // matrix is in D.off, checked on the host); val is zero where no entry is.
 __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const double *__restrict__ s,
                                                        Dia D, float *__restrict__ val,
+                                                       uint16_t *__restrict__ val16,
                                                        uint8_t *__restrict__ mask) {
   const int64_t T = (int64_t)gridDim.x * kBlock;
   const int64_t n = A.n;
@@ -121,7 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
       const int off = c - (int)i;
       int d = 0;
       while (d < D.ndiag - 1 && D.off[d] != off) d++;
-      val[(int64_t)d * n + i] = (float)((A.value[e] * si) * s[c]);
+      const float v = (float)((A.value[e] * si) * s[c]);
+      val[(int64_t)d * n + i] = v;
+      val16[(int64_t)d * n + i] = __builtin_bit_cast(uint16_t, (_Float16)v);
       m |= 1u << d;
     }
     mask[i] = (uint8_t)m;
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
 // thread owns, pv = pre(i) early, then epi(i, t, pv) with the sum t of the
 // stored products of row i (from 0, in stored order); returns the sum of
 // the epilogue's dot contributions.
-template <int K, typename Pre, typename Epi>
+template <int K, bool HALF = false, typename Pre, typename Epi>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
                                    Epi epi) {
   double tpart = 0.0;
@@ -140,7 +143,7 @@ __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, f
     const int nrb = ell_chunks(A.dia.n);
     for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
       const int r = rb * kBlock + (int)threadIdx.x;
-      if (r < A.dia.n) tpart += dia_row_pre_epi(A.dia, x, r, pre, epi);
+      if (r < A.dia.n) tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi);
     }
   } else if (K == kOpEll) {
     const int nrb = ell_chunks(A.ell.n);
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   if (st && st->cur.done) return;
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
-  const double tpart = op32_rows<K>(
+  const double tpart = op32_rows<K, K == kOpDia>(
       A, b1, prod, [&](int i) { return float4{b1[i], r[i], b2 ? b2[i] : 0.0f, 0.0f}; },
       [&](int i, float t0, float4 pv) -> double {
         const float bi = pv.x, ri = pv.y, bb = pv.z;
@@ -606,8 +609,9 @@ hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64
 }
 
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
-                      uint8_t *mask, hipStream_t st) {
-  hipLaunchKernelGGL(k_scaled_dia, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, D, val, mask);
+                      uint16_t *val16, uint8_t *mask, hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_dia, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, D, val, val16,
+                     mask);
   return hipGetLastError();
 }
 

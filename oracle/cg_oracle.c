@@ -605,8 +605,29 @@ typedef struct {
   int N, unit, nrb, gmax;
   int *ptr, *index, *starts;
   float *value;
-  double *s; /* scaled: s_i, else NULL */
+  float *value16; /* DIA: the values rounded to FP16 (Clenshaw steps), else NULL */
+  double *s;      /* scaled: s_i, else NULL */
 } op32;
+
+/* v rounded to the nearest IEEE binary16 value, ties to even (the
+ * conversion of the library's DIA builder): 11 significant bits for
+ * |v| >= 2^-14, multiples of 2^-24 below, +-inf from 65520 up.  Division
+ * and multiplication by powers of two are exact, and nearbyintf rounds to
+ * nearest even in the default rounding mode. */
+float orc_round_half(float v) {
+  if (v != v) return v;
+  float a = fabsf(v);
+  if (a >= 65520.0f) return copysignf(INFINITY, v);
+  float q;
+  if (a < 6.103515625e-05f) { /* 2^-14: subnormal halves */
+    q = 5.9604644775390625e-08f; /* 2^-24 */
+  } else {
+    int e;
+    frexpf(a, &e); /* a = m 2^e, m in [0.5, 1) */
+    q = ldexpf(1.0f, e - 11);
+  }
+  return copysignf(nearbyintf(a / q) * q, v);
+}
 
 /* The library's choice of the DIA layout for the scaled operator
  * (cghip_prepare_fp32, include/cghip.h): every row's columns ascending, the
@@ -631,7 +652,7 @@ static int dia_chosen(const orc_problem *p, int64_t nnz_nd) {
 }
 
 static void op32_free(op32 *o) {
-  free(o->ptr), free(o->index), free(o->starts), free(o->value), free(o->s);
+  free(o->ptr), free(o->index), free(o->starts), free(o->value), free(o->value16), free(o->s);
 }
 
 /* Returns 0, or -3 if scaled and a row has no positive diagonal entry. */
@@ -684,7 +705,11 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
       o->nrb = (N + ORC_ROWS_MAX - 1) / ORC_ROWS_MAX;
       for (int b = 0; b <= o->nrb; b++)
         o->starts[b] = b * ORC_ROWS_MAX < N ? b * ORC_ROWS_MAX : N;
-      if (dia) o->gmax = ORC_ELL_GMAX;
+      if (dia) {
+        o->gmax = ORC_ELL_GMAX;
+        o->value16 = (float *)malloc(sizeof(float) * (size_t)(k > 0 ? k : 1));
+        for (int e = 0; e < k; e++) o->value16[e] = orc_round_half(o->value[e]);
+      }
     }
   }
   return 0;
@@ -693,6 +718,14 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
 static float op32_row(const op32 *o, const float *x, int i) {
   float t = 0.0f;
   for (int e = o->ptr[i]; e < o->ptr[i + 1]; e++) t += o->value[e] * x[o->index[e]];
+  return o->unit ? x[i] + t : t;
+}
+
+/* The operator of the Clenshaw steps: the FP16-rounded values in DIA. */
+static float op32_row_poly(const op32 *o, const float *x, int i) {
+  if (!o->value16) return op32_row(o, x, i);
+  float t = 0.0f;
+  for (int e = o->ptr[i]; e < o->ptr[i + 1]; e++) t += o->value16[e] * x[o->index[e]];
   return o->unit ? x[i] + t : t;
 }
 
@@ -711,7 +744,7 @@ static void poly_clenshaw(const op32 *o, const float *u, const orc_poly *pl,
     float *b0 = w[j % 3];
     const float cj = pl->c[j];
     for (int i = 0; i < o->N; i++) {
-      const float t = op32_row(o, b1, i);
+      const float t = op32_row_poly(o, b1, i);
       const float sv = tb * t - b1[i];
       const float bb = b2 ? b2[i] : 0.0f;
       b0[i] = j > 0 ? (cj * u[i] + 2.0f * sv) - bb : (cj * u[i] + sv) - bb;

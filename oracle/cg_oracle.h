@@ -187,8 +187,13 @@ typedef struct {
   float two_over_b;
 } orc_poly;
 
+/* v rounded to binary16 (nearest, ties to even), returned as a float. */
+float orc_round_half(float v);
+
 /* z = p(A_hat) r in FP32 on the scaled operator A_hat = D^-1/2 A D^-1/2
- * (built as in orc_mixed_pcg_poly), with the library's expression order. */
+ * (built as in orc_mixed_pcg_poly), with the library's expression order;
+ * when the library stores A_hat in DIA, the Clenshaw steps use its values
+ * rounded to FP16. */
 int orc_poly_apply(int N, const int *ptr, const int *index,
                    const double *value, const float *r32, const orc_poly *pl,
                    float *z);

@@ -14,6 +14,10 @@ Pins (none of these has a reference counterpart):
     <B u, v> = <u, B v>.
   - The oracle's mixed solver with the polynomial reaches the FP64 true
     residual and x = 1.
+  - FP16 rounding of the DIA operator's values (the Clenshaw steps read
+    them): against numpy's float16 conversion (round to nearest even, with
+    subnormals and overflow), and the Clenshaw application on a DIA matrix
+    against the dense FP64 operator built from the FP16-rounded A_hat.
 """
 import numpy as np
 import pytest
@@ -162,3 +166,62 @@ def test_oracle_mixed_poly_converges(degree):
     assert info["true_residual"] <= 1e-9
     assert np.abs(x - 1).max() <= 1e-5
     assert info["iterations"] < infoj["iterations"]
+
+
+def test_oracle_round_half_matches_numpy():
+    import ctypes as C
+    L = oracle.lib()
+    L.orc_round_half.restype = C.c_float
+    L.orc_round_half.argtypes = [C.c_float]
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([
+        rng.standard_normal(4000), rng.uniform(-1, 1, 4000) * 1e-5,  # normal and subnormal halves
+        rng.uniform(-1, 1, 1000) * 1e-8, rng.uniform(-7e4, 7e4, 1000),  # underflow, overflow
+        # exact ties between neighbouring halves (round to even) and their neighbours
+        (np.arange(1, 2049) + 0.5) / 2048.0, (np.arange(1, 200) + 0.5) * 2.0 ** -24,
+        [0.0, -0.0, 65504.0, 65519.99, 65520.0, 2.0 ** -14, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26],
+    ]).astype(np.float32)
+    got = np.array([L.orc_round_half(float(v)) for v in vals], dtype=np.float32)
+    want = vals.astype(np.float16).astype(np.float32)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def _fp16_poly_operator(m, coeffs, b):
+    """p(A16) with A16 the library's DIA values: FP32 (a_ij s_i) s_j rounded
+    to FP16, unit diagonal; FP64 three-term recurrence (brute force)."""
+    n = m.n
+    s = 1.0 / np.sqrt(m.diagonal())
+    A = np.eye(n)
+    for i in range(n):
+        for e in range(m.ptr[i], m.ptr[i + 1]):
+            j = m.index[e]
+            if j != i:
+                v32 = np.float32((m.value[e] * s[i]) * s[j])
+                A[i, j] = float(np.float16(v32))
+    Y = 2.0 * A / b - np.eye(n)
+    T0, T1 = np.eye(n), Y
+    P = coeffs[0] * T0 + coeffs[1] * T1
+    for j in range(2, len(coeffs)):
+        T0, T1 = T1, 2.0 * Y @ T1 - T0
+        P = P + coeffs[j] * T1
+    return P
+
+
+@pytest.mark.parametrize("degree", [2, 8])
+def test_oracle_poly_apply_fp16_dia(degree):
+    """A 3D variable-coefficient diffusion lies on 6 diagonals: the oracle
+    applies the polynomial with the FP16-rounded values, within FP32 rounding
+    of the dense FP16 operator and far from the FP32 one."""
+    m = problems.diffusion3d(6, sigma=1.0, corr=2.0, seed=3)
+    b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+    c = spectrum.cheb_residual_coeffs(b / 50, b, degree)
+    c32 = c.astype(np.float32).astype(np.float64)
+    b32 = 2.0 / float(np.float32(2.0 / b))
+    B16 = _fp16_poly_operator(m, c32, b32)
+    B32 = dense_poly_operator(dense(m), c32, b32)
+    r = np.random.default_rng(degree).standard_normal(m.n).astype(np.float32)
+    z = oracle.poly_apply((m.ptr, m.index, m.value), r, c, b).astype(np.float64)
+    tol = 2e-6 * (degree + 1) * np.abs(c32).sum() * np.abs(r).max()
+    e16 = np.abs(z - B16 @ r).max()
+    assert e16 <= tol
+    assert np.abs(z - B32 @ r).max() > 10 * e16
