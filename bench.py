@@ -334,7 +334,8 @@ def main() -> int:
             "unit": "GB/s",
             "frac": step_gbps / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(f"k_poly_step<false, {kind}>"),
-            "kernel": f"k_poly_step<false, {kind}> (Clenshaw step, FP32 {layout} scaled operator)",
+            "kernel": f"k_poly_step<false, {kind}> (Clenshaw step, scaled operator in {layout}"
+                      + (", FP16 values)" if layout == "DIA" else ", FP32 values)"),
             "polynomial": prof_src,
             "algorithmic_bytes_per_launch": step_bytes,
         },
