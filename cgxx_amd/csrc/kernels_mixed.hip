@@ -135,19 +135,40 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
 // thread owns, pv = pre(i) early, then epi(i, t, pv) with the sum t of the
 // stored products of row i (from 0, in stored order); returns the sum of
 // the epilogue's dot contributions.
-template <int K, bool HALF = false, typename Pre, typename Epi>
+// Chunks of block blockIdx.x: rb0, rb0 + stride, ... below rend.  Round
+// robin (block g owns chunks g, g + G, ...: the order the dot trees and the
+// oracle assume), or, for kernels without a reduction, XCD-contiguous:
+// blocks are dispatched to the 8 XCDs round robin, so block g runs on XCD
+// g % 8, and that XCD's blocks walk its own contiguous eighth of the
+// chunks; the +-1, +-S and +-S^2 neighbours of a stencil row then mostly
+// hit the same L2 (3.5% faster Clenshaw steps at 256^3).  Needs G % 8 == 0
+// (else round robin).
+template <bool XCD>
+__device__ inline void chunk_walk(int nrb, int &rb0, int &stride, int &rend) {
+  rb0 = blockIdx.x, stride = gridDim.x, rend = nrb;
+  if (XCD && (gridDim.x & 7) == 0) {
+    const int x = blockIdx.x & 7, R = (nrb + 7) >> 3;
+    rb0 = x * R + (blockIdx.x >> 3);
+    stride = gridDim.x >> 3;
+    rend = min(nrb, (x + 1) * R);
+  }
+}
+
+template <int K, bool HALF = false, bool XCD = false, typename Pre, typename Epi>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
                                    Epi epi) {
   double tpart = 0.0;
   if (K == kOpDia) {
-    const int nrb = ell_chunks(A.dia.n);
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    int rb0, stride, rend;
+    chunk_walk<XCD>(ell_chunks(A.dia.n), rb0, stride, rend);
+    for (int rb = rb0; rb < rend; rb += stride) {
       const int r = rb * kBlock + (int)threadIdx.x;
       if (r < A.dia.n) tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi);
     }
   } else if (K == kOpEll) {
-    const int nrb = ell_chunks(A.ell.n);
-    for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x) {
+    int rb0, stride, rend;
+    chunk_walk<XCD>(ell_chunks(A.ell.n), rb0, stride, rend);
+    for (int rb = rb0; rb < rend; rb += stride) {
       const int r = rb * kBlock + (int)threadIdx.x;
       if (r < A.ell.n) tpart += ell_row_pre_epi(A.ell, x, r, pre, epi);
     }
@@ -508,7 +529,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   if (st && st->cur.done) return;
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
-  const double tpart = op32_rows<K, K == kOpDia>(
+  const double tpart = op32_rows<K, K == kOpDia, !FINAL>(
       A, b1, prod, [&](int i) { return float4{b1[i], r[i], b2 ? b2[i] : 0.0f, 0.0f}; },
       [&](int i, float t0, float4 pv) -> double {
         const float bi = pv.x, ri = pv.y, bb = pv.z;
