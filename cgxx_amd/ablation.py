@@ -1,8 +1,9 @@
 """What the trained network gains over the method without it, on held-out
 systems of the anisotropic diffusion family, measured on the GPU.
 
-For each held-out seed, four solves from x = 0 to FP64 true relative
-residual 1e-9:
+For each held-out seed, five solves from x = 0 to FP64 true relative
+residual 1e-9, each timed after an untimed run of the same method on the
+same system:
   learned  mixed precision, polynomial from the network (features and
            inference timed with the solve), Jacobi fallback
   fixed    mixed precision, Chebyshev on [theta_min(10) / rho*, b] (rho*
@@ -77,10 +78,15 @@ def main() -> int:
         ctx.upload("K", k)
         del ptr, idx, val, k, C
         r = {"seed": seed, "n": n}
+        # each method runs once untimed first, so every timed run finds its
+        # graphs and buffers for this system already built (as in bench.py)
+        learned.solve_learned(ctx, mlp, model, n)
         lr = learned.solve_learned(ctx, mlp, model, n)
         r.update({f"learned_{key}": lr[key] for key in ("spmvs", "wall_s", "true_residual",
                                                            "poly_iterations", "jacobi_iterations")})
         r["learned_method"] = lr["method"]
+        f, b, theta10 = learned.system_features(ctx, n)
+        learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b)
         t1 = time.perf_counter()
         f, b, theta10 = learned.system_features(ctx, n)
         fr = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b)
@@ -88,6 +94,7 @@ def main() -> int:
         r.update({f"fixed_{key}": fr[key] for key in ("spmvs", "wall_s", "true_residual",
                                                          "poly_iterations", "jacobi_iterations")})
         r["fixed_method"] = fr["method"]
+        learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, 1.0), b)
         t1 = time.perf_counter()
         f, b, theta10 = learned.system_features(ctx, n)
         rr_ = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, 1.0), b)
@@ -96,9 +103,13 @@ def main() -> int:
                                                          "poly_iterations", "jacobi_iterations")})
         r["ritz_method"] = rr_["method"]
         ctx.fill("X", 0.0)
+        ctx.solve_mixed(precond="jacobi")
+        ctx.fill("X", 0.0)
         jm = ctx.solve_mixed(precond="jacobi")
         r.update({"jacobi_spmvs": jm["iterations"], "jacobi_wall_s": jm["wall_s"],
                   "jacobi_true_residual": jm["true_residual"]})
+        ctx.fill("X", 0.0)
+        ctx.solve(precond="jacobi", max_iter=100000, tol=1e-9)
         ctx.fill("X", 0.0)
         f64 = ctx.solve(precond="jacobi", max_iter=100000, tol=1e-9)
         nr, nb = ctx.true_residual("X", "K")

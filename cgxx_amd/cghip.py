@@ -28,6 +28,7 @@ SYMBOLS = [
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
     "cghip_set_poly", "cghip_matrix_stats", "cghip_time_poly_step", "cghip_fp32_layout",
+    "cghip_mixed_iteration_bytes",
     "cghip_mlp_param_count", "cghip_mlp_create", "cghip_mlp_destroy", "cghip_mlp_last_error",
     "cghip_mlp_set_params", "cghip_mlp_get_params", "cghip_mlp_forward", "cghip_mlp_gradient",
     "cghip_mlp_train",
@@ -117,6 +118,7 @@ def load() -> C.CDLL:
     L.cghip_matrix_stats.argtypes = [vp, _f64p, C.c_int]
     L.cghip_time_poly_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.cghip_fp32_layout.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cghip_mixed_iteration_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.cghip_mlp_param_count.argtypes = [C.c_int] * 4
     L.cghip_mlp_create.argtypes = [C.c_int] * 6 + [C.POINTER(vp)]
@@ -246,6 +248,13 @@ class Context:
 
     def prepare_fp32(self):
         self._chk(self.L.cghip_prepare_fp32(self._h), "cghip_prepare_fp32")
+
+    def mixed_iteration_bytes(self, precond: str) -> float:
+        """Algorithmic bytes of one mixed-precision iteration (cost model)."""
+        v = C.c_double()
+        self._chk(self.L.cghip_mixed_iteration_bytes(self._h, PRECOND[precond], C.byref(v)),
+                  "cghip_mixed_iteration_bytes")
+        return v.value
 
     def fp32_layout(self) -> str | None:
         """Layout of the scaled FP32 operator: "CRS", "ELL", "DIA" or None."""

@@ -1195,6 +1195,39 @@ int cghip_matrix_stats(cghip_ctx *c, double *stats, int cap) {
   return CGHIP_OK;
 }
 
+int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) {
+  if (!c || !bytes) return CGHIP_ERR_ARG;
+  auto *cc = const_cast<cghip_ctx *>(c);
+  if (!c->has_fp32) return fail(cc, CGHIP_ERR_STATE, "call cghip_prepare_fp32 first");
+  const double n = c->n;
+  if (precond == CGHIP_PRECOND_NONE) {
+    // FP32 CRS copy of A: values, indices, row pointer; x, q; y, r, p, q; r, p
+    *bytes = 8.0 * c->crs_nnz + 4.0 * (n + 1) + 8.0 * n + 24.0 * n + 12.0 * n;
+    return CGHIP_OK;
+  }
+  if (precond != CGHIP_PRECOND_JACOBI && precond != CGHIP_PRECOND_POLY)
+    return fail(cc, CGHIP_ERR_ARG, "unknown preconditioner %d", precond);
+  if (!c->has_scaled) return fail(cc, CGHIP_ERR_STATE, "no scaled operator");
+  if (precond == CGHIP_PRECOND_POLY && !c->has_poly)
+    return fail(cc, CGHIP_ERR_STATE, "call cghip_set_poly first");
+  const double nnz = c->nnz_nd, d = c->dia_ndiag;
+  // operator stream of one SpMV: FP32 values in the CG SpMV; the Clenshaw
+  // steps read DIA's FP16 copy
+  const double op32 = c->dia_nd ? 4.0 * d * n + n : c->ell_nd ? 8.0 * nnz + 4.0 * n
+                                                              : 8.0 * nnz + 4.0 * (n + 1);
+  const double op16 = c->dia_nd ? 2.0 * d * n + n : op32;
+  const double spmv = op32 + 8.0 * n;
+  if (precond == CGHIP_PRECOND_JACOBI) {
+    *bytes = spmv + 24.0 * n + 12.0 * n;  // + update (y, r, p, q) + p update (r, p)
+    return CGHIP_OK;
+  }
+  const double k = c->poly_degree;
+  // update also writes B_k; each Clenshaw step reads B_{j+1}, r, writes
+  // B_j, and all but the first read B_{j+2}; p update reads z, p
+  *bytes = spmv + 28.0 * n + k * (op16 + 12.0 * n) + (k - 1.0) * 4.0 * n + 12.0 * n;
+  return CGHIP_OK;
+}
+
 int cghip_time_poly_step(cghip_ctx *c, int reps, double *avg_ms) {
   if (!c) return CGHIP_ERR_ARG;
   if (reps <= 0 || !avg_ms) return fail(c, CGHIP_ERR_ARG, "invalid arguments");

@@ -204,23 +204,14 @@ bool LearnedPoly::solve(int n, double tol, int maxIter, LearnedResult &res, std:
   int budgetIters = std::max(1, maxIter / (k + 1));
   bool usePoly = accepted;
   if (accepted && mode_ != "fixed") {
-    if (!costsKnown_) {
-      cghip_mixed_opts mo{CGHIP_PRECOND_JACOBI, 3, 1e-300, 0.0, 0, CGHIP_PROFILE_KERNELS};
-      cghip_mixed_info mi{};
-      if (cghip_fill_vector(ctx_, CGHIP_VEC_X, 0.0) || cghip_solve_mixed(ctx_, &mo, &mi))
-        return fail("timing a Jacobi iteration");
-      msJacobiIter_ = (mi.ms_matvec + mi.ms_update + mi.ms_xpay) / std::max(mi.n_matvec, 1);
-      mo.precond = CGHIP_PRECOND_POLY;
-      mo.max_iter = 2;
-      mi = cghip_mixed_info{};
-      if (cghip_fill_vector(ctx_, CGHIP_VEC_X, 0.0) || cghip_solve_mixed(ctx_, &mo, &mi))
-        return fail("timing a polynomial iteration");
-      msPolyIter_ = (mi.ms_matvec + mi.ms_update + mi.ms_poly + mi.ms_xpay) / std::max(mi.n_matvec, 1);
-      costsKnown_ = true;
-    }
-    const double costJ = msJacobiIter_ * itJ, costP = msPolyIter_ * itP;
+    // the library's algorithmic bytes per iteration rank the two by time
+    double bytesJ = 0, bytesP = 0;
+    if (cghip_mixed_iteration_bytes(ctx_, CGHIP_PRECOND_JACOBI, &bytesJ) ||
+        cghip_mixed_iteration_bytes(ctx_, CGHIP_PRECOND_POLY, &bytesP))
+      return fail("iteration bytes");
+    const double costJ = bytesJ * itJ, costP = bytesP * itP;
     usePoly = costP < costJ;
-    budgetIters = std::max(1, (int)std::ceil(1.25 * costJ / msPolyIter_));
+    budgetIters = std::max(1, (int)std::ceil(1.25 * costJ / bytesP));
   }
 
   // ---- solve, with the Jacobi fallback ----

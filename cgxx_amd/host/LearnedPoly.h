@@ -6,8 +6,9 @@
 // on the GPU for log p at the Chebyshev points of [0, b] and the predicted
 // iteration counts, the discrete Chebyshev transform to coefficients, a
 // choice between the polynomial and Jacobi from the predicted iterations
-// times the per-iteration times (timed once on the device), and the Jacobi
-// fallback judged by the FP64 true residual.  No reference counterpart.
+// times the library's bytes per iteration (cghip_mixed_iteration_bytes), and
+// the Jacobi fallback judged by the FP64 true residual.  No reference
+// counterpart.
 #pragma once
 
 #include <string>
@@ -51,8 +52,6 @@ class LearnedPoly {
   PolyModel model_;
   std::string mode_;
   cghip_mlp *mlp_ = nullptr;
-  bool costsKnown_ = false;
-  double msJacobiIter_ = 0, msPolyIter_ = 0;
 };
 
 // Extreme eigenvalues of the Lanczos tridiagonal of m PCG steps

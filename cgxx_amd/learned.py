@@ -243,29 +243,18 @@ def predict_coeffs(mlp, model, f, b: float) -> np.ndarray:
 
 
 def iteration_costs(ctx, coeffs, b: float) -> tuple[float, float] | None:
-    """Device time (ms) of one mixed-precision iteration with Jacobi and with
-    the polynomial on the uploaded system, from two short kernel-timed
-    solves; cached on the context per system and degree.  None if the
-    library refuses the polynomial."""
+    """Algorithmic bytes of one mixed-precision iteration with Jacobi and
+    with the polynomial on the uploaded system (the library's byte model,
+    cghip_mixed_iteration_bytes: its loop kernels stream at similar rates,
+    so bytes rank the two by time).  None if the library refuses the
+    polynomial."""
     from .cghip import CGHIPError
 
-    key = (ctx.n, len(coeffs) - 1)
-    cache = getattr(ctx, "_iteration_costs", {})
-    if key in cache:
-        return cache[key]
     try:
         ctx.set_poly(coeffs, b)
     except CGHIPError:
         return None
-    ctx.fill("X", 0.0)
-    j = ctx.solve_mixed(precond="jacobi", max_iter=3, tol=1e-300, profile=1)
-    ctx.fill("X", 0.0)
-    p = ctx.solve_mixed(precond="poly", max_iter=2, tol=1e-300, profile=1)
-    tj = (j["ms_matvec"] + j["ms_update"] + j["ms_xpay"]) / max(j["n_matvec"], 1)
-    tp = (p["ms_matvec"] + p["ms_update"] + p["ms_poly"] + p["ms_xpay"]) / max(p["n_matvec"], 1)
-    cache[key] = (tj, tp)
-    ctx._iteration_costs = cache
-    return cache[key]
+    return ctx.mixed_iteration_bytes("jacobi"), ctx.mixed_iteration_bytes("poly")
 
 
 def fixed_rule_coeffs(theta10: float, b: float, rho: float) -> np.ndarray:
@@ -324,12 +313,13 @@ def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 10
 
     With iteration predictions (11-output model) the predicted costs decide:
     the polynomial runs only if (predicted polynomial iterations) x (its
-    per-iteration time) is below the same product for Jacobi, and then with
-    an SpMV budget worth 1.25x Jacobi's predicted time; past it, or if the
+    bytes per iteration) is below the same product for Jacobi, and then with
+    an SpMV budget worth 1.25x Jacobi's predicted cost; past it, or if the
     library refuses the polynomial, Jacobi takes over
     (solve_poly_with_fallback).  Otherwise Jacobi runs from the start.
-    The per-iteration times are timed once per system and context
-    (iteration_costs).  The wall time includes features and inference."""
+    The per-iteration costs are the library's algorithmic bytes per
+    iteration (iteration_costs).  The wall time includes features and
+    inference."""
     t0 = time.perf_counter()
     f, b, _ = system_features(ctx, n)
     pr = predict(mlp, model, f, b)
@@ -339,12 +329,12 @@ def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 10
     if "it_jacobi" in pr and budget is None:
         costs = iteration_costs(ctx, c, b)
         if costs is not None:
-            tj, tp = costs
-            cost_j, cost_p = tj * pr["it_jacobi"], tp * pr["it_poly"]
+            bj, bp = costs
+            cost_j, cost_p = bj * pr["it_jacobi"], bp * pr["it_poly"]
             decision = {"it_jacobi_pred": pr["it_jacobi"], "it_poly_pred": pr["it_poly"],
-                        "ms_jacobi_iter": tj, "ms_poly_iter": tp,
+                        "bytes_jacobi_iter": bj, "bytes_poly_iter": bp,
                         "use_poly": cost_p < cost_j}
-            budget = int(np.ceil(1.25 * cost_j / tp)) * (k + 1)
+            budget = int(np.ceil(1.25 * cost_j / bp)) * (k + 1)
     if decision is not None and not decision["use_poly"]:
         ctx.fill("X", 0.0)
         info = ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=tol)

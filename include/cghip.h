@@ -248,6 +248,13 @@ int cghip_time_matvec(cghip_ctx *ctx, int x, int y, int reps, double *avg_ms);
  * events, divided by reps x degree.  Overwrites the FP32 work vectors. */
 int cghip_time_poly_step(cghip_ctx *ctx, int reps, double *avg_ms);
 
+/* Algorithmic HBM bytes of one iteration of cghip_solve_mixed with the given
+ * preconditioner (CGHIP_PRECOND_NONE, _JACOBI or _POLY with the degree of
+ * cghip_set_poly) on the prepared FP32 operator: each array a kernel must
+ * read or write counted once (the SpMV's x once).  A cost model for choosing
+ * between the preconditioners: the kernels stream at similar rates. */
+int cghip_mixed_iteration_bytes(const cghip_ctx *ctx, int precond, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,3 +138,42 @@ def test_matrix_stats_rejects_missing_diagonal(ctx):
     ctx.upload_crs(ptr, idx, val)
     with pytest.raises(CGHIPError, match="diagonal"):
         ctx.matrix_stats()
+
+
+@pytest.mark.parametrize("layout_case", ["dia", "ell"])
+def test_iteration_bytes_rank_like_device_time(ctx, layout_case):
+    """The byte model that chooses between the polynomial and Jacobi
+    (cghip_mixed_iteration_bytes) against the device: the ratio of the
+    polynomial's to Jacobi's bytes per iteration is within 30% of the ratio
+    of their kernel-timed iterations on a 2M-row system (DIA) and on a
+    random-coupling system stored in ELL."""
+    from cgxx_amd import spectrum
+
+    if layout_case == "dia":
+        m = problems.diffusion3d(128, 1.0, 8.0, 5)
+    else:  # the same grid renumbered at random: 7 entries per row on many diagonals
+        import scipy.sparse as sp
+
+        d = problems.diffusion3d(128, 1.0, 8.0, 5)
+        perm = np.random.default_rng(1).permutation(d.n)
+        B = sp.csr_matrix((d.value, d.index, d.ptr), shape=(d.n, d.n))[perm][:, perm]
+        B.sort_indices()
+        m = problems.CRS(d.n, B.indptr.astype(np.int32), B.indices.astype(np.int32),
+                         B.data.astype(np.float64))
+    ctx.set_size(m.n)
+    ctx.upload_crs(m.ptr, m.index, m.value)
+    ctx.set_format("CRS")
+    ctx.prepare_fp32()
+    ctx.upload("K", m.ones_rhs())
+    assert ctx.fp32_layout() == layout_case.upper()
+    b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+    ctx.set_poly(spectrum.cheb_residual_coeffs(b / 100.0, b, 8), b)
+    bj, bp = ctx.mixed_iteration_bytes("jacobi"), ctx.mixed_iteration_bytes("poly")
+    assert bp > bj > 0
+    ms = {}
+    for pre, its in (("jacobi", 20), ("poly", 5)):
+        ctx.fill("X", 0.0)
+        info = ctx.solve_mixed(precond=pre, max_iter=its, tol=1e-300, profile=True)
+        ms[pre] = (info["ms_matvec"] + info["ms_update"] + info["ms_poly"] + info["ms_xpay"]) / info["n_matvec"]
+    ratio_t, ratio_b = ms["poly"] / ms["jacobi"], bp / bj
+    assert abs(ratio_t / ratio_b - 1.0) <= 0.3, (ratio_t, ratio_b)
