@@ -76,7 +76,8 @@ struct cghip_ctx {
   int *ptr_nd = nullptr, *index_nd = nullptr, *rb_nd = nullptr;
   float *value_nd = nullptr;
   double *scale64 = nullptr;
-  // the scaled operator in column-major ELL when its padding is small
+  // the scaled operator in column-major packed ELL when its padding is
+  // small (and DIA does not apply)
   bool ell_nd = false;
   int ell_w_nd = 0;
   int *ell_len_nd = nullptr;

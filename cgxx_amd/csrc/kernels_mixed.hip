@@ -8,8 +8,10 @@
 // data.  With the Jacobi or polynomial preconditioner it runs on the scaled
 // operator A_hat = D^-1/2 A D^-1/2 whose unit diagonal is implicit, so a
 // CG step on A_hat is a Jacobi-PCG step on A without any preconditioner
-// vector in the loop: the SpMV streams 8 bytes per off-diagonal entry
-// (FP32 value + int32 index) and the vector kernels only y, r, p, q.  Every
+// vector in the loop: the SpMV streams 4 bytes per off-diagonal entry in
+// the DIA layout (FP32 value; 2 in the polynomial's Clenshaw steps, which
+// read an FP16 copy) or 8 in ELL and CRS (value + int32 index), and the
+// vector kernels only y, r, p, q.  Every
 // dot product is accumulated in FP64 from the products (double)a * (double)b,
 // which are exact for float operands, so only the summation rounds, in a
 // fixed tree order.  alpha and beta are computed in FP64 and rounded to FP32

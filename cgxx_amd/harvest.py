@@ -11,7 +11,7 @@ and compute on the host
   - the simulated iterations of the fixed rule a = theta_min(10 steps) / rho
     for rho in RHO, the method without the network.
 
-    python -m cgxx_amd.harvest --seeds 0:1000 --out harvest.npz [--workers 16]
+    python -m cgxx_amd.harvest --seeds 0:1000 --out harvest.npz [--workers 16] [--sides 176,256]
 
 Prints one progress line per 20 systems.
 """
@@ -28,10 +28,10 @@ RHO = 2.0 ** np.arange(0, 11)  # 1 .. 1024
 MAX_ITER = 8000
 
 
-def _generate(seed: int):
+def _generate(seed: int, sides=None):
     from cgxx_amd import problems
 
-    m = problems.anisotropic_family(seed)
+    m = problems.anisotropic_family(seed) if sides is None else problems.anisotropic_family(seed, sides)
     return seed, m.ptr, m.index, m.value, m.ones_rhs(), 1.0 / m.diagonal(), m.info
 
 
@@ -55,8 +55,11 @@ def main() -> int:
     ap.add_argument("--out", required=True)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--inflight", type=int, default=20, help="systems generated ahead at most")
+    ap.add_argument("--sides", default=None,
+                    help="comma-separated grid sides to draw from (default: 64..160 by 16)")
     args = ap.parse_args()
     lo, hi = (int(v) for v in args.seeds.split(":"))
+    sides = tuple(int(v) for v in args.sides.split(",")) if args.sides else None
     seeds = list(range(lo, hi))
 
     # Worker processes start before this process touches the GPU.
@@ -77,7 +80,7 @@ def main() -> int:
     def top_up():
         nonlocal nxt
         while nxt < len(seeds) and len(gen_q) < args.inflight:
-            gen_q.append(pool.apply_async(_generate, (seeds[nxt],)))
+            gen_q.append(pool.apply_async(_generate, (seeds[nxt], sides)))
             nxt += 1
 
     top_up()
