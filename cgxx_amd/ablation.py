@@ -12,7 +12,7 @@ same system:
   jacobi   mixed precision, Jacobi
   fp64     the plain FP64 Jacobi-PCG loop (the reference's method)
 
-    python -m cgxx_amd.ablation --seeds 100000:100200 [--model PATH] [--out res.npz]
+    python -m cgxx_amd.ablation --seeds 100000:100200 [--model PATH] [--out res.npz] [--sides 176,256]
 
 Prints a progress line per 10 systems and a JSON summary at the end.
 """
@@ -27,10 +27,10 @@ import time
 import numpy as np
 
 
-def _generate(seed: int):
+def _generate(seed: int, sides=None):
     from cgxx_amd import problems
 
-    m = problems.anisotropic_family(seed)
+    m = problems.anisotropic_family(seed) if sides is None else problems.anisotropic_family(seed, sides)
     return seed, m.ptr, m.index, m.value, m.ones_rhs(), 1.0 / m.diagonal()
 
 
@@ -41,9 +41,12 @@ def main() -> int:
     ap.add_argument("--out", default=None)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--inflight", type=int, default=12)
+    ap.add_argument("--sides", default=None,
+                    help="comma-separated grid sides to draw from (default: 64..160 by 16)")
     args = ap.parse_args()
     lo, hi = (int(v) for v in args.seeds.split(":"))
     seeds = list(range(lo, hi))
+    sides = tuple(int(v) for v in args.sides.split(",")) if args.sides else None
     pool = mp.get_context("spawn").Pool(args.workers)
 
     from cgxx_amd import learned
@@ -61,7 +64,7 @@ def main() -> int:
     def top_up():
         nonlocal nxt
         while nxt < len(seeds) and len(q) < args.inflight:
-            q.append(pool.apply_async(_generate, (seeds[nxt],)))
+            q.append(pool.apply_async(_generate, (seeds[nxt], sides)))
             nxt += 1
 
     top_up()
