@@ -822,12 +822,24 @@ int cghip_prepare_fp32(cghip_ctx *c) {
         free_fp32(c);
         return rc;
       }
+      int *d_over = reinterpret_cast<int *>(c->dot_out);
+      HIPCHK(c, hipMemsetAsync(d_over, 0, sizeof(int), c->stream));
       HIPCHK(c, hipMemsetAsync(c->dia_val, 0, sizeof(float) * slots, c->stream));
       HIPCHK(c, hipMemsetAsync(c->dia_val16, 0, sizeof(uint16_t) * slots, c->stream));
       HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_val16,
-                           c->dia_mask, c->stream));
+                           c->dia_mask, d_over, c->stream));
+      int over = 0;
+      HIPCHK(c, hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      if (int r = sync(c)) return r;
       c->dia_nd = true;
-    } else if (ell_pays(w, n, c->nnz_nd)) {
+      if (over) {  // no FP16 copy possible: the next layout instead
+        dfree(c->dia_val);
+        dfree(c->dia_val16);
+        dfree(c->dia_mask);
+        c->dia_nd = false;
+      }
+    }
+    if (!c->dia_nd && ell_pays(w, n, c->nnz_nd)) {
       const size_t slots = (size_t)w * (size_t)n;
       rc = dalloc(c, &c->ell_len_nd, (size_t)n);
       if (!rc) rc = dalloc(c, &c->ell_slot_nd, slots);

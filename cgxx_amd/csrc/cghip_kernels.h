@@ -263,9 +263,10 @@ hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, flo
 hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64_t *slot,
                       hipStream_t st);
 // The same entries in DIA (Dia): val and val16 zeroed by the caller, mask
-// written.
+// written; overflow[0] = 1 (zeroed by the caller) if a value's magnitude is
+// 65520 or more (its FP16 copy would be infinite).
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
-                      uint16_t *val16, uint8_t *mask, hipStream_t st);
+                      uint16_t *val16, uint8_t *mask, int *overflow, hipStream_t st);
 // x64 += scale o (double)y32 (scale nullable: plain add)
 hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s);
 // r64 = k - q64; partials.a = ||r64||^2 (pairs order)

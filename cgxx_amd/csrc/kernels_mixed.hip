@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
 __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const double *__restrict__ s,
                                                        Dia D, float *__restrict__ val,
                                                        uint16_t *__restrict__ val16,
-                                                       uint8_t *__restrict__ mask) {
+                                                       uint8_t *__restrict__ mask,
+                                                       int *__restrict__ overflow) {
   const int64_t T = (int64_t)gridDim.x * kBlock;
   const int64_t n = A.n;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
@@ -125,6 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
       int d = 0;
       while (d < D.ndiag - 1 && D.off[d] != off) d++;
       const float v = (float)((A.value[e] * si) * s[c]);
+      if (!(fabsf(v) < 65520.0f)) overflow[0] = 1;  // FP16 would round it to inf
       val[(int64_t)d * n + i] = v;
       val16[(int64_t)d * n + i] = __builtin_bit_cast(uint16_t, (_Float16)v);
       m |= 1u << d;
@@ -632,9 +634,9 @@ hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64
 }
 
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
-                      uint16_t *val16, uint8_t *mask, hipStream_t st) {
+                      uint16_t *val16, uint8_t *mask, int *overflow, hipStream_t st) {
   hipLaunchKernelGGL(k_scaled_dia, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, D, val, val16,
-                     mask);
+                     mask, overflow);
   return hipGetLastError();
 }
 

@@ -159,8 +159,9 @@ int cghip_solve(cghip_ctx *ctx, const cghip_solve_opts *opts,
  * vectors.  Call after cghip_upload_crs; cghip_upload_jacobi is not
  * needed (the scaling comes from the matrix's own diagonal).  The scaled
  * operator is stored in DIA when every row's columns are ascending, its
- * off-diagonal entries lie on at most dia_max (cghip_geometry) diagonals and
- * 4 ndiag N + N <= 8 nnz_offdiag + 4 N; else in packed ELL when
+ * off-diagonal entries lie on at most dia_max (cghip_geometry) diagonals,
+ * 4 ndiag N + N <= 8 nnz_offdiag + 4 N and every scaled value is below
+ * 65520 in magnitude (finite in FP16); else in packed ELL when
  * W N <= 2 (nnz_offdiag + N), W the longest off-diagonal row; else in CRS.
  * The row sums are the same in every layout (stored order). */
 int cghip_prepare_fp32(cghip_ctx *ctx);

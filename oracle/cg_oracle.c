@@ -701,6 +701,9 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
     for (int i = 0; i < N; i++)
       if (o->ptr[i + 1] - o->ptr[i] > w) w = o->ptr[i + 1] - o->ptr[i];
     int dia = dia_chosen(p, k);
+    /* ... and only if every value has a finite FP16 copy */
+    for (int e = 0; dia && e < k; e++)
+      if (!(fabsf(o->value[e]) < 65520.0f)) dia = 0;
     if (dia || w * (int64_t)N <= 2 * ((int64_t)k + N)) {
       o->nrb = (N + ORC_ROWS_MAX - 1) / ORC_ROWS_MAX;
       for (int b = 0; b <= o->nrb; b++)

@@ -146,3 +146,23 @@ def test_mixed_dia_grid_above_2048_vs_oracle(ctx, precond):
     assert_bitwise(x, ox, "x")
     r = k - m.matvec(x)
     assert np.linalg.norm(r) / np.linalg.norm(k) <= 1.05e-9
+
+
+def test_dia_refused_when_fp16_would_overflow(ctx):
+    """A tridiagonal matrix whose scaled off-diagonal entries reach 1e6
+    (a_ii = 1e-6, a_i,i+-1 = -1: positive diagonal, not SPD) cannot keep an
+    FP16 copy of its values, so the scaled operator goes to ELL; the same
+    pattern with a_ii = 4 is DIA."""
+    n = 1000
+    for diag, layout in ((1e-6, "ELL"), (4.0, "DIA")):
+        rows, cols, vals = [], [], []
+        for i in range(n):
+            for j, v in ((i - 1, -1.0), (i, diag), (i + 1, -1.0)):
+                if 0 <= j < n:
+                    rows.append(i), cols.append(j), vals.append(v)
+        ptr = np.searchsorted(np.array(rows), np.arange(n + 1)).astype(np.int32)
+        ctx.set_size(n)
+        ctx.upload_crs(ptr, np.array(cols, np.int32), np.array(vals))
+        ctx.set_format("CRS")
+        ctx.prepare_fp32()
+        assert ctx.fp32_layout() == layout
