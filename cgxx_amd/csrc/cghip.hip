@@ -1027,18 +1027,18 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
     HIPCHK(c, mp_matvec_dot(A32, p32, q32, pa, c->d_mstate, c->stream));
     if (e) HIPCHK(c, hipEventRecord(e[1], c->stream));
     if (poly) {
-      HIPCHK(c, mp_update_poly(n, gpq, y32, r32, p32, q32, c->d_poly, w[degree % 3], pa, prr,
-                               c->d_mstate, h, c->stream));
+      HIPCHK(c, mp_update_poly(n, gpq, r32, q32, c->d_poly, w[degree % 3], pa, prr, c->d_mstate,
+                               h, c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
       HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, nullptr, prz, c->d_mstate,
                               c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
-      HIPCHK(c, mp_xpay_poly(n, p32, w[0], prr, prz, gz, c->d_mstate, h, c->stream));
+      HIPCHK(c, mp_xpay_poly(n, y32, p32, w[0], prr, prz, gz, c->d_mstate, h, c->stream));
     } else {
-      HIPCHK(c, mp_update(n, gpq, y32, r32, p32, q32, pa, prr, c->d_mstate, h, c->stream));
+      HIPCHK(c, mp_update(n, gpq, r32, q32, pa, prr, c->d_mstate, h, c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
-      HIPCHK(c, mp_xpay(n, p32, r32, prr, c->d_mstate, h, c->stream));
+      HIPCHK(c, mp_xpay(n, y32, p32, r32, prr, c->d_mstate, h, c->stream));
     }
     if (e) HIPCHK(c, hipEventRecord(e[4], c->stream));
     return CGHIP_OK;
@@ -1214,8 +1214,9 @@ int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) 
   if (!c->has_fp32) return fail(cc, CGHIP_ERR_STATE, "call cghip_prepare_fp32 first");
   const double n = c->n;
   if (precond == CGHIP_PRECOND_NONE) {
-    // FP32 CRS copy of A: values, indices, row pointer; x, q; y, r, p, q; r, p
-    *bytes = 8.0 * c->crs_nnz + 4.0 * (n + 1) + 8.0 * n + 24.0 * n + 12.0 * n;
+    // FP32 CRS copy of A: values, indices, row pointer; x, q; r update (r,
+    // q, r); y and p update (y, p, r, y, p)
+    *bytes = 8.0 * c->crs_nnz + 4.0 * (n + 1) + 8.0 * n + 12.0 * n + 20.0 * n;
     return CGHIP_OK;
   }
   if (precond != CGHIP_PRECOND_JACOBI && precond != CGHIP_PRECOND_POLY)
@@ -1231,13 +1232,13 @@ int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) 
   const double op16 = c->dia_nd ? 2.0 * d * n + n : op32;
   const double spmv = op32 + 8.0 * n;
   if (precond == CGHIP_PRECOND_JACOBI) {
-    *bytes = spmv + 24.0 * n + 12.0 * n;  // + update (y, r, p, q) + p update (r, p)
+    *bytes = spmv + 12.0 * n + 20.0 * n;  // + r update (r, q; r) + y, p update (y, p, r; y, p)
     return CGHIP_OK;
   }
   const double k = c->poly_degree;
-  // update also writes B_k; each Clenshaw step reads B_{j+1}, r, writes
-  // B_j, and all but the first read B_{j+2}; p update reads z, p
-  *bytes = spmv + 28.0 * n + k * (op16 + 12.0 * n) + (k - 1.0) * 4.0 * n + 12.0 * n;
+  // r update also writes B_k; each Clenshaw step reads B_{j+1}, r, writes
+  // B_j, and all but the first read B_{j+2}; y, p update reads y, p, z
+  *bytes = spmv + 16.0 * n + k * (op16 + 12.0 * n) + (k - 1.0) * 4.0 * n + 20.0 * n;
   return CGHIP_OK;
 }
 

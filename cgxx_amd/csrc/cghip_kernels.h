@@ -161,6 +161,8 @@ struct MixedScalars {
 
 struct MixedState {
   MixedScalars cur, next;
+  float alpha32;     // (float)alpha of the running iteration: written by the
+                     // x/r update, read by the p update (which also does y)
   double nrm2_0;     // ||k - A x0|| (FP64)
   double inner0;     // ||r32|| at the first reliable update
   double true_res;   // ||k - A x|| / nrm2_0 at the last reliable update
@@ -287,18 +289,19 @@ hipError_t mp_restart_poly(int n, const double *r64, const double *scale, float 
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s);
-hipError_t mp_update_poly(int n, int gpq, float *y32, float *r32, const float *p32,
-                          const float *q32, const PolyParams *poly, float *bk, Partials pq,
-                          Partials prr, MixedState *st, History h, hipStream_t s);
-hipError_t mp_xpay_poly(int n, float *p32, const float *z32, Partials prr, Partials prz, int gz,
-                        MixedState *st, History h, hipStream_t s);
+hipError_t mp_update_poly(int n, int gpq, float *r32, const float *q32, const PolyParams *poly,
+                          float *bk, Partials pq, Partials prr, MixedState *st, History h,
+                          hipStream_t s);
+hipError_t mp_xpay_poly(int n, float *y32, float *p32, const float *z32, Partials prr, Partials prz,
+                        int gz, MixedState *st, History h, hipStream_t s);
 hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, MixedState *st,
                          hipStream_t s);
-// alpha = rho / <p,q>; y32 += alpha p32; r32 -= alpha q32; partials of <r,r>
-hipError_t mp_update(int n, int gpq, float *y32, float *r32, const float *p32, const float *q32,
-                     Partials pq, Partials pr, MixedState *st, History h, hipStream_t s);
+// alpha = rho / <p,q>; r32 -= alpha q32; partials of <r,r>
+hipError_t mp_update(int n, int gpq, float *r32, const float *q32, Partials pq, Partials pr,
+                     MixedState *st, History h, hipStream_t s);
+// y32 += alpha p32 (the old p: each p element is read once per iteration);
 // beta = <r,r>_new / <r,r>; p32 = r32 + beta p32; stop / reliable-update test
-hipError_t mp_xpay(int n, float *p32, const float *r32, Partials pr, MixedState *st, History h,
-                   hipStream_t s);
+hipError_t mp_xpay(int n, float *y32, float *p32, const float *r32, Partials pr, MixedState *st,
+                   History h, hipStream_t s);
 
 }  // namespace cghip

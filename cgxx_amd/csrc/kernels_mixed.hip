@@ -308,63 +308,93 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_mp_matvec_dot(Op32 A,
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
 
-__device__ inline void mp_update_elem(float &y, float &r, float p, float q, float a, float na,
-                                      double &s_rr) {
-  y = y + a * p;
+__device__ inline void mp_update_elem(float &r, float q, float na, double &s_rr) {
   r = r + na * q;
   s_rr += (double)r * (double)r;
 }
 
-// alpha = rho / <p,q>; y32 += alpha p32; r32 -= alpha q32; partials of <r,r>.
-__global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__restrict__ y,
-                                                      float *__restrict__ r,
-                                                      const float *__restrict__ p,
+// alpha = rho / <p,q> (into st->alpha32 for the p update, which also does
+// y32 += alpha p32); r32 -= alpha q32; partials of <r,r>.
+__global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__restrict__ r,
                                                       const float *__restrict__ q, Partials ppq,
-                                                      Partials prr, const MixedState *st,
-                                                      History h) {
+                                                      Partials prr, MixedState *st, History h) {
   __shared__ double lds[4];
   const MixedScalars c = st->cur;
   if (c.done) return;
   const double pq = fold_partials(ppq.a, gpq, lds);
   const double alpha = c.rho / pq;
+  const float na = (float)(-alpha);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha32 = (float)alpha;
     if (h.pq) h.pq[c.iteration] = pq;
     if (h.alpha) h.alpha[c.iteration] = alpha;
   }
-  const float a = (float)alpha, na = (float)(-alpha);
   Quads Q(n);
   double s_rr = 0.0;
   for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 yv[kQuadUnroll], rv[kQuadUnroll], pv[kQuadUnroll], qv[kQuadUnroll];
+    float4 rv[kQuadUnroll], qv[kQuadUnroll];
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
-      yv[u] = v4(y)[j];
       rv[u] = v4(r)[j];
-      pv[u] = v4(p)[j];
       qv[u] = v4(q)[j];
     }
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
-      mp_update_elem(yv[u].x, rv[u].x, pv[u].x, qv[u].x, a, na, s_rr);
-      mp_update_elem(yv[u].y, rv[u].y, pv[u].y, qv[u].y, a, na, s_rr);
-      mp_update_elem(yv[u].z, rv[u].z, pv[u].z, qv[u].z, a, na, s_rr);
-      mp_update_elem(yv[u].w, rv[u].w, pv[u].w, qv[u].w, a, na, s_rr);
-      v4(y)[j] = yv[u];
+      mp_update_elem(rv[u].x, qv[u].x, na, s_rr);
+      mp_update_elem(rv[u].y, qv[u].y, na, s_rr);
+      mp_update_elem(rv[u].z, qv[u].z, na, s_rr);
+      mp_update_elem(rv[u].w, qv[u].w, na, s_rr);
       v4(r)[j] = rv[u];
     }
   }
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      float ye = y[e], re = r[e];
-      mp_update_elem(ye, re, p[e], q[e], a, na, s_rr);
-      y[e] = ye;
+      float re = r[e];
+      mp_update_elem(re, q[e], na, s_rr);
       r[e] = re;
     }
   }
   const double b0 = block_fold(s_rr, lds);
   if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
+}
+
+// y32 += a p32 and p32 = u + b p32 over quads (the old p read once).
+__device__ inline void mp_yp_update(int n, float *__restrict__ y, float *__restrict__ p,
+                                    const float *__restrict__ u, float a, float b) {
+  Quads Q(n);
+  for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
+    float4 yv[kQuadUnroll], pv[kQuadUnroll], uv[kQuadUnroll];
+#pragma unroll
+    for (int k = 0; k < kQuadUnroll; k++) {
+      const int64_t j = Q.q + k * Q.T;
+      yv[k] = v4(y)[j];
+      pv[k] = v4(p)[j];
+      uv[k] = v4(u)[j];
+    }
+#pragma unroll
+    for (int k = 0; k < kQuadUnroll; k++) {
+      const int64_t j = Q.q + k * Q.T;
+      float4 yo, po;
+      yo.x = yv[k].x + a * pv[k].x;
+      yo.y = yv[k].y + a * pv[k].y;
+      yo.z = yv[k].z + a * pv[k].z;
+      yo.w = yv[k].w + a * pv[k].w;
+      po.x = uv[k].x + b * pv[k].x;
+      po.y = uv[k].y + b * pv[k].y;
+      po.z = uv[k].z + b * pv[k].z;
+      po.w = uv[k].w + b * pv[k].w;
+      v4(y)[j] = yo;
+      v4(p)[j] = po;
+    }
+  }
+  for (; Q.q < Q.nquad; Q.q += Q.T)
+    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
+      const float pe = p[e];
+      y[e] = y[e] + a * pe;
+      p[e] = u[e] + b * pe;
+    }
 }
 
 // Writes `next`: iteration count, rho = rz, r2 = rr, the inner residual and
@@ -384,9 +414,10 @@ __device__ inline void mp_advance(MixedState *st, const MixedScalars &c, double 
   if (h.beta) h.beta[it] = beta;
 }
 
-// beta = <r,r>_new / <r,r>; p32 = r32 + beta p32 (always: a reliable update
-// keeps p).  Writes `next`.
-__global__ __launch_bounds__(kBlock) void k_mp_xpay(int n, float *__restrict__ p,
+// y32 += alpha p32 (alpha from the x/r update); beta = <r,r>_new / <r,r>;
+// p32 = r32 + beta p32 (always: a reliable update keeps p).  Writes `next`.
+__global__ __launch_bounds__(kBlock) void k_mp_xpay(int n, float *__restrict__ y,
+                                                    float *__restrict__ p,
                                                     const float *__restrict__ r, Partials prr,
                                                     MixedState *st, History h) {
   __shared__ double lds[4];
@@ -394,88 +425,64 @@ __global__ __launch_bounds__(kBlock) void k_mp_xpay(int n, float *__restrict__ p
   if (c.done) return;
   const double rr = fold_partials(prr.a, gridDim.x, lds);  // the update ran on this grid
   const double beta = rr / c.rho;
+  const float a = st->alpha32;
   if (blockIdx.x == 0 && threadIdx.x == 0) mp_advance(st, c, rr, rr, beta, h);
-  const float b = (float)beta;
-  Quads Q(n);
-  for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 pv[kQuadUnroll], rv[kQuadUnroll];
-#pragma unroll
-    for (int u = 0; u < kQuadUnroll; u++) {
-      const int64_t j = Q.q + u * Q.T;
-      pv[u] = v4(p)[j];
-      rv[u] = v4(r)[j];
-    }
-#pragma unroll
-    for (int u = 0; u < kQuadUnroll; u++) {
-      float4 o;
-      o.x = rv[u].x + b * pv[u].x;
-      o.y = rv[u].y + b * pv[u].y;
-      o.z = rv[u].z + b * pv[u].z;
-      o.w = rv[u].w + b * pv[u].w;
-      v4(p)[Q.q + u * Q.T] = o;
-    }
-  }
-  for (; Q.q < Q.nquad; Q.q += Q.T)
-    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) p[e] = r[e] + b * p[e];
+  mp_yp_update(n, y, p, r, a, (float)beta);
 }
 
 // ---------------------------------------------------------------------------
 // Polynomial preconditioner (Clenshaw steps fused into FP32 SpMVs).
 
-// y32 += alpha p32; r32 -= alpha q32; B_k = c_k r32; partials of <r,r>.
-__global__ __launch_bounds__(kBlock) void k_mp_update_poly(int n, int gpq, float *__restrict__ y,
-                                                           float *__restrict__ r,
-                                                           const float *__restrict__ p,
+// alpha = rho / <p,q> (into st->alpha32); r32 -= alpha q32; B_k = c_k r32;
+// partials of <r,r>.  y32 += alpha p32 is done by the p update.
+__global__ __launch_bounds__(kBlock) void k_mp_update_poly(int n, int gpq, float *__restrict__ r,
                                                            const float *__restrict__ q,
                                                            const PolyParams *poly,
                                                            float *__restrict__ bk, Partials ppq,
-                                                           Partials prr, const MixedState *st,
+                                                           Partials prr, MixedState *st,
                                                            History h) {
   __shared__ double lds[4];
   const MixedScalars c = st->cur;
   if (c.done) return;
   const double pq = fold_partials(ppq.a, gpq, lds);
   const double alpha = c.rho / pq;
+  const float na = (float)(-alpha);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha32 = (float)alpha;
     if (h.pq) h.pq[c.iteration] = pq;
     if (h.alpha) h.alpha[c.iteration] = alpha;
   }
-  const float a = (float)alpha, na = (float)(-alpha);
   const float ck = poly->c[poly->degree];
   Quads Q(n);
   double s_rr = 0.0;
   for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 yv[kQuadUnroll], rv[kQuadUnroll], pv[kQuadUnroll], qv[kQuadUnroll];
+    float4 rv[kQuadUnroll], qv[kQuadUnroll];
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
-      yv[u] = v4(y)[j];
       rv[u] = v4(r)[j];
-      pv[u] = v4(p)[j];
       qv[u] = v4(q)[j];
     }
 #pragma unroll
     for (int u = 0; u < kQuadUnroll; u++) {
       const int64_t j = Q.q + u * Q.T;
-      mp_update_elem(yv[u].x, rv[u].x, pv[u].x, qv[u].x, a, na, s_rr);
-      mp_update_elem(yv[u].y, rv[u].y, pv[u].y, qv[u].y, a, na, s_rr);
-      mp_update_elem(yv[u].z, rv[u].z, pv[u].z, qv[u].z, a, na, s_rr);
-      mp_update_elem(yv[u].w, rv[u].w, pv[u].w, qv[u].w, a, na, s_rr);
+      mp_update_elem(rv[u].x, qv[u].x, na, s_rr);
+      mp_update_elem(rv[u].y, qv[u].y, na, s_rr);
+      mp_update_elem(rv[u].z, qv[u].z, na, s_rr);
+      mp_update_elem(rv[u].w, qv[u].w, na, s_rr);
       float4 bv;
       bv.x = ck * rv[u].x;
       bv.y = ck * rv[u].y;
       bv.z = ck * rv[u].z;
       bv.w = ck * rv[u].w;
-      v4(y)[j] = yv[u];
       v4(r)[j] = rv[u];
       v4(bk)[j] = bv;
     }
   }
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      float ye = y[e], re = r[e];
-      mp_update_elem(ye, re, p[e], q[e], a, na, s_rr);
-      y[e] = ye;
+      float re = r[e];
+      mp_update_elem(re, q[e], na, s_rr);
       r[e] = re;
       bk[e] = ck * re;
     }
@@ -554,9 +561,11 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   }
 }
 
-// beta = <r,z>_new / <r,z>; p32 = z + beta p32; stop test / reliable-update
-// trigger on the inner residual.  Writes `next`.
-__global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restrict__ p,
+// y32 += alpha p32 (alpha from the x/r update); beta = <r,z>_new / <r,z>;
+// p32 = z + beta p32; stop test / reliable-update trigger on the inner
+// residual.  Writes `next`.
+__global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restrict__ y,
+                                                         float *__restrict__ p,
                                                          const float *__restrict__ z, Partials prr,
                                                          Partials prz, int gz, MixedState *st,
                                                          History h) {
@@ -566,28 +575,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restric
   const double rr = fold_partials(prr.a, gridDim.x, lds);
   const double rz = fold_partials(prz.a, gz, lds);
   const double beta = rz / c.rho;
+  const float a = st->alpha32;
   if (blockIdx.x == 0 && threadIdx.x == 0) mp_advance(st, c, rr, rz, beta, h);
-  const float b = (float)beta;
-  Quads Q(n);
-  for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 pv[kQuadUnroll], zv[kQuadUnroll];
-#pragma unroll
-    for (int u = 0; u < kQuadUnroll; u++) {
-      pv[u] = v4(p)[Q.q + u * Q.T];
-      zv[u] = v4(z)[Q.q + u * Q.T];
-    }
-#pragma unroll
-    for (int u = 0; u < kQuadUnroll; u++) {
-      float4 o;
-      o.x = zv[u].x + b * pv[u].x;
-      o.y = zv[u].y + b * pv[u].y;
-      o.z = zv[u].z + b * pv[u].z;
-      o.w = zv[u].w + b * pv[u].w;
-      v4(p)[Q.q + u * Q.T] = o;
-    }
-  }
-  for (; Q.q < Q.nquad; Q.q += Q.T)
-    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) p[e] = z[e] + b * p[e];
+  mp_yp_update(n, y, p, z, a, (float)beta);
 }
 
 int row_grid(int n) {
@@ -706,18 +696,18 @@ hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly
   return hipGetLastError();
 }
 
-hipError_t mp_update_poly(int n, int gpq, float *y32, float *r32, const float *p32,
-                          const float *q32, const PolyParams *poly, float *bk, Partials pq,
-                          Partials prr, MixedState *st, History h, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_update_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, y32, r32,
-                     p32, q32, poly, bk, pq, prr, st, h);
+hipError_t mp_update_poly(int n, int gpq, float *r32, const float *q32, const PolyParams *poly,
+                          float *bk, Partials pq, Partials prr, MixedState *st, History h,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_update_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, r32, q32,
+                     poly, bk, pq, prr, st, h);
   return hipGetLastError();
 }
 
-hipError_t mp_xpay_poly(int n, float *p32, const float *z32, Partials prr, Partials prz, int gz,
-                        MixedState *st, History h, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_xpay_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, p32, z32, prr, prz,
-                     gz, st, h);
+hipError_t mp_xpay_poly(int n, float *y32, float *p32, const float *z32, Partials prr, Partials prz,
+                        int gz, MixedState *st, History h, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_xpay_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, y32, p32, z32, prr,
+                     prz, gz, st, h);
   return hipGetLastError();
 }
 
@@ -739,16 +729,17 @@ hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, M
   return hipGetLastError();
 }
 
-hipError_t mp_update(int n, int gpq, float *y32, float *r32, const float *p32, const float *q32,
-                     Partials pq, Partials pr, MixedState *st, History h, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_update, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, y32, r32, p32,
-                     q32, pq, pr, st, h);
+hipError_t mp_update(int n, int gpq, float *r32, const float *q32, Partials pq, Partials pr,
+                     MixedState *st, History h, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_update, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, r32, q32, pq, pr,
+                     st, h);
   return hipGetLastError();
 }
 
-hipError_t mp_xpay(int n, float *p32, const float *r32, Partials pr, MixedState *st, History h,
-                   hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_xpay, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, p32, r32, pr, st, h);
+hipError_t mp_xpay(int n, float *y32, float *p32, const float *r32, Partials pr, MixedState *st,
+                   History h, hipStream_t s) {
+  hipLaunchKernelGGL(k_mp_xpay, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, y32, p32, r32, pr, st,
+                     h);
   return hipGetLastError();
 }
 
