@@ -601,10 +601,10 @@ int enqueue_iteration(cghip_ctx *c, const double *C, History h, hipEvent_t *ev) 
   else
     HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, pq_of(c), c->d_state, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  HIPCHK(c, pcg_update(c->n, spmv_blocks(c), x, r, p, q, C, pq_of(c), pr_of(c), c->d_state, h,
+  HIPCHK(c, pcg_update(c->n, spmv_blocks(c), r, q, C, pq_of(c), pr_of(c), c->d_state, h,
                        c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-  HIPCHK(c, pcg_xpay(c->n, p, r, C, pr_of(c), c->d_state, h, c->stream));
+  HIPCHK(c, pcg_xpay(c->n, x, p, r, C, pr_of(c), c->d_state, h, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
   return CGHIP_OK;
 }

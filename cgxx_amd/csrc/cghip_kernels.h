@@ -72,6 +72,8 @@ struct SolveState {
   Scalars cur, next;
   double nrm2_0, tol;  // written once, before the loop
   int max_iter, pad;
+  double alpha;  // alpha of the running iteration: written by the r update,
+                 // read by the p update (which also does x += alpha p)
 };
 
 // Per-iteration history (nullable arrays of max_iter + 1 entries).
@@ -131,10 +133,12 @@ hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, 
                               SolveState *st, hipStream_t s);
 hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
                               SolveState *st, hipStream_t s);
-hipError_t pcg_update(int n, int gpq, double *x, double *r, const double *p, const double *q,
-                      const double *C, Partials pq, Partials pr, SolveState *st, History h,
-                      hipStream_t s);
-hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, Partials pr,
+// alpha = rho / <p,q>; r -= alpha q; partials of <r,r> and <r,Cr>
+hipError_t pcg_update(int n, int gpq, double *r, const double *q, const double *C, Partials pq,
+                      Partials pr, SolveState *st, History h, hipStream_t s);
+// x += alpha p (the old p: each p element is read once outside the SpMV);
+// stop test; p = C r + beta p unless converged
+hipError_t pcg_xpay(int n, double *x, double *p, const double *r, const double *C, Partials pr,
                     SolveState *st, History h, hipStream_t s);
 
 // ---- mixed precision: FP32 inner PCG with FP64 reliable updates ----------

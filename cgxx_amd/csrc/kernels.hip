@@ -297,32 +297,29 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_finish(Partials pa, int G, 
   }
 }
 
-// a = rho / <p,q> (CG.cpp:334); x += a p; r += (-a) q (CG.cpp:340-342);
-// partials of <r,r> and <r,z> with z = C o r (CG.cpp:344, 363-366).
 template <bool JAC>
-__device__ inline void update_elem(double &xv, double &rv, double pv, double qv, double cv,
-                                   double a, double na, double &s_rr, double &s_rz) {
-  xv = xv + a * pv;
+__device__ inline void update_elem(double &rv, double qv, double cv, double na, double &s_rr,
+                                   double &s_rz) {
   rv = rv + na * qv;
   const double zv = JAC ? cv * rv : rv;
   s_rr += rv * rv;
   s_rz += rv * zv;
 }
-
+// a = rho / <p,q> (CG.cpp:334), kept in st->alpha; r += (-a) q (CG.cpp:342;
+// the x += a p of CG.cpp:340 is done by the last kernel, which reads p
+// anyway); partials of <r,r> and <r,z> with z = C o r (CG.cpp:344, 363-366).
 template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *__restrict__ x,
-                                                       double *__restrict__ r,
-                                                       const double *__restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *__restrict__ r,
                                                        const double *__restrict__ q,
                                                        const double *__restrict__ C, Partials ppq,
-                                                       Partials prr, const SolveState *st,
-                                                       History h) {
+                                                       Partials prr, SolveState *st, History h) {
   __shared__ double lds[4];
   const Scalars c = st->cur;
   if (c.done) return;
   const double pq = fold_partials(ppq.a, gpq, lds);
   const double a = c.rho / pq;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = a;
     if (h.pq) h.pq[c.iteration] = pq;
     if (h.alpha) h.alpha[c.iteration] = a;
   }
@@ -330,30 +327,26 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *_
   Pairs P(n);
   double s_rr = 0.0, s_rz = 0.0;
   for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
-    double2 xv[kUnroll], rv[kUnroll], pv[kUnroll], qv[kUnroll], cv[kUnroll];
+    double2 rv[kUnroll], qv[kUnroll], cv[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int64_t j = P.q + u * P.T;
-      xv[u] = v2(x)[j];
       rv[u] = v2(r)[j];
-      pv[u] = v2(p)[j];
       qv[u] = v2(q)[j];
       if (JAC) cv[u] = v2(C)[j];
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int64_t j = P.q + u * P.T;
-      update_elem<JAC>(xv[u].x, rv[u].x, pv[u].x, qv[u].x, JAC ? cv[u].x : 0.0, a, na, s_rr, s_rz);
-      update_elem<JAC>(xv[u].y, rv[u].y, pv[u].y, qv[u].y, JAC ? cv[u].y : 0.0, a, na, s_rr, s_rz);
-      v2(x)[j] = xv[u];
+      update_elem<JAC>(rv[u].x, qv[u].x, JAC ? cv[u].x : 0.0, na, s_rr, s_rz);
+      update_elem<JAC>(rv[u].y, qv[u].y, JAC ? cv[u].y : 0.0, na, s_rr, s_rz);
       v2(r)[j] = rv[u];
     }
   }
   for (; P.q < P.npair; P.q += P.T) {
     for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
-      double xe = x[e], re = r[e];
-      update_elem<JAC>(xe, re, p[e], q[e], JAC ? C[e] : 0.0, a, na, s_rr, s_rz);
-      x[e] = xe;
+      double re = r[e];
+      update_elem<JAC>(re, q[e], JAC ? C[e] : 0.0, na, s_rr, s_rz);
       r[e] = re;
     }
   }
@@ -364,12 +357,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *_
     prr.b[blockIdx.x] = b1;
   }
 }
-
-// Last kernel of an iteration: r2 and the stop test (CG.cpp:344-355); if not
-// converged rho = <r,z>, b = rho / rho_old (CG.cpp:357-370) and
-// p = z + b p with z = C o r recomputed (CG.cpp:375-381).  Writes `next`.
+// Last kernel of an iteration: x += a p with the old p (CG.cpp:340); r2 and
+// the stop test (CG.cpp:344-355); if not converged rho = <r,z>,
+// b = rho / rho_old (CG.cpp:357-370) and p = z + b p with z = C o r
+// recomputed (CG.cpp:375-381).  Writes `next`.
 template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__ x,
+                                                     double *__restrict__ p,
                                                      const double *__restrict__ r,
                                                      const double *__restrict__ C, Partials prr,
                                                      SolveState *st, History h) {
@@ -383,6 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
   const bool converged = res <= st->tol;
   const double rho = JAC ? rz : r2;
   const double b = rho / c.rho;
+  const double a = st->alpha;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const int it = c.iteration;
     Scalars s = c;
@@ -400,30 +395,44 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
     }
     st->next = s;
   }
-  if (converged) return;
+  // x += a p always; p = z + b p unless the loop stops here.  Four streams
+  // per element: unrolled twice to stay within 64 VGPRs.
+  constexpr int U = 2;
   Pairs P(n);
-  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
-    double2 pv[kUnroll], rv[kUnroll], cv[kUnroll];
+  if (converged) {
+    for (; P.q < P.npair; P.q += P.T)
+      for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) x[e] = x[e] + a * p[e];
+    return;
+  }
+  for (; P.q + (U - 1) * P.T < P.full; P.q += U * P.T) {
+    double2 xv[U], pv[U], rv[U], cv[U];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < U; u++) {
       const int64_t j = P.q + u * P.T;
+      xv[u] = v2(x)[j];
       pv[u] = v2(p)[j];
       rv[u] = v2(r)[j];
       if (JAC) cv[u] = v2(C)[j];
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
+    for (int u = 0; u < U; u++) {
+      const int64_t j = P.q + u * P.T;
+      xv[u].x = xv[u].x + a * pv[u].x;
+      xv[u].y = xv[u].y + a * pv[u].y;
       const double zx = JAC ? cv[u].x * rv[u].x : rv[u].x;
       const double zy = JAC ? cv[u].y * rv[u].y : rv[u].y;
       pv[u].x = zx + b * pv[u].x;
       pv[u].y = zy + b * pv[u].y;
-      v2(p)[P.q + u * P.T] = pv[u];
+      v2(x)[j] = xv[u];
+      v2(p)[j] = pv[u];
     }
   }
   for (; P.q < P.npair; P.q += P.T) {
     for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
+      const double pe = p[e];
+      x[e] = x[e] + a * pe;
       const double ze = JAC ? C[e] * r[e] : r[e];
-      p[e] = ze + b * p[e];
+      p[e] = ze + b * pe;
     }
   }
 }
@@ -545,24 +554,23 @@ hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, 
   return hipGetLastError();
 }
 
-hipError_t pcg_update(int n, int gpq, double *x, double *r, const double *p, const double *q,
-                      const double *C, Partials pq, Partials pr, SolveState *st, History h,
-                      hipStream_t s) {
+hipError_t pcg_update(int n, int gpq, double *r, const double *q, const double *C, Partials pq,
+                      Partials pr, SolveState *st, History h, hipStream_t s) {
   if (C)
-    hipLaunchKernelGGL(k_pcg_update<true>, vgrid(n), dim3(kBlock), 0, s, n, gpq, x, r, p, q, C,
-                       pq, pr, st, h);
+    hipLaunchKernelGGL(k_pcg_update<true>, vgrid(n), dim3(kBlock), 0, s, n, gpq, r, q, C, pq, pr,
+                       st, h);
   else
-    hipLaunchKernelGGL(k_pcg_update<false>, vgrid(n), dim3(kBlock), 0, s, n, gpq, x, r, p, q, C,
-                       pq, pr, st, h);
+    hipLaunchKernelGGL(k_pcg_update<false>, vgrid(n), dim3(kBlock), 0, s, n, gpq, r, q, C, pq, pr,
+                       st, h);
   return hipGetLastError();
 }
 
-hipError_t pcg_xpay(int n, double *p, const double *r, const double *C, Partials pr,
+hipError_t pcg_xpay(int n, double *x, double *p, const double *r, const double *C, Partials pr,
                     SolveState *st, History h, hipStream_t s) {
   if (C)
-    hipLaunchKernelGGL(k_pcg_xpay<true>, vgrid(n), dim3(kBlock), 0, s, n, p, r, C, pr, st, h);
+    hipLaunchKernelGGL(k_pcg_xpay<true>, vgrid(n), dim3(kBlock), 0, s, n, x, p, r, C, pr, st, h);
   else
-    hipLaunchKernelGGL(k_pcg_xpay<false>, vgrid(n), dim3(kBlock), 0, s, n, p, r, C, pr, st, h);
+    hipLaunchKernelGGL(k_pcg_xpay<false>, vgrid(n), dim3(kBlock), 0, s, n, x, p, r, C, pr, st, h);
   return hipGetLastError();
 }
 
