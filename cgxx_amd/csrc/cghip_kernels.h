@@ -166,7 +166,7 @@ struct MixedScalars {
 struct MixedState {
   MixedScalars cur, next;
   float alpha32;     // (float)alpha of the running iteration: written by the
-                     // x/r update, read by the p update (which also does y)
+                     // r update, read by the p update (which also does y)
   double nrm2_0;     // ||k - A x0|| (FP64)
   double inner0;     // ||r32|| at the first reliable update
   double true_res;   // ||k - A x|| / nrm2_0 at the last reliable update

@@ -414,7 +414,7 @@ __device__ inline void mp_advance(MixedState *st, const MixedScalars &c, double 
   if (h.beta) h.beta[it] = beta;
 }
 
-// y32 += alpha p32 (alpha from the x/r update); beta = <r,r>_new / <r,r>;
+// y32 += alpha p32 (alpha from the r update); beta = <r,r>_new / <r,r>;
 // p32 = r32 + beta p32 (always: a reliable update keeps p).  Writes `next`.
 __global__ __launch_bounds__(kBlock) void k_mp_xpay(int n, float *__restrict__ y,
                                                     float *__restrict__ p,
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   }
 }
 
-// y32 += alpha p32 (alpha from the x/r update); beta = <r,z>_new / <r,z>;
+// y32 += alpha p32 (alpha from the r update); beta = <r,z>_new / <r,z>;
 // p32 = z + beta p32; stop test / reliable-update trigger on the inner
 // residual.  Writes `next`.
 __global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restrict__ y,

@@ -11,9 +11,9 @@
 // three kernels per iteration, the scalars (alpha, beta, the stop test)
 // computed on the device, kernel times measured with HIP events and
 // charged to the reference's categories
-// (SpMV + <p,q> -> MatVec; x/r update + <r,r> + <r,z> -> axpy; Jacobi and
-// p update -> xpay; vectorDot and Preconditioner are fused into those and
-// report 0).  Environment (in addition to the reference's CG_* variables):
+// (SpMV + <p,q> -> MatVec; r update + <r,r> + <r,z> -> axpy; x update,
+// Jacobi and p update -> xpay; vectorDot and Preconditioner are fused into
+// those and report 0).  Environment (in addition to the reference's CG_* variables):
 //   CG_HIP_DEVICE   device index (default 0)
 //   CG_HIP_FUSED    0: run the reference loop hook by hook instead
 //   CG_HIP_DOT      tree (default) | serial: summation order of the dot hook;

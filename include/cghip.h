@@ -105,7 +105,8 @@ int cghip_apply_preconditioner(cghip_ctx *ctx, int x, int y);  /* CG.h:235 y = C
 
 /* --- fused solver ----------------------------------------------------------
  * The whole loop of CG::solve (CG.cpp:284-385) on the device: three kernels
- * per iteration (SpMV + <p,q>; x/r update + <r,r> + <r,z>; p = C r + b p).
+ * per iteration (SpMV + <p,q>; r update + <r,r> + <r,z>; x += a p and
+ * p = C r + b p).
  * Each kernel writes per-block partial sums; the next kernel folds them (in
  * a fixed order) and computes alpha, beta and the stop test itself, so no
  * host round trip is needed per iteration.  Iterations are replayed from a
@@ -130,7 +131,8 @@ typedef struct {
   double r2;          /* last <r,r> */
   double wall_s;      /* host wall time of the solve, device synchronised */
   /* profile != NONE: summed kernel time (ms) and counts of the kernels that
-   * did work (the SpMV + <p,q>, the x/r update, the p update, the set-up) */
+   * did work (the SpMV + <p,q>, the r update, the x and p update, the
+   * set-up) */
   double ms_matvec, ms_update, ms_xpay, ms_init;
   int n_matvec, n_update, n_xpay, n_init;
 } cghip_solve_info;
@@ -200,7 +202,7 @@ typedef struct {
   double inner_residual;   /* final FP32 recursive relative residual */
   double wall_s;
   /* profile != NONE: summed kernel time (ms) and launch counts of the SpMV +
-   * <p,q>, the x/r update, the p update, the reliable updates and (the
+   * <p,q>, the r update, the y and p update, the reliable updates and (the
    * polynomial) the Clenshaw SpMV steps */
   double ms_matvec, ms_update, ms_xpay, ms_refresh, ms_poly;
   int n_matvec, n_update, n_xpay, n_refresh, n_poly;
