@@ -110,7 +110,7 @@ def build_host() -> str | None:
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
     tmp = BIN + ".tmp"
     _run(["g++", "-std=c++17", "-O3", "-DNDEBUG", "-Wall", "-I" + os.path.join(REPO, "include"),
-          "-o", tmp, *srcs, "-L" + os.path.dirname(LIB), "-lcghip",
+          "-pthread", "-o", tmp, *srcs, "-L" + os.path.dirname(LIB), "-lcghip",
           "-Wl,-rpath,$ORIGIN/../lib"])
     os.replace(tmp, BIN)
     return BIN

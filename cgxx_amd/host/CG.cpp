@@ -26,6 +26,8 @@ const char *kMatrixFormat = "CG_MATRIX_FORMAT";
 const char *kPreconditioner = "CG_PRECONDITIONER";
 const char *kWorkDistribution = "CG_WORK_DISTRIBUTION";
 const char *kOverlappedGather = "CG_OVERLAPPED_GATHER";
+// Additions of this backend, no reference counterpart.
+const char *kMatrixCache = "CG_MATRIX_CACHE";
 
 const char *env(const char *name) {
   const char *v = std::getenv(name);
@@ -106,6 +108,8 @@ void CG::parseEnvironment() {
     if (overlappedGather && (getNumberOfChunks() == -1 || !supportsOverlappedGather()))
       fail("No support for overlapped gather!");
   }
+  if (const char *v = env(kMatrixCache)) matrixCache = v;
+  hostThreads();  // validates CG_IO_THREADS before any work
 }
 
 void CG::allocateMatrixCRS() { matrixCRS.reset(new MatrixCRS); }
@@ -125,7 +129,7 @@ void CG::deallocateX() {
 void CG::init(const char *matrixFile) {
   std::cout << "Reading matrix from " << matrixFile << "..." << std::endl;
   const time_point startIO = now();
-  matrixCOO.reset(new MatrixCOO(matrixFile));
+  matrixCOO.reset(new MatrixCOO(matrixFile, matrixCache.empty() ? nullptr : matrixCache.c_str()));
   N = matrixCOO->N;
   nz = matrixCOO->nz;
 
@@ -155,7 +159,8 @@ void CG::init(const char *matrixFile) {
   // k = A * (1, ..., 1)^T, row sums accumulated in COO order (CG.cpp:256-261)
   allocateK();
   std::memset(k, 0, sizeof(floatType) * (size_t)N);
-  for (int e = 0; e < nz; e++) k[matrixCOO->I[e]] += matrixCOO->V[e];
+  const MatrixCOO &coo = *matrixCOO;
+  coo.forEachEntryByRow(hostThreads(), [&](int e) { k[coo.I[e]] += coo.V[e]; });
 
   allocateX();
   resetX();

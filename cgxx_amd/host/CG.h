@@ -63,6 +63,7 @@ protected:
   bool overlappedGather = false;
 
   MatrixFormat matrixFormat;
+  std::string matrixCache;  // CG_MATRIX_CACHE: binary copy of the read matrix
   std::unique_ptr<MatrixCOO> matrixCOO;
   std::unique_ptr<MatrixCRS> matrixCRS;
   std::unique_ptr<MatrixELL> matrixELL;

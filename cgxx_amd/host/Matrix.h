@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <memory>
 
+#include "Parallel.h"
 #include "def.h"
 
 struct Matrix {
@@ -26,12 +27,48 @@ struct MatrixCOO : Matrix {
   std::unique_ptr<int[]> J;
   std::unique_ptr<floatType[]> V;
   std::unique_ptr<int[]> nzPerRow;
+  bool rowsSorted = false;  // I non-decreasing (every general file written by rows)
 
   MatrixCOO() = delete;
-  explicit MatrixCOO(const char *file);
+  // Reads `file` on hostThreads() threads.  With `cache` (a path), the
+  // expanded arrays are loaded from that binary file when it was written for
+  // the same source (size and modification time), else the source is parsed
+  // and the cache (re)written.
+  explicit MatrixCOO(const char *file, const char *cache = nullptr);
 
   int getMaxNz() const { return getMaxNz(0, N); }
   int getMaxNz(int from, int to) const;
+
+  // Calls f(e) for every entry e, in COO order within each row, on T
+  // threads that each own a range of rows, so per-row state needs no
+  // locking and per-row results match a sequential pass in COO order.  With
+  // sorted rows each thread visits only its own entries; otherwise each
+  // scans I in full and skips the rows it does not own.
+  template <class F>
+  void forEachEntryByRow(int T, F &&f) const {
+    if (rowsSorted) {
+      parallelFor(T, [&](int t) {
+        long long e0 = splitPoint(nz, T, t), e1 = splitPoint(nz, T, t + 1);
+        while (e0 > 0 && e0 < nz && I[e0] == I[e0 - 1]) e0++;
+        while (e1 > 0 && e1 < nz && I[e1] == I[e1 - 1]) e1++;
+        for (long long e = e0; e < e1; e++) f((int)e);
+      });
+    } else {
+      parallelFor(T, [&](int t) {
+        const int r0 = (int)splitPoint(N, T, t), r1 = (int)splitPoint(N, T, t + 1);
+        for (int e = 0; e < nz; e++) {
+          const int r = I[e];
+          if (r >= r0 && r < r1) f(e);
+        }
+      });
+    }
+  }
+
+ private:
+  void parse(const char *file);
+  bool loadCache(const char *cache, const char *file);
+  void saveCache(const char *cache, const char *file) const;
+  bool countRows();
 };
 
 // CRS storage (Matrix.h:48-71).

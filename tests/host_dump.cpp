@@ -1,7 +1,7 @@
 // Test helper: reads a Matrix Market file with the product's C++ reader and
 // converters (cgxx_amd/host/Matrix.cpp, Preconditioner.cpp) and writes the
 // CRS, ELL and Jacobi arrays as raw binary for comparison with the oracle.
-// Usage: host_dump <matrix.mtx> <out.bin>
+// Usage: host_dump <matrix.mtx> <out.bin> [<cache>]  (threads: CG_IO_THREADS)
 #include <cstdio>
 #include <cstdlib>
 
@@ -9,8 +9,8 @@
 #include "Preconditioner.h"
 
 int main(int argc, char **argv) {
-  if (argc != 3) return 1;
-  MatrixCOO coo(argv[1]);
+  if (argc != 3 && argc != 4) return 1;
+  MatrixCOO coo(argv[1], argc == 4 ? argv[3] : nullptr);
   MatrixCRS crs;
   crs.convert(coo);
   MatrixELL ell;
