@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
@@ -127,6 +128,7 @@ void CG::deallocateX() {
 }
 
 void CG::init(const char *matrixFile) {
+  matrixFileName = matrixFile;
   std::cout << "Reading matrix from " << matrixFile << "..." << std::endl;
   const time_point startIO = now();
   matrixCOO.reset(new MatrixCOO(matrixFile, matrixCache.empty() ? nullptr : matrixCache.c_str()));
@@ -284,6 +286,7 @@ bool CG::check() {
   for (int i = 0; i < N; i++)
     if (std::abs(x[i] - 1.0) > checkTolerance) errors++;
   timing.check = now() - start;
+  checkErrors = errors;
   if (errors == 0)
     std::cout << "Solution is correct!" << std::endl;
   else
@@ -340,6 +343,88 @@ void CG::printSummary() {
   printPadded("vectorDot time:", std::to_string(timing.vectorDot.count()));
   if (preconditioner != PreconditionerNone)
     printPadded("Preconditioner time:", std::to_string(timing.preconditioner.count()));
+}
+
+namespace {
+
+void jsonString(std::ostream &os, const std::string &v) {
+  os << '"';
+  for (const unsigned char c : v) {
+    if (c == '"' || c == '\\') {
+      os << '\\' << c;
+    } else if (c < 0x20) {
+      char buf[8];
+      std::snprintf(buf, sizeof buf, "\\u%04x", c);
+      os << buf;
+    } else {
+      os << c;
+    }
+  }
+  os << '"';
+}
+
+// Shortest text that reads back to the same double; null when not finite.
+void jsonNumber(std::ostream &os, double v) {
+  if (!std::isfinite(v)) {
+    os << "null";
+    return;
+  }
+  char buf[32];
+  for (int prec = 15; prec <= 17; prec++) {
+    std::snprintf(buf, sizeof buf, "%.*g", prec, v);
+    if (std::strtod(buf, nullptr) == v) break;
+  }
+  os << buf;
+}
+
+}  // namespace
+
+void CG::recordBackend(std::ostream &os) { os << "\"name\": \"base\""; }
+
+bool CG::writeRecord(const char *path, int repetition) {
+  std::ostringstream os;
+  const char *formats[] = {"COO", "CRS", "ELL"};
+  os << "{\"matrix\": ";
+  jsonString(os, matrixFileName);
+  os << ", \"rows\": " << N << ", \"nonzeros\": " << nz << ", \"format\": \"" << formats[matrixFormat]
+     << "\", \"preconditioner\": \"" << (preconditioner == PreconditionerNone ? "none" : "jacobi")
+     << "\", \"tolerance\": ";
+  jsonNumber(os, tolerance);
+  os << ", \"max_iterations\": " << maxIterations << ", \"repetition\": " << repetition
+     << ", \"iterations\": " << iteration << ", \"residual\": ";
+  jsonNumber(os, residual);
+  os << ", \"check_tolerance\": ";
+  jsonNumber(os, checkTolerance);
+  os << ", \"incorrect_elements\": " << checkErrors << ", \"times\": {";
+  const std::pair<const char *, double> times[] = {
+      {"io", timing.io.count()},
+      {"converting", timing.converting.count()},
+      {"transfer_to", timing.transferTo.count()},
+      {"solve", timing.solve.count()},
+      {"transfer_from", timing.transferFrom.count()},
+      {"check", timing.check.count()},
+      {"matvec", timing.matvec.count()},
+      {"axpy", timing.axpy.count()},
+      {"xpay", timing.xpay.count()},
+      {"vector_dot", timing.vectorDot.count()},
+      {"preconditioner", timing.preconditioner.count()},
+  };
+  bool first = true;
+  for (const auto &t : times) {
+    os << (first ? "" : ", ") << '"' << t.first << "\": ";
+    jsonNumber(os, t.second);
+    first = false;
+  }
+  os << "}, \"matvec_gflops\": ";
+  jsonNumber(os, 2.0 * (iteration + 1) * nz / 1e9 / timing.matvec.count());
+  os << ", \"backend\": {";
+  recordBackend(os);
+  os << "}}\n";
+  FILE *f = std::fopen(path, "a");
+  if (f == nullptr) return false;
+  const std::string line = os.str();
+  const bool ok = std::fwrite(line.data(), 1, line.size(), f) == line.size();
+  return std::fclose(f) == 0 && ok;
 }
 
 void CG::cleanup() {

@@ -11,12 +11,17 @@
 // the whole loop on the device overrides solveFused() and returns true, and
 // solve() then takes its iteration count, residual and per-category kernel
 // times.  The default returns false and solve() runs the reference loop.
+//
+// A second addition: writeRecord(), one JSON object per solve with the
+// report's figures (CG_REPORT_JSON), extended by the backend through
+// recordBackend().
 #ifndef CGHIP_HOST_CG_H
 #define CGHIP_HOST_CG_H
 
 #include <cassert>
 #include <chrono>
 #include <memory>
+#include <ostream>
 #include <string>
 
 #include "Matrix.h"
@@ -44,6 +49,8 @@ private:
     duration solve{0}, matvec{0}, axpy{0}, xpay{0}, vectorDot{0}, preconditioner{0};
   };
   Timing timing;
+  std::string matrixFileName;
+  int checkErrors = -1;  // elements the last check() found incorrect
 
   using time_point = Timing::clock::time_point;
   time_point now() const { return Timing::clock::now(); }
@@ -121,6 +128,10 @@ protected:
 
   static void printPadded(const char *label, const std::string &value);
 
+  // Members of the backend's object in the solve record ("backend": {...}),
+  // written without the braces; the default writes "name" only.
+  virtual void recordBackend(std::ostream &os);
+
 public:
   virtual ~CG() {}
 
@@ -137,6 +148,10 @@ public:
 
   virtual void printSummary();
   bool check();
+  // Appends one line to `path`: a JSON object with the last solve's
+  // figures (those of printSummary, unrounded) and `repetition`.  Returns
+  // false if the file cannot be written.
+  bool writeRecord(const char *path, int repetition);
   virtual void cleanup();
 
   int getIterations() const { return iteration; }

@@ -33,8 +33,10 @@
 //   CG_HIP_MODEL    model file for CG_HIP_POLY (default: ../data/poly_mlp.bin
 //                   next to this executable).
 #include <cerrno>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <iomanip>
 #include <iostream>
 #include <string>
 
@@ -80,7 +82,13 @@ class CGHIP : public CG {
   std::string modelPath;
   std::unique_ptr<LearnedPoly> learned;
 
+  // What the last solve ran, for the solve record.
+  std::string lastLoop = "hooks";
+  cghip_mixed_info lastMixed{};
+  LearnedResult lastLearned;
+
   bool solveMixed(int maxIterations, floatType tolerance, FusedResult &result);
+  void recordBackend(std::ostream &os) override;
 
   void chk(int rc, const char *what) {
     if (rc != CGHIP_OK) hip_fail(ctx, what);
@@ -216,6 +224,8 @@ bool CGHIP::solveMixed(int maxIterations, floatType tolerance, FusedResult &resu
     std::cout << "HIP mixed precision, polynomial (" << polyMode << "): " << r.method << ", "
               << r.polyIterations << " polynomial + " << r.jacobiIterations
               << " Jacobi inner iterations" << std::endl;
+    lastLoop = "mixed";
+    lastLearned = r;
     result.iterations = r.polyIterations + r.jacobiIterations;
     result.residual = r.trueResidual;
     return true;
@@ -230,6 +240,8 @@ bool CGHIP::solveMixed(int maxIterations, floatType tolerance, FusedResult &resu
   cghip_mixed_info info{};
   chk(cghip_solve_mixed(ctx, &o, &info), "solve_mixed");
   std::cout << "HIP mixed precision: " << info.updates << " reliable updates" << std::endl;
+  lastLoop = "mixed";
+  lastMixed = info;
   result.iterations = info.iterations;
   result.residual = info.true_residual;
   result.matvec = (info.ms_matvec + info.ms_refresh) * 1e-3;
@@ -241,8 +253,10 @@ bool CGHIP::solveMixed(int maxIterations, floatType tolerance, FusedResult &resu
 void CGHIP::doTransferFrom() { chk(cghip_download_vector(ctx, CGHIP_VEC_X, x), "download x"); }
 
 bool CGHIP::solveFused(int maxIterations, floatType tolerance, FusedResult &result) {
+  lastLoop = "hooks";
   if (mixed) return solveMixed(maxIterations, tolerance, result);
   if (!fused) return false;
+  lastLoop = graph ? "fused-graph" : "fused";
   cghip_solve_opts o{};
   o.precond = preconditioner == PreconditionerJacobi ? CGHIP_PRECOND_JACOBI : CGHIP_PRECOND_NONE;
   o.max_iter = maxIterations;
@@ -257,6 +271,35 @@ bool CGHIP::solveFused(int maxIterations, floatType tolerance, FusedResult &resu
   result.axpy = info.ms_update * 1e-3;
   result.xpay = info.ms_xpay * 1e-3;
   return true;
+}
+
+// "backend": {"name": "hip", "device": d, "loop": hooks | fused | fused-graph
+// | mixed, "dot": tree | serial, and for mixed solves "mixed": {...}}.
+void CGHIP::recordBackend(std::ostream &os) {
+  os << "\"name\": \"hip\", \"device\": " << device << ", \"loop\": \"" << lastLoop
+     << "\", \"dot\": \"" << (dotMode == CGHIP_DOT_SERIAL ? "serial" : "tree") << "\"";
+  if (lastLoop != "mixed") return;
+  const auto num = [&os](double v) {
+    if (std::isfinite(v))
+      os << std::setprecision(17) << v;
+    else
+      os << "null";
+  };
+  os << ", \"mixed\": {\"polynomial\": \"" << polyMode << "\"";
+  if (learned) {
+    os << ", \"method\": \"" << lastLearned.method << "\", \"poly_iterations\": "
+       << lastLearned.polyIterations << ", \"jacobi_iterations\": " << lastLearned.jacobiIterations
+       << ", \"true_residual\": ";
+    num(lastLearned.trueResidual);
+  } else {
+    os << ", \"inner_iterations\": " << lastMixed.iterations
+       << ", \"reliable_updates\": " << lastMixed.updates << ", \"restarts\": " << lastMixed.restarts
+       << ", \"true_residual\": ";
+    num(lastMixed.true_residual);
+    os << ", \"inner_residual\": ";
+    num(lastMixed.inner_residual);
+  }
+  os << "}";
 }
 
 CG *CG::getInstance() { return new CGHIP; }

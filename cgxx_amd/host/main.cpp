@@ -3,7 +3,8 @@
 // then per repetition transfer, solve, transfer back, check and report.
 // The exit status is 0 whatever the check says (as in the reference) unless
 // CG_STRICT_CHECK is set to a non-zero value, which makes a failed check
-// exit with status 2.
+// exit with status 2.  With CG_REPORT_JSON=<path>, every repetition also
+// appends its figures to <path> as one line of JSON (CG::writeRecord).
 #include <cerrno>
 #include <cstdlib>
 #include <cstring>
@@ -29,6 +30,8 @@ int main(int argc, char *argv[]) {
   }
   const char *strict = std::getenv("CG_STRICT_CHECK");
   const bool strictCheck = strict && *strict && std::strcmp(strict, "0") != 0;
+  const char *report = std::getenv("CG_REPORT_JSON");
+  if (report != nullptr && *report == 0) report = nullptr;
 
   std::unique_ptr<CG> cg(CG::getInstance());
   cg->parseEnvironment();
@@ -42,6 +45,10 @@ int main(int argc, char *argv[]) {
     if (cg->needsTransfer()) cg->transferFrom();
     allCorrect = cg->check() && allCorrect;
     cg->printSummary();
+    if (report != nullptr && !cg->writeRecord(report, i)) {
+      std::cerr << "Could not write the solve record to " << report << "!" << std::endl;
+      std::exit(1);
+    }
     cg->resetX();
     cg->resetNonInitTimings();
   }

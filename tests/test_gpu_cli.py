@@ -124,3 +124,53 @@ def test_mixed_precision_cli_rejects(mtx):
     nomodel = run(EXE, mtx["lap2d_256"], CG_MATRIX_FORMAT="CRS", CG_HIP_MIXED=1,
                   CG_HIP_POLY="learned", CG_HIP_MODEL="/nonexistent/model.bin")
     assert nomodel.returncode == 1 and "cannot open model file" in nomodel.stderr
+
+
+def field(stdout, label, index=0):
+    vals = [ln.split(":", 1)[1].strip() for ln in stdout.splitlines() if ln.startswith(label + ":")]
+    return vals[index]
+
+
+@pytest.mark.parametrize("mode", ["fused", "hooks", "mixed", "learned"])
+def test_solve_record(mtx, mode, tmp_path):
+    """CG_REPORT_JSON: one JSON line per repetition carrying the printed
+    report's figures unrounded, plus what the backend ran."""
+    rec = tmp_path / "solves.jsonl"
+    env = dict(CG_MATRIX_FORMAT="CRS", CG_PRECONDITIONER="jacobi", CG_REPORT_JSON=str(rec), _reps=2)
+    if mode == "hooks":
+        env["CG_HIP_FUSED"] = 0
+    if mode in ("mixed", "learned"):
+        env.update(CG_HIP_MIXED=1, CG_MAX_ITER=20000)
+    if mode == "learned":
+        env["CG_HIP_POLY"] = "learned"
+    ours = run(EXE, mtx["diff3d_24"], **env)
+    assert ours.returncode == 0, ours.stderr
+    import json
+    lines = rec.read_text().splitlines()
+    assert len(lines) == 2
+    for i, ln in enumerate(lines):
+        r = json.loads(ln)
+        assert r["repetition"] == i and r["matrix"] == mtx["diff3d_24"]
+        assert (r["rows"], r["nonzeros"]) == tuple(int(v) for v in field(ours.stdout, "# rows / # nonzeros", i).split(" / "))
+        assert r["format"] == "CRS" and r["preconditioner"] == "jacobi"
+        assert r["iterations"] == int(field(ours.stdout, "Iterations", i))
+        assert "%e" % r["residual"] == field(ours.stdout, "Residual", i)
+        assert r["incorrect_elements"] == 0
+        assert abs(r["times"]["solve"] - float(field(ours.stdout, "Solve time", i))) <= 1e-6
+        assert abs(r["times"]["io"] - float(field(ours.stdout, "IO time", i))) <= 1e-6
+        b = r["backend"]
+        assert b["name"] == "hip" and b["dot"] == "tree"
+        assert b["loop"] == {"fused": "fused", "hooks": "hooks", "mixed": "mixed", "learned": "mixed"}[mode]
+        if mode == "mixed":
+            assert b["mixed"]["polynomial"] == "none" and b["mixed"]["true_residual"] == r["residual"]
+            assert b["mixed"]["inner_iterations"] == r["iterations"]
+        if mode == "learned":
+            m = b["mixed"]
+            assert m["polynomial"] == "learned" and m["method"] in ("poly", "poly+jacobi", "jacobi (refused)",
+                                                                   "jacobi-selected")
+            assert m["poly_iterations"] + m["jacobi_iterations"] == r["iterations"]
+
+
+def test_solve_record_unwritable(mtx, tmp_path):
+    ours = run(EXE, mtx["lap2d_256"], CG_REPORT_JSON=str(tmp_path / "no" / "such" / "dir.jsonl"))
+    assert ours.returncode == 1 and "Could not write the solve record" in ours.stderr
