@@ -23,7 +23,7 @@
 //                   print as 0).  Default: one iteration at a time with every
 //                   kernel timed by HIP events (one host round trip per
 //                   iteration).
-//   CG_HIP_MIXED    1: FP32-inner CG with FP64 reliable updates (CRS only;
+//   CG_HIP_MIXED    1: FP32-inner CG with FP64 reliable updates (CRS or COO;
 //                   with Jacobi on the scaled operator D^-1/2 A D^-1/2); the
 //                   reported residual is the FP64 true relative residual.
 //   CG_HIP_POLY     none (default) | learned | fixed: with CG_HIP_MIXED=1 and
@@ -39,6 +39,7 @@
 #include <iomanip>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include <limits.h>
 #include <unistd.h>
@@ -94,8 +95,9 @@ class CGHIP : public CG {
     if (rc != CGHIP_OK) hip_fail(ctx, what);
   }
 
+  // COO matrices run on the CRS kernels (see doTransferTo).
   bool supportsMatrixFormat(MatrixFormat format) override {
-    return format == MatrixFormatCRS || format == MatrixFormatELL;
+    return format == MatrixFormatCOO || format == MatrixFormatCRS || format == MatrixFormatELL;
   }
   bool supportsPreconditioner(Preconditioner p) override { return p == PreconditionerJacobi; }
   bool needsTransfer() override { return true; }
@@ -170,7 +172,26 @@ public:
 
 void CGHIP::doTransferTo() {
   chk(cghip_set_size(ctx, N), "set_size");
-  if (matrixFormat == MatrixFormatCRS) {
+  if (matrixFormat == MatrixFormatCOO) {
+    // SerialCG's COO SpMV (SerialCG.cpp:79-85) adds each row's products from
+    // y = 0 in COO order, which is the CRS kernels' left-to-right order over
+    // the stable CRS of the same entries.  Rows in ascending order (general
+    // files written by rows) make J and V that CRS as they are, with row
+    // pointers from the row counts; other orders (symmetric files) go
+    // through the stable parallel conversion first.
+    const MatrixCOO &coo = *matrixCOO;
+    if (coo.rowsSorted) {
+      std::vector<int> ptr((size_t)N + 1, 0);
+      for (int i = 0; i < N; i++) ptr[i + 1] = ptr[i] + coo.nzPerRow[i];
+      chk(cghip_upload_crs(ctx, N, nz, ptr.data(), coo.J.get(), coo.V.get()), "upload_crs");
+    } else {
+      MatrixCRS crs;
+      crs.convert(coo);
+      chk(cghip_upload_crs(ctx, N, nz, crs.ptr, crs.index, crs.value), "upload_crs");
+      crs.deallocate();
+    }
+    chk(cghip_set_format(ctx, CGHIP_FORMAT_CRS), "set_format");
+  } else if (matrixFormat == MatrixFormatCRS) {
     chk(cghip_upload_crs(ctx, N, nz, matrixCRS->ptr, matrixCRS->index, matrixCRS->value),
         "upload_crs");
     chk(cghip_set_format(ctx, CGHIP_FORMAT_CRS), "set_format");
@@ -184,8 +205,8 @@ void CGHIP::doTransferTo() {
   chk(cghip_upload_vector(ctx, CGHIP_VEC_K, k), "upload k");
   chk(cghip_upload_vector(ctx, CGHIP_VEC_X, x), "upload x");
   if (mixed) {
-    if (matrixFormat != MatrixFormatCRS) {
-      std::cerr << "CG_HIP_MIXED needs CG_MATRIX_FORMAT=CRS!" << std::endl;
+    if (matrixFormat == MatrixFormatELL) {
+      std::cerr << "CG_HIP_MIXED needs CG_MATRIX_FORMAT=CRS (or COO)!" << std::endl;
       std::exit(1);
     }
     chk(cghip_prepare_fp32(ctx), "prepare_fp32");

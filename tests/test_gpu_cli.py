@@ -58,7 +58,7 @@ def mtx(tmp_path_factory):
 
 
 @pytest.mark.parametrize("name", ["lap2d_256", "rand_sym_40", "diff3d_24"])
-@pytest.mark.parametrize("fmt", ["CRS", "ELL"])
+@pytest.mark.parametrize("fmt", ["CRS", "ELL", "COO"])
 def test_reference_order_output_identical(mtx, name, fmt):
     env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=5000)
     ref = run(REF, mtx[name], **env)
@@ -67,8 +67,8 @@ def test_reference_order_output_identical(mtx, name, fmt):
     assert untimed(ours.stdout) == untimed(ref.stdout)
 
 
-@pytest.mark.parametrize("name", ["lap2d_256", "diff3d_24"])
-@pytest.mark.parametrize("fmt", ["CRS", "ELL"])
+@pytest.mark.parametrize("name", ["lap2d_256", "rand_sym_40", "diff3d_24"])
+@pytest.mark.parametrize("fmt", ["CRS", "ELL", "COO"])
 def test_fused_matches_reference_run(mtx, name, fmt):
     env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=5000)
     ref = run(REF, mtx[name], **env)
@@ -98,12 +98,15 @@ def test_repetitions_and_strict_check(mtx):
 
 
 @pytest.mark.parametrize("poly", ["none", "learned", "fixed"])
-@pytest.mark.parametrize("name", ["lap2d_256", "diff3d_24"])
-def test_mixed_precision_cli(mtx, name, poly):
+@pytest.mark.parametrize("name,fmt", [("lap2d_256", "CRS"), ("diff3d_24", "CRS"), ("diff3d_24", "COO"),
+                                      ("rand_sym_40", "COO")])
+def test_mixed_precision_cli(mtx, name, fmt, poly):
     """CG_HIP_MIXED=1: the FP32-inner solver reaches the FP64 true residual
     (the printed residual) and passes the reference's check; CG_HIP_POLY adds
-    the polynomial preconditioner (learned: the committed network)."""
-    env = dict(CG_MATRIX_FORMAT="CRS", CG_PRECONDITIONER="jacobi", CG_MAX_ITER=20000,
+    the polynomial preconditioner (learned: the committed network).  COO
+    files run on the same device CRS (sorted rows: the file's own arrays;
+    the symmetric file: the stable conversion)."""
+    env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=20000,
                CG_HIP_MIXED=1, CG_HIP_POLY=poly, CG_STRICT_CHECK=1)
     ours = run(EXE, mtx[name], **env)
     assert ours.returncode == 0, ours.stderr + ours.stdout

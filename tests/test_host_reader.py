@@ -304,6 +304,8 @@ def test_cli_and_environment_errors():
         env = dict(os.environ, **{var: val})
         r = subprocess.run([exe, "x.mtx"], capture_output=True, text=True, env=env)
         assert r.returncode == 1 and r.stderr.strip() == msg, (var, r.stderr)
+    # COO is accepted (it runs on the device CRS); the run then stops later,
+    # at the missing file or, without a GPU, at the device.
     env = dict(os.environ, CG_MATRIX_FORMAT="coo")
     r = subprocess.run([exe, "x.mtx"], capture_output=True, text=True, env=env)
-    assert r.returncode == 1 and r.stderr.strip() == "No support for this matrix format!"
+    assert r.returncode == 1 and "matrix format" not in r.stderr
