@@ -13,7 +13,7 @@ GOLDEN_DIR = os.path.join(REPO, "tests", "golden")
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (select with -m gpu)")
 
 
 def load_golden():
