@@ -109,11 +109,17 @@ def main() -> int:
                     help="short run for rocprofv3 --pmc: one D2D copy of a vector (the "
                          "calibration kernel of known size), 20 FP64 CRS, 20 FP64 ELL, 20 mixed "
                          "Jacobi and 5 mixed polynomial iterations")
+    ap.add_argument("--workload", choices=("diffusion256", "poisson128"), default="diffusion256",
+                    help="poisson128: the plain FP64 path on the 128^3 7-point Poisson system "
+                         "(CRS and ELL, per-hook GB/s, the CPU ports' full solves beside it); "
+                         "a report of its own, not the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         return 2
+    if args.workload == "poisson128":
+        return poisson128(args)
 
     import numpy as np
 
@@ -343,6 +349,85 @@ def main() -> int:
     }
     mlp.close()
     ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def poisson128(args) -> int:
+    """The plain FP64 path on the 3D 7-point Poisson system on 128^3
+    (N = 2,097,152, 14,581,760 nonzeros, values 6/-1), k = A*1, x0 = 0,
+    Jacobi, stop at relative residual 1e-9: the fused device loop and the
+    reference's loop hook by hook, in CRS and in ELL; each hook's kernel
+    time and GB/s (algorithmic bytes: SpMV as in crs_bytes/ell_bytes, dot
+    16 N, axpy/xpay/Jacobi 24 N, copy 16 N); beside them the oracle's
+    SerialCG port (1 core) and OpenMP port (up to 16 threads), full solves
+    on this host.  The matrix (217 MB) fits in the 256 MB last-level cache,
+    so the rates are effective ones.  Prints one JSON object."""
+    import numpy as np
+
+    from cgxx_amd import problems
+    from cgxx_amd.cghip import Context, reference_loop
+
+    m = problems.laplace3d(128)
+    n, nnz = m.n, m.nnz
+    k = m.ones_rhs()
+    c = 1.0 / m.diagonal()
+    ctx = Context(0)
+    ctx.upload_crs(m.ptr, m.index, m.value)
+    ctx.upload_ell(*m.to_ell())
+    ctx.upload_jacobi(c)
+    ctx.upload("K", k)
+    log(f"128^3 Poisson: N={n} nnz={nnz}")
+    out = {"workload": "3D 7-point Poisson 128^3 (6/-1), k=A*1, x0=0, Jacobi, relative residual 1e-9",
+           "n": n, "nnz": nnz, "formats": {}}
+    vec_bytes = {"dot": 16 * n, "axpy": 24 * n, "xpay": 24 * n, "jacobi": 24 * n, "cpy": 16 * n}
+    for fmt in ("CRS", "ELL"):
+        ctx.set_format(fmt)
+
+        def fused():
+            ctx.fill("X", 0.0)
+            return ctx.solve(precond="jacobi", max_iter=1000, tol=1e-9)
+
+        fused()
+        t0 = time.perf_counter()
+        infos = [fused() for _ in range(args.steps)]
+        sec = (time.perf_counter() - t0) / args.steps
+        nr, nb = ctx.true_residual("X", "K")
+        err = float(np.max(np.abs(ctx.download("X") - 1.0)))
+        ctx.fill("X", 0.0)
+        t0 = time.perf_counter()
+        it_h, res_h, _ = reference_loop(ctx, "jacobi", 1000, 1e-9)
+        sec_h = time.perf_counter() - t0
+        for v in ("P", "R", "Q", "Z"):
+            ctx.fill(v, 1.0)
+        by = {"matvec": crs_bytes(n, nnz, 8, 8) if fmt == "CRS" else ell_bytes(n, nnz, 8, 8),
+              **vec_bytes}
+        hooks = {}
+        for h in ("matvec", "dot", "axpy", "xpay", "jacobi", "cpy"):
+            ms = ctx.time_hook(h, reps=50)
+            hooks[h] = {"ms": ms, "bytes": by[h], "gbps": by[h] / (ms * 1e-3) / 1e9}
+        out["formats"][fmt] = {
+            "fused_seconds": sec, "iterations": infos[-1]["iterations"],
+            "residual": infos[-1]["residual"], "true_relative_residual": nr / nb,
+            "max_abs_error_vs_ones": err,
+            "hook_by_hook_seconds": sec_h, "hook_by_hook_iterations": it_h, "hooks": hooks}
+        log(f"{fmt}: fused {infos[-1]['iterations']} iterations {sec:.4f} s, hook by hook {it_h} "
+            f"iterations {sec_h:.3f} s; " + ", ".join(f"{h} {v['gbps']:.0f} GB/s" for h, v in hooks.items()))
+    ctx.close()
+    if not args.no_cpu:
+        from oracle import oracle
+
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        cpu = {}
+        for name, order, nt in (("serial", 0, 1), ("openmp", 3, threads)):
+            t0 = time.perf_counter()
+            _, it, res, _ = oracle.pcg(n=n, fmt="CRS", precond="jacobi", order=order, nthreads=nt, k=k,
+                                       tol=1e-9, max_iter=1000, crs=(m.ptr, m.index, m.value), C_jac=c,
+                                       history=False)
+            cpu[name] = {"seconds": time.perf_counter() - t0, "iterations": it, "residual": res,
+                         "cores": nt, "kind": "port"}
+            log(f"cpu {name} ({nt} cores): {it} iterations {cpu[name]['seconds']:.2f} s")
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     return 0
 

@@ -25,7 +25,7 @@ SYMBOLS = [
     "cghip_set_size", "cghip_upload_crs", "cghip_upload_ell", "cghip_upload_jacobi",
     "cghip_set_format", "cghip_upload_vector", "cghip_download_vector", "cghip_set_dot_mode",
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
-    "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec",
+    "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec", "cghip_time_hook",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
     "cghip_set_poly", "cghip_matrix_stats", "cghip_time_poly_step", "cghip_fp32_layout",
     "cghip_mixed_iteration_bytes",
@@ -111,6 +111,7 @@ def load() -> C.CDLL:
     L.cghip_solve.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]
     L.cghip_history.argtypes = [vp, C.c_int, _f64p, C.c_int, C.POINTER(C.c_int)]
     L.cghip_time_matvec.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.cghip_time_hook.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.cghip_fill_vector.argtypes = [vp, C.c_int, C.c_double]
     L.cghip_true_residual.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.cghip_prepare_fp32.argtypes = [vp]
@@ -306,6 +307,14 @@ class Context:
     def time_matvec(self, x="P", y="Q", reps=20) -> float:
         ms = C.c_double()
         self._chk(self.L.cghip_time_matvec(self._h, VEC[x], VEC[y], reps, C.byref(ms)), "cghip_time_matvec")
+        return ms.value
+
+    HOOKS = {"matvec": 0, "axpy": 1, "xpay": 2, "dot": 3, "jacobi": 4, "cpy": 5}
+
+    def time_hook(self, hook: str, reps=20) -> float:
+        """Average kernel time (ms) of one hook's device work (cghip_time_hook)."""
+        ms = C.c_double()
+        self._chk(self.L.cghip_time_hook(self._h, self.HOOKS[hook], reps, C.byref(ms)), "cghip_time_hook")
         return ms.value
 
 

@@ -1280,4 +1280,40 @@ int cghip_time_matvec(cghip_ctx *c, int x, int y, int reps, double *avg_ms) {
   return CGHIP_OK;
 }
 
+int cghip_time_hook(cghip_ctx *c, int hook, int reps, double *avg_ms) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (hook < CGHIP_HOOK_MATVEC || hook > CGHIP_HOOK_CPY || reps <= 0 || !avg_ms)
+    return fail(c, CGHIP_ERR_ARG, "invalid arguments");
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  if (hook == CGHIP_HOOK_MATVEC)
+    if (int rc = check_ready(c)) return rc;
+  if (hook == CGHIP_HOOK_JACOBI && !c->has_jac)
+    return fail(c, CGHIP_ERR_STATE, "no Jacobi preconditioner uploaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (int rc = ensure_events(c, 2)) return rc;
+  double *P = c->vec[CGHIP_VEC_P], *Q = c->vec[CGHIP_VEC_Q], *R = c->vec[CGHIP_VEC_R],
+         *Z = c->vec[CGHIP_VEC_Z];
+  const size_t bytes = sizeof(double) * (size_t)c->n;
+  auto launch = [&]() -> hipError_t {
+    switch (hook) {
+      case CGHIP_HOOK_MATVEC: return launch_matvec(c, P, Q);
+      case CGHIP_HOOK_AXPY: return axpy(c->n, 1e-3, P, Q, c->stream);
+      case CGHIP_HOOK_XPAY: return xpay(c->n, P, 0.5, R, c->stream);
+      case CGHIP_HOOK_DOT:
+        return c->dot_mode == CGHIP_DOT_SERIAL
+                   ? dot_serial(c->n, P, R, c->dot_out, c->stream)
+                   : dot_tree(c->n, P, R, pr_of(c), c->dot_out, c->stream);
+      case CGHIP_HOOK_JACOBI: return jacobi(c->n, c->jac, R, Z, c->stream);
+      default: return hipMemcpyAsync(Q, P, bytes, hipMemcpyDeviceToDevice, c->stream);
+    }
+  };
+  HIPCHK(c, launch());  // warm-up
+  HIPCHK(c, hipEventRecord(c->events[0], c->stream));
+  for (int i = 0; i < reps; i++) HIPCHK(c, launch());
+  HIPCHK(c, hipEventRecord(c->events[1], c->stream));
+  HIPCHK(c, hipEventSynchronize(c->events[1]));
+  *avg_ms = elapsed(c->events[0], c->events[1]) / reps;
+  return CGHIP_OK;
+}
+
 }  // extern "C"

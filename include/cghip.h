@@ -245,6 +245,18 @@ int cghip_matrix_stats(cghip_ctx *ctx, double *stats, int cap);
  * x -> y, timed with HIP events on the context's stream. */
 int cghip_time_matvec(cghip_ctx *ctx, int x, int y, int reps, double *avg_ms);
 
+/* Average duration (ms) of `reps` back-to-back launches of one hook's device
+ * work on the context's vectors, timed with HIP events on the context's
+ * stream (no host synchronisation between launches): MATVEC Q = A P (the
+ * selected format), AXPY Q += 1e-3 P, XPAY R = P + 0.5 R, DOT <P, R> (the dot
+ * mode's order, partials and final fold), JACOBI Z = C o R, CPY Q = P.  The
+ * vectors must hold finite values; Q, R and Z are overwritten.  No reference
+ * counterpart: the measurement behind the per-hook bandwidths of the
+ * reference's timing categories (CG.h:97-128). */
+enum { CGHIP_HOOK_MATVEC = 0, CGHIP_HOOK_AXPY = 1, CGHIP_HOOK_XPAY = 2, CGHIP_HOOK_DOT = 3,
+       CGHIP_HOOK_JACOBI = 4, CGHIP_HOOK_CPY = 5 };
+int cghip_time_hook(cghip_ctx *ctx, int hook, int reps, double *avg_ms);
+
 /* Average duration (ms) of one Clenshaw step of the polynomial set by
  * cghip_set_poly on the scaled FP32 operator: `reps` applications of the
  * whole recurrence back to back on the context's stream between two HIP

@@ -19,9 +19,12 @@ from tests.conftest import golden_runs, load_golden, run_id
 
 pytestmark = pytest.mark.gpu
 
+# COO runs execute on the device CRS (the stable CRS of the COO entries adds
+# each row in COO order, as SerialCG's COO SpMV does), so the reference's COO
+# histories are targets of the device runs too.
 SMALL = golden_runs(max_n=70_000)
-DEVICE_RUNS = [p for p in SMALL if p[1]["env"]["CG_MATRIX_FORMAT"] != "COO"]
-ALL_DEVICE_RUNS = [p for p in golden_runs() if p[1]["env"]["CG_MATRIX_FORMAT"] != "COO"]
+DEVICE_RUNS = SMALL
+ALL_DEVICE_RUNS = golden_runs()
 
 
 @pytest.fixture(scope="module")
@@ -33,9 +36,17 @@ def ctx():
     c.close()
 
 
+def device_fmt(fmt: str) -> str:
+    """The device format a run's CG_MATRIX_FORMAT executes in."""
+    return "ELL" if fmt == "ELL" else "CRS"
+
+
 def _upload(ctx, case: Case, fmt: str, precond: str):
-    ctx.upload_crs(*case.crs) if fmt == "CRS" else ctx.upload_ell(*case.ell)
-    ctx.set_format(fmt)
+    if fmt == "ELL":
+        ctx.upload_ell(*case.ell)
+    else:  # CRS, and COO on the device CRS
+        ctx.upload_crs(*case.crs)
+    ctx.set_format("ELL" if fmt == "ELL" else "CRS")
     if precond == "jacobi":
         ctx.upload_jacobi(case.C)
     ctx.upload("K", case.k)
@@ -186,7 +197,7 @@ def test_reference_loop_tree_dots_match_oracle(ctx, pair, tmp_path):
     fmt, pre, maxit, tol = run_env(run)
     _fresh(ctx, case, fmt, pre)
     it, res, h = reference_loop(ctx, pre, maxit, tol)
-    ox, oit, ores, oh = oracle.pcg(n=case.n, fmt=fmt, precond=pre, order=2, k=case.k, tol=tol,
+    ox, oit, ores, oh = oracle.pcg(n=case.n, fmt=device_fmt(fmt), precond=pre, order=2, k=case.k, tol=tol,
                                    max_iter=maxit, crs=case.crs, ell=case.ell, C_jac=case.C)
     assert it == oit
     assert_bitwise(h["r2"], oh["r2"], "r2")
@@ -207,7 +218,7 @@ def test_fused_solve(ctx, pair, tmp_path):
     h = {k: ctx.history(k, it + 1) for k in ("r2", "rho", "pq", "alpha", "beta")}
     x = ctx.download("X")
     if case.n <= 300_000:
-        ox, oit, ores, oh = oracle.pcg(n=case.n, fmt=fmt, precond=pre, order=1, k=case.k, tol=tol,
+        ox, oit, ores, oh = oracle.pcg(n=case.n, fmt=device_fmt(fmt), precond=pre, order=1, k=case.k, tol=tol,
                                        max_iter=maxit, crs=case.crs, ell=case.ell, C_jac=case.C)
         assert it == oit
         assert_bitwise(h["r2"], oh["r2"], "r2")
@@ -284,3 +295,24 @@ def test_device_sqrt_and_division_are_correctly_rounded(ctx, tmp_path):
     alpha = ctx.history("alpha", it)
     assert info["residual"] == float(np.sqrt(r2[-1]) / np.sqrt(r2[0]))
     assert_bitwise(alpha, rho[:it] / pq, "alpha")
+
+
+@pytest.mark.parametrize("fmt", ["CRS", "ELL"])
+def test_time_hook(ctx, fmt, tmp_path):
+    """cghip_time_hook runs every hook's device work and returns a positive
+    average; the hooks it times leave finite values and the same results as
+    the hook calls themselves."""
+    case = Case(load_golden()[0], str(tmp_path))
+    _fresh(ctx, case, fmt, "jacobi")
+    n = case.n
+    x = np.linspace(0.5, 1.5, n)
+    for v in ("P", "R", "Q", "Z"):
+        ctx.upload(v, x)
+    for hook in ("matvec", "axpy", "xpay", "dot", "jacobi", "cpy"):
+        ms = ctx.time_hook(hook, reps=3)
+        assert ms > 0, hook
+    ctx.upload("R", x)
+    ctx.time_hook("jacobi", reps=1)
+    assert np.array_equal(ctx.download("Z"), case.C * x)
+    ctx.time_hook("matvec", reps=1)
+    assert np.all(np.isfinite(ctx.download("Q")))
