@@ -990,7 +990,15 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   if (int rc = sync(c)) return rc;
 
   const bool profile = o->profile == CGHIP_PROFILE_KERNELS;
-  const int batch = profile ? 1 : (o->batch > 0 ? o->batch : (n >= (1 << 20) ? 8 : 32));
+  // Iterations per graph launch: the kernels of a batch that follow a stop or
+  // a reliable-update request exit at once but still cost a launch each, so
+  // the long polynomial iterations of large systems take small batches (256^3:
+  // 155.1-155.3 ms at 4-8, 158.3 ms at 32), the short Jacobi ones large batches
+  // (271.8 ms at 8, 270.0 ms at 32).
+  const int batch = profile ? 1
+                    : o->batch > 0 ? o->batch
+                    : (poly && n >= (1 << 20)) ? 8
+                                               : 32;
   if (profile && ensure_events(c, 7)) return CGHIP_ERR_HIP;
   hipEvent_t *ev = profile ? c->events.data() : nullptr;
 
