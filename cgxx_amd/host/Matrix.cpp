@@ -240,7 +240,10 @@ int env_threads() {
   return (int)t;
 }
 
+// What identifies the source file of a cache: device, inode, size and
+// modification time.
 struct Source {
+  long long dev = -1, ino = -1;
   long long size = -1;
   long long mtime_ns = -1;
 };
@@ -249,6 +252,8 @@ Source stat_source(const char *file) {
   struct stat st;
   Source s;
   if (::stat(file, &st) == 0) {
+    s.dev = (long long)st.st_dev;
+    s.ino = (long long)st.st_ino;
     s.size = (long long)st.st_size;
     s.mtime_ns = (long long)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
   }
@@ -264,8 +269,10 @@ struct CacheHeader {
   int64_t nz;
   int64_t source_size;
   int64_t source_mtime_ns;
+  int64_t source_dev;
+  int64_t source_ino;
 };
-const char kCacheMagic[8] = {'C', 'G', 'C', 'O', 'O', 'v', '1', 0};
+const char kCacheMagic[8] = {'C', 'G', 'C', 'O', 'O', 'v', '2', 0};
 
 }  // namespace
 
@@ -435,7 +442,8 @@ bool MatrixCOO::loadCache(const char *cache, const char *file) {
   if (!f) return false;
   CacheHeader h;
   bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, kCacheMagic, 8) == 0 &&
-            h.version == 1 && h.source_size == src.size && h.source_mtime_ns == src.mtime_ns &&
+            h.version == 2 && h.source_size == src.size && h.source_mtime_ns == src.mtime_ns &&
+            h.source_dev == src.dev && h.source_ino == src.ino &&
             h.N > 0 && h.nz >= 0 && h.nz <= 0x7fffffffLL;
   if (ok) {
     struct stat st;
@@ -468,11 +476,13 @@ void MatrixCOO::saveCache(const char *cache, const char *file) const {
   const Source src = stat_source(file);
   CacheHeader h;
   std::memcpy(h.magic, kCacheMagic, 8);
-  h.version = 1;
+  h.version = 2;
   h.N = N;
   h.nz = nz;
   h.source_size = src.size;
   h.source_mtime_ns = src.mtime_ns;
+  h.source_dev = src.dev;
+  h.source_ino = src.ino;
   const std::string tmp = std::string(cache) + ".tmp" + std::to_string((long)::getpid());
   FILE *f = std::fopen(tmp.c_str(), "wb");
   bool ok = f != nullptr;

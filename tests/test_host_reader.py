@@ -272,11 +272,18 @@ def test_matrix_cache(tmp_path, dump_bin):
 
     # Corrupt an index in the cache: it is not used, the source is parsed.
     raw = bytearray((tmp_path / "swap.cache").read_bytes())
-    hdr = 8 + 4 + 4 + 8 + 8 + 8
+    hdr = 8 + 4 + 4 + 8 + 8 + 8 + 8 + 8
     raw[hdr: hdr + 4] = struct.pack("<i", 1 << 30)
     (tmp_path / "swap.cache").write_bytes(bytes(raw))
     r = run_dump(dump_bin, p, second, 2, tmp_path / "swap.cache")
     assert r.returncode == 0 and read_dump(str(second))[3][2][0] == 2.5
+
+    # Another file (other inode) of the same size and time stamp does not use the cache.
+    other = tmp_path / "other.mtx"
+    diag_file(other, ["3.75"] * 40)
+    os.utime(other, ns=(st.st_atime_ns, st.st_mtime_ns + 1_000_000_000))
+    run_dump(dump_bin, other, second, 2, tmp_path / "swap.cache")
+    assert read_dump(str(second))[3][2][0] == 3.75
 
 
 def test_missing_file(tmp_path, dump_bin):
