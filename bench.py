@@ -264,9 +264,11 @@ def main() -> int:
     deg = learned.DEGREE
     step_ms_events = per(profl, "poly")  # kernel by kernel, events between iterations
     step_ms = ctx.time_poly_step(reps=5)  # steady state: applications back to back
-    # Clenshaw step j: the operator stream, B_{j+1} gathered, r read, B_j
-    # written, B_{j+2} read in all but the first step of the recurrence
-    step_bytes = poly_op_bytes + 4 * n + 4 * n + 4 * n + (deg - 1) / deg * 4 * n
+    # Clenshaw steps, averaged over one application: each streams the
+    # operator and writes B_j; the first reads r only (gathered, as
+    # B_k = c_k r), the second B_{k-1} and r (also as B_k), every later one
+    # B_{j+1} (gathered), r and B_{j+2}
+    step_bytes = poly_op_bytes + (8 * n + 12 * n + (deg - 2) * 16 * n) / deg
     step_gbps = step_bytes / (step_ms * 1e-3) / 1e9
     log(f"mixed+learned: method {infol['method']}, {infol['poly_iterations']} poly + "
         f"{infol['jacobi_iterations']} Jacobi iterations ({infol['spmvs']} FP32 SpMVs), true residual "
@@ -339,8 +341,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": step_gbps / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(f"k_poly_step<false, {kind}>"),
-            "kernel": f"k_poly_step<false, {kind}> (Clenshaw step, scaled operator in {layout}"
+            "traffic": traffic_from_profiles(f"k_poly_step<false, {kind}, 0>"),
+            "kernel": f"k_poly_step<false, {kind}, 0> (Clenshaw step, scaled operator in {layout}"
                       + (", FP16 values)" if layout == "DIA" else ", FP32 values)"),
             "polynomial": prof_src,
             "algorithmic_bytes_per_launch": step_bytes,

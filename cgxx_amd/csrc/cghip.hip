@@ -1015,7 +1015,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
     HIPCHK(c, matvec_crs(crs_of(c), x64, q64, c->stream));
     HIPCHK(c, mp_residual(n, k64, q64, r64, pt, c->stream));
     if (poly) {
-      HIPCHK(c, mp_restart_poly(n, r64, scale, r32, y32, c->d_poly, w[degree % 3], prr, c->stream));
+      HIPCHK(c, mp_restart(n, r64, scale, r32, y32, p32, false, prr, c->stream));
       HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, first ? p32 : nullptr, prz, nullptr,
                               c->stream));
     } else {
@@ -1035,8 +1035,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
     HIPCHK(c, mp_matvec_dot(A32, p32, q32, pa, c->d_mstate, c->stream));
     if (e) HIPCHK(c, hipEventRecord(e[1], c->stream));
     if (poly) {
-      HIPCHK(c, mp_update_poly(n, gpq, r32, q32, c->d_poly, w[degree % 3], pa, prr, c->d_mstate,
-                               h, c->stream));
+      HIPCHK(c, mp_update(n, gpq, r32, q32, pa, prr, c->d_mstate, h, c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
       HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, nullptr, prz, c->d_mstate,
                               c->stream));
@@ -1243,10 +1242,14 @@ int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) 
     *bytes = spmv + 12.0 * n + 20.0 * n;  // + r update (r, q; r) + y, p update (y, p, r; y, p)
     return CGHIP_OK;
   }
-  const double k = c->poly_degree;
-  // r update also writes B_k; each Clenshaw step reads B_{j+1}, r, writes
-  // B_j, and all but the first read B_{j+2}; y, p update reads y, p, z
-  *bytes = spmv + 16.0 * n + k * (op16 + 12.0 * n) + (k - 1.0) * 4.0 * n + 20.0 * n;
+  const int k = c->poly_degree;
+  // r update (r, q; r).  Clenshaw steps: the first reads r (gathered, as
+  // B_k = c_k r) and writes B_{k-1}; the second reads B_{k-1}, r (also as
+  // B_k) and writes B_{k-2}; every later one reads B_{j+1}, r, B_{j+2} and
+  // writes B_j.  y, p update reads y, p, z and writes y, p.
+  const double steps = k * op16 + 8.0 * n + (k >= 2 ? 12.0 * n : 0.0) +
+                       (k > 2 ? (k - 2) * 16.0 * n : 0.0);
+  *bytes = spmv + 12.0 * n + steps + 20.0 * n;
   return CGHIP_OK;
 }
 

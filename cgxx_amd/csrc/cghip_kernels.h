@@ -286,16 +286,12 @@ hipError_t mp_restart(int n, const double *r64, const double *scale, float *r32,
 // (rz, gz partials) into the restarted state.
 hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, int gz, bool first,
                            MixedState *st, History h, hipStream_t s);
-// Polynomial variants.  Work buffers w[0..2] hold the Clenshaw B_j (B_j in
-// w[j % 3]); z ends in w[0].
-hipError_t mp_restart_poly(int n, const double *r64, const double *scale, float *r32, float *y32,
-                           const PolyParams *poly, float *bk, Partials prr, hipStream_t s);
+// Polynomial variant (its r update and restart are mp_update and mp_restart).
+// Work buffers w[0..2] hold the Clenshaw B_j (B_j in w[j % 3]; B_k = c_k r is
+// recomputed where it is read, not stored); z ends in w[0].
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s);
-hipError_t mp_update_poly(int n, int gpq, float *r32, const float *q32, const PolyParams *poly,
-                          float *bk, Partials pq, Partials prr, MixedState *st, History h,
-                          hipStream_t s);
 hipError_t mp_xpay_poly(int n, float *y32, float *p32, const float *z32, Partials prr, Partials prz,
                         int gz, MixedState *st, History h, hipStream_t s);
 hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, MixedState *st,

@@ -11,6 +11,16 @@ namespace cghip {
 namespace dev {
 
 constexpr int kRounds = kNnzCap / kBlock;  // load rounds per row block
+
+// The SpMV helpers multiply each stored value by xf(x[col]); XIdent is the
+// plain SpMV.  A scaling transform gives (c x) products bit for bit, without
+// a stored copy of c x.
+struct XIdent {
+  template <typename T>
+  __device__ T operator()(T v) const {
+    return v;
+  }
+};
 constexpr int kEllChunk = 8;               // ELL slots in flight per thread
 constexpr int kUnroll = 4;                 // element pairs in flight per thread
 
@@ -54,9 +64,9 @@ __device__ inline double fold_partials(const double *partial, int G, double *lds
 // epi(r, s, pv) with the row's sum s (products added left to right from 0)
 // and gets back its contribution to a fused dot (0 for threads without a
 // row).
-template <typename V, typename X, typename Pre, typename Epi>
+template <typename V, typename X, typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restrict__ x, int rb,
-                                              X *__restrict__ prod, Pre pre, Epi epi) {
+                                              X *__restrict__ prod, Pre pre, Epi epi, XF xf = XF()) {
   const int t = threadIdx.x;
   const int r0 = A.rb_start[rb], r1 = A.rb_start[rb + 1];
   const int e0 = A.ptr[r0], e1 = A.ptr[r1];
@@ -84,7 +94,7 @@ __device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restri
 #pragma unroll
     for (int u = 0; u < kRounds; u++) {
       const int e = e0 + u * kBlock + t;
-      if (e < e1) prod[e - e0] = X(v[u]) * x[c[u]];
+      if (e < e1) prod[e - e0] = X(v[u]) * xf(x[c[u]]);
     }
     __syncthreads();
     if (own) {
@@ -100,7 +110,7 @@ __device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restri
     if (t == 0) pv = pre(r0);
     for (int c0 = e0; c0 < e1; c0 += kNnzCap) {
       const int ce = min(c0 + kNnzCap, e1);
-      for (int e = c0 + t; e < ce; e += kBlock) prod[e - c0] = X(A.value[e]) * x[A.index[e]];
+      for (int e = c0 + t; e < ce; e += kBlock) prod[e - c0] = X(A.value[e]) * xf(x[A.index[e]]);
       __syncthreads();
       if (t == 0)
         for (int j = 0; j < ce - c0; j++) s += prod[j];
@@ -177,9 +187,9 @@ __device__ inline double ell_row_pre_epi(const Ell<V> &A, const X *__restrict__ 
 
 // The same over packed FP32 slots (index | value bits << 32): one 8-byte
 // non-temporal load per slot.
-template <typename Pre, typename Epi>
+template <typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__restrict__ x, int r,
-                                         Pre pre, Epi epi) {
+                                         Pre pre, Epi epi, XF xf = XF()) {
   const int64_t n = A.n;
   const int len = __builtin_nontemporal_load(A.length + r);
   const auto pv = pre(r);
@@ -191,7 +201,7 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
       if (j0 + u < len) e[u] = __builtin_nontemporal_load(A.slot + ((int64_t)(j0 + u) * n + r));
 #pragma unroll
     for (int u = 0; u < kEllChunk; u++)
-      if (j0 + u < len) s += __uint_as_float((uint32_t)(e[u] >> 32)) * x[(int)(uint32_t)e[u]];
+      if (j0 + u < len) s += __uint_as_float((uint32_t)(e[u] >> 32)) * xf(x[(int)(uint32_t)e[u]]);
   }
   return epi(r, s, pv);
 }
@@ -201,9 +211,9 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
 // summed left to right from 0, then epi(r, s, pv).  Values and the mask are
 // read once (non-temporal); x at r + off[d] is contiguous across a wave.
 // HALF: the values come from val16 (FP16, converted exactly to FP32).
-template <bool HALF, typename Pre, typename Epi>
+template <bool HALF, typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__ x, int r, Pre pre,
-                                         Epi epi) {
+                                         Epi epi, XF xf = XF()) {
   const uint32_t m = __builtin_nontemporal_load(A.mask + r);
   const auto pv = pre(r);
   const char *vb = HALF ? reinterpret_cast<const char *>(A.val16)
@@ -237,7 +247,7 @@ __device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__
     if (d < A.ndiag && ((m >> d) & 1u)) {
       const float a = HALF ? (float)__builtin_bit_cast(_Float16, (uint16_t)v[d])
                            : __builtin_bit_cast(float, v[d]);
-      s += a * xv[d];
+      s += a * xf(xv[d]);
     }
   return epi(r, s, pv);
 }

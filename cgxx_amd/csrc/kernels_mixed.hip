@@ -158,27 +158,28 @@ __device__ inline void chunk_walk(int nrb, int &rb0, int &stride, int &rend) {
   }
 }
 
-template <int K, bool HALF = false, bool XCD = false, typename Pre, typename Epi>
+template <int K, bool HALF = false, bool XCD = false, typename Pre, typename Epi,
+          typename XF = XIdent>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
-                                   Epi epi) {
+                                   Epi epi, XF xf = XF()) {
   double tpart = 0.0;
   if (K == kOpDia) {
     int rb0, stride, rend;
     chunk_walk<XCD>(ell_chunks(A.dia.n), rb0, stride, rend);
     for (int rb = rb0; rb < rend; rb += stride) {
       const int r = rb * kBlock + (int)threadIdx.x;
-      if (r < A.dia.n) tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi);
+      if (r < A.dia.n) tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi, xf);
     }
   } else if (K == kOpEll) {
     int rb0, stride, rend;
     chunk_walk<XCD>(ell_chunks(A.ell.n), rb0, stride, rend);
     for (int rb = rb0; rb < rend; rb += stride) {
       const int r = rb * kBlock + (int)threadIdx.x;
-      if (r < A.ell.n) tpart += ell_row_pre_epi(A.ell, x, r, pre, epi);
+      if (r < A.ell.n) tpart += ell_row_pre_epi(A.ell, x, r, pre, epi, xf);
     }
   } else {
     for (int rb = blockIdx.x; rb < A.crs.nrb; rb += gridDim.x)
-      tpart += crs_rowblock_pre_epi<float, float>(A.crs, x, rb, prod, pre, epi);
+      tpart += crs_rowblock_pre_epi<float, float>(A.crs, x, rb, prod, pre, epi, xf);
   }
   return tpart;
 }
@@ -433,101 +434,24 @@ __global__ __launch_bounds__(kBlock) void k_mp_xpay(int n, float *__restrict__ y
 // ---------------------------------------------------------------------------
 // Polynomial preconditioner (Clenshaw steps fused into FP32 SpMVs).
 
-// alpha = rho / <p,q> (into st->alpha32); r32 -= alpha q32; B_k = c_k r32;
-// partials of <r,r>.  y32 += alpha p32 is done by the p update.
-__global__ __launch_bounds__(kBlock) void k_mp_update_poly(int n, int gpq, float *__restrict__ r,
-                                                           const float *__restrict__ q,
-                                                           const PolyParams *poly,
-                                                           float *__restrict__ bk, Partials ppq,
-                                                           Partials prr, MixedState *st,
-                                                           History h) {
-  __shared__ double lds[4];
-  const MixedScalars c = st->cur;
-  if (c.done) return;
-  const double pq = fold_partials(ppq.a, gpq, lds);
-  const double alpha = c.rho / pq;
-  const float na = (float)(-alpha);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->alpha32 = (float)alpha;
-    if (h.pq) h.pq[c.iteration] = pq;
-    if (h.alpha) h.alpha[c.iteration] = alpha;
-  }
-  const float ck = poly->c[poly->degree];
-  Quads Q(n);
-  double s_rr = 0.0;
-  for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
-    float4 rv[kQuadUnroll], qv[kQuadUnroll];
-#pragma unroll
-    for (int u = 0; u < kQuadUnroll; u++) {
-      const int64_t j = Q.q + u * Q.T;
-      rv[u] = v4(r)[j];
-      qv[u] = v4(q)[j];
-    }
-#pragma unroll
-    for (int u = 0; u < kQuadUnroll; u++) {
-      const int64_t j = Q.q + u * Q.T;
-      mp_update_elem(rv[u].x, qv[u].x, na, s_rr);
-      mp_update_elem(rv[u].y, qv[u].y, na, s_rr);
-      mp_update_elem(rv[u].z, qv[u].z, na, s_rr);
-      mp_update_elem(rv[u].w, qv[u].w, na, s_rr);
-      float4 bv;
-      bv.x = ck * rv[u].x;
-      bv.y = ck * rv[u].y;
-      bv.z = ck * rv[u].z;
-      bv.w = ck * rv[u].w;
-      v4(r)[j] = rv[u];
-      v4(bk)[j] = bv;
-    }
-  }
-  for (; Q.q < Q.nquad; Q.q += Q.T) {
-    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      float re = r[e];
-      mp_update_elem(re, q[e], na, s_rr);
-      r[e] = re;
-      bk[e] = ck * re;
-    }
-  }
-  const double b0 = block_fold(s_rr, lds);
-  if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
-}
-
-// Restart of the polynomial variant: r32 = (float)(s o r64), y32 = 0,
-// B_k = c_k r32, partials of <r,r>.
-__global__ __launch_bounds__(kBlock) void k_mp_restart_poly(int n, const double *__restrict__ r64,
-                                                            const double *__restrict__ scale,
-                                                            float *__restrict__ r32,
-                                                            float *__restrict__ y32,
-                                                            const PolyParams *poly,
-                                                            float *__restrict__ bk, Partials prr) {
-  __shared__ double lds[4];
-  const float ck = poly->c[poly->degree];
-  Quads Q(n);
-  double s_rr = 0.0;
-  for (; Q.q < Q.nquad; Q.q += Q.T) {
-    for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      const float re = restart_value(r64, scale, e);
-      r32[e] = re;
-      y32[e] = 0.0f;
-      bk[e] = ck * re;
-      s_rr += (double)re * (double)re;
-    }
-  }
-  const double b0 = block_fold(s_rr, lds);
-  if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
-}
+// Which Clenshaw B the step reads from r (see k_poly_step).
+enum { kStepPlain = 0, kStepFirst = 1, kStepSecond = 2 };
 
 // Clenshaw step j with one SpMV t = A_hat B_{j+1}:
 //   s = (2/b) t - B_{j+1}
 //   j > 0: B_j = (c_j r + 2 s) - B_{j+2}
 //   j = 0: z   = (c_0 r + s) - B_2, and the partials of <r, z>
-// (B_{j+2} = 0 when it does not exist).  st == nullptr runs the step
+// (B_{j+2} = 0 when it does not exist).  B_k = c_k r is never stored: the
+// first step (MODE kStepFirst, j = k - 1) gathers c_k r[col] as its B_{j+1}
+// and the second (kStepSecond, j = k - 2) takes c_k r[i] as its B_{j+2},
+// the same products a stored B_k would give.  st == nullptr runs the step
 // unconditionally (restart); otherwise it is skipped once the iteration is
 // off.  A_hat is the scaled operator: its unit diagonal is implicit (the
 // launcher refuses any other).  The kernel keeps at most 80 SGPRs: above
 // that, fewer than 8 blocks of 256 fit on a CU of gfx950 and the grid of
 // 2048 blocks runs in two rounds (295 us instead of 200 us per step at
 // 256^3).
-template <bool FINAL, int K>
+template <bool FINAL, int K, int MODE>
 __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
                                                       const float *__restrict__ r,
@@ -540,8 +464,15 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   if (st && st->cur.done) return;
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
+  const float ck = poly->c[poly->degree];
   const double tpart = op32_rows<K, K == kOpDia, !FINAL>(
-      A, b1, prod, [&](int i) { return float4{b1[i], r[i], b2 ? b2[i] : 0.0f, 0.0f}; },
+      A, MODE == kStepFirst ? r : b1, prod,
+      [&](int i) {
+        const float ri = r[i];
+        if (MODE == kStepFirst) return float4{ck * ri, ri, 0.0f, 0.0f};
+        if (MODE == kStepSecond) return float4{b1[i], ri, ck * ri, 0.0f};
+        return float4{b1[i], ri, b2 ? b2[i] : 0.0f, 0.0f};
+      },
       [&](int i, float t0, float4 pv) -> double {
         const float bi = pv.x, ri = pv.y, bb = pv.z;
         const float t = bi + t0;
@@ -554,7 +485,8 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
         }
         b0[i] = (cj * ri + 2.0f * sv) - bb;
         return 0.0;
-      });
+      },
+      [ck](float v) { return MODE == kStepFirst ? ck * v : v; });
   if (FINAL) {
     const double bp = block_fold(tpart, lds);
     if (threadIdx.x == 0) prz.a[blockIdx.x] = bp;
@@ -655,26 +587,33 @@ hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, 
   return hipGetLastError();
 }
 
-hipError_t mp_restart_poly(int n, const double *r64, const double *scale, float *r32, float *y32,
-                           const PolyParams *poly, float *bk, Partials prr, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_restart_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, r64, scale, r32,
-                     y32, poly, bk, prr);
-  return hipGetLastError();
-}
-
 template <int K>
 static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
   const dim3 grid(op32_grid(A)), block(kBlock);
+  // B_j in w[j % 3]; B_k (= c_k r) and B_{k+1} (= 0) are not stored
   for (int j = degree - 1; j >= 1; j--) {
-    const float *b2 = (j + 2 <= degree) ? w[(j + 2) % 3] : nullptr;
-    hipLaunchKernelGGL((k_poly_step<false, K>), grid, block, 0, s, A, w[(j + 1) % 3], b2, r32,
-                       poly, j, w[j % 3], (float *)nullptr, prz, st);
+    const float *b1 = w[(j + 1) % 3], *b2 = (j + 2 <= degree - 1) ? w[(j + 2) % 3] : nullptr;
+    if (j == degree - 1)
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst>), grid, block, 0, s, A, nullptr,
+                         nullptr, r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
+    else if (j == degree - 2)
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond>), grid, block, 0, s, A, b1, nullptr,
+                         r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
+    else
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain>), grid, block, 0, s, A, b1, b2, r32,
+                         poly, j, w[j % 3], (float *)nullptr, prz, st);
   }
-  const float *b2 = degree >= 2 ? w[2] : nullptr;
-  hipLaunchKernelGGL((k_poly_step<true, K>), grid, block, 0, s, A, w[1], b2, r32, poly, 0, w[0],
-                     pcopy, prz, st);
+  if (degree == 1)
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst>), grid, block, 0, s, A, nullptr, nullptr,
+                       r32, poly, 0, w[0], pcopy, prz, st);
+  else if (degree == 2)
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond>), grid, block, 0, s, A, w[1], nullptr,
+                       r32, poly, 0, w[0], pcopy, prz, st);
+  else
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain>), grid, block, 0, s, A, w[1], w[2], r32,
+                       poly, 0, w[0], pcopy, prz, st);
 }
 
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
@@ -693,14 +632,6 @@ hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly
       if (A.crs.unit != 1) return hipErrorInvalidValue;
       poly_apply_t<kOpCrs>(A, r32, poly, degree, w, pcopy, prz, st, s);
   }
-  return hipGetLastError();
-}
-
-hipError_t mp_update_poly(int n, int gpq, float *r32, const float *q32, const PolyParams *poly,
-                          float *bk, Partials pq, Partials prr, MixedState *st, History h,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_mp_update_poly, dim3(vec_grid4(n)), dim3(kBlock), 0, s, n, gpq, r32, q32,
-                     poly, bk, pq, prr, st, h);
   return hipGetLastError();
 }
 

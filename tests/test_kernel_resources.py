@@ -19,7 +19,7 @@ from cgxx_amd import build
 HOT = [
     r"k_pcg_matvec_dot_crs\(", r"k_pcg_matvec_dot_ell<true>", r"k_matvec_crs<double, double>",
     r"k_matvec_ell<double, double, true>", r"k_pcg_update<", r"k_pcg_xpay<", r"k_mp_matvec_dot<",
-    r"k_mp_update\(", r"k_mp_xpay\(", r"k_poly_step<", r"k_mp_update_poly\(", r"k_mp_xpay_poly\(",
+    r"k_mp_update\(", r"k_mp_xpay\(", r"k_poly_step<", r"k_mp_xpay_poly\(",
 ]
 
 
