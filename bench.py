@@ -65,6 +65,19 @@ def traffic_from_profiles(name: str):
     return None if k is None else k.get("bytes_per_launch")
 
 
+def application_traffic(kind: int, degree: int):
+    """PMC bytes per Clenshaw step averaged over one application: the first
+    step (template argument 1), the second (2), the plain ones (0) and the
+    last (final, 0)."""
+    first = traffic_from_profiles(f"k_poly_step<false, {kind}, 1>")
+    second = traffic_from_profiles(f"k_poly_step<false, {kind}, 2>")
+    plain = traffic_from_profiles(f"k_poly_step<false, {kind}, 0>")
+    last = traffic_from_profiles(f"k_poly_step<true, {kind}, 0>")
+    if degree < 3 or None in (first, second, plain, last):
+        return None
+    return (first + second + (degree - 3) * plain + last) / degree
+
+
 def cpu_baseline(m, k, c, iterations: int, sample_iters: int) -> dict:
     """The oracle's ports of the reference's CPU solvers on this host: the
     OpenMP loop (CGOpenMP.cpp:188-265, order 3) on up to 16 threads, and
@@ -341,8 +354,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": step_gbps / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(f"k_poly_step<false, {kind}, 0>"),
-            "kernel": f"k_poly_step<false, {kind}, 0> (Clenshaw step, scaled operator in {layout}"
+            "traffic": application_traffic(kind, deg),
+            "kernel": f"k_poly_step<*, {kind}, *>: the Clenshaw steps of one degree-{deg} application, "
+                      f"per step (scaled operator in {layout}"
                       + (", FP16 values)" if layout == "DIA" else ", FP32 values)"),
             "polynomial": prof_src,
             "algorithmic_bytes_per_launch": step_bytes,
