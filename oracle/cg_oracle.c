@@ -692,7 +692,7 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
   o->nrb = orc_crs_rowblocks(N, o->ptr, o->starts);
   o->gmax = ORC_SPMV_GMAX;
   if (scaled) {
-    /* the library stores the scaled operator as DIA (up to 8192 blocks) or
+    /* the library stores the scaled operator as DIA (up to 4096 blocks) or
      * else as ELL (up to 2048 blocks) when W * N <= 2 (nnz + N), W the
      * longest row, both over 256-row chunks: same row sums (DIA adds a
      * row's entries by ascending diagonal, i.e. in stored order), the dot
@@ -709,7 +709,7 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
       for (int b = 0; b <= o->nrb; b++)
         o->starts[b] = b * ORC_ROWS_MAX < N ? b * ORC_ROWS_MAX : N;
       if (dia) {
-        o->gmax = ORC_ELL_GMAX;
+        o->gmax = ORC_DIA_GMAX;
         o->value16 = (float *)malloc(sizeof(float) * (size_t)(k > 0 ? k : 1));
         for (int e = 0; e < k; e++) o->value16[e] = orc_round_half(o->value[e]);
       }

@@ -46,7 +46,8 @@ extern "C" {
 #define ORC_ROWS_MAX 256  /* max rows of one SpMV row block (one per thread) */
 #define ORC_NNZ_CAP 2048  /* max nonzeros staged per CRS row block */
 #define ORC_SPMV_GMAX 2048 /* max blocks of the CRS SpMV and FP32 ELL kernels */
-#define ORC_ELL_GMAX 8192  /* max blocks of the FP64 ELL and FP32 DIA SpMV kernels */
+#define ORC_ELL_GMAX 8192  /* max blocks of the FP64 ELL SpMV kernels */
+#define ORC_DIA_GMAX 4096  /* max blocks of the FP32 DIA kernels */
 #define ORC_DIA_MAX 8      /* max diagonals of the FP32 DIA layout */
 
 /* Matrix Market COO, after the reader's symmetric mirroring. */

@@ -118,14 +118,18 @@ def test_mixed_full_size_c2(ctx):
     assert np.abs(x_mixed - 1.0).max() <= 1e-5
 
 
+@pytest.mark.parametrize("side", [84, 104])
 @pytest.mark.parametrize("precond", ["jacobi", "poly"])
-def test_mixed_dia_grid_above_2048_vs_oracle(ctx, precond):
-    """84^3 variable-coefficient diffusion (592,704 rows): the DIA kernels
-    run 2316 blocks, more than the 2048 of the ELL and CRS kernels, so the
-    <p,q> and <r,z> partials use the DIA grid.  Bitwise against the oracle."""
+def test_mixed_dia_grid_above_2048_vs_oracle(ctx, precond, side):
+    """Variable-coefficient diffusion on 84^3 (592,704 rows) and 104^3
+    (1,124,864 rows): the DIA kernels run 2316 blocks, more than the 2048 of
+    the ELL and CRS kernels, and 4096 blocks (the DIA cap; 4394 chunks), so
+    the <p,q> and <r,z> partials use the DIA grid, below and at its cap.
+    Bitwise against the oracle."""
     from cgxx_amd import spectrum
 
-    m = problems.diffusion3d(84, sigma=1.0, corr=8.0, seed=7)
+    print(f"DIA grid parity: {side}^3 {precond}", flush=True)
+    m = problems.diffusion3d(side, sigma=1.0, corr=8.0, seed=7)
     k = m.ones_rhs()
     C = 1.0 / m.diagonal()
     _setup(ctx, m.ptr, m.index, m.value, k, C)
