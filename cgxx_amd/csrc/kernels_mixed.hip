@@ -140,13 +140,14 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
 // stored products of row i (from 0, in stored order); returns the sum of
 // the epilogue's dot contributions.
 // Chunks of block blockIdx.x: rb0, rb0 + stride, ... below rend.  Round
-// robin (block g owns chunks g, g + G, ...: the order the dot trees and the
-// oracle assume), or, for kernels without a reduction, XCD-contiguous:
-// blocks are dispatched to the 8 XCDs round robin, so block g runs on XCD
-// g % 8, and that XCD's blocks walk its own contiguous eighth of the
-// chunks; the +-1, +-S and +-S^2 neighbours of a stencil row then mostly
-// hit the same L2 (3.5% faster Clenshaw steps at 256^3).  Needs G % 8 == 0
-// (else round robin).
+// robin (block g owns chunks g, g + G, ...), or XCD-contiguous: blocks are
+// dispatched to the 8 XCDs round robin, so block g runs on XCD g % 8, and
+// that XCD's blocks walk its own contiguous eighth of the chunks; the +-1,
+// +-S and +-S^2 neighbours of a stencil row then mostly hit the same L2
+// (3.5% faster Clenshaw steps at 256^3).  Needs G % 8 == 0 (else round
+// robin).  The DIA kernels walk XCD-contiguously, their dots included (the
+// oracle's orc_rowblock_dot_tree_walk restates that assignment); the ELL
+// and CRS kernels with a dot walk round robin.
 template <bool XCD>
 __device__ inline void chunk_walk(int nrb, int &rb0, int &stride, int &rend) {
   rb0 = blockIdx.x, stride = gridDim.x, rend = nrb;
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_mp_matvec_dot(Op32 A,
   __shared__ double lds[4];
   if (!mp_begin(st)) return;
   const int unit = K == kOpCrs ? A.crs.unit : 1;
-  const double tpart = op32_rows<K>(
+  const double tpart = op32_rows<K, false, K == kOpDia>(
       A, p, prod, [&](int i) { return p[i]; },
       [&](int i, float t, float pi) -> double {
         const float qi = unit ? pi + t : t;
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
   const float ck = poly->c[poly->degree];
-  const double tpart = op32_rows<K, K == kOpDia, !FINAL>(
+  const double tpart = op32_rows<K, K == kOpDia, K == kOpDia || !FINAL>(
       A, MODE == kStepFirst ? r : b1, prod,
       [&](int i) {
         const float ri = r[i];

@@ -395,13 +395,26 @@ double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
 
 double orc_rowblock_dot_tree_g(int nrb, const int *starts, const double *p,
                                const double *q, int gmax) {
+  return orc_rowblock_dot_tree_walk(nrb, starts, p, q, gmax, 0);
+}
+
+/* kernels_mixed.hip chunk_walk<XCD> */
+double orc_rowblock_dot_tree_walk(int nrb, const int *starts, const double *p,
+                                  const double *q, int gmax, int xcd) {
   if (nrb <= 0) return 0.0;
   int G = nrb < gmax ? nrb : gmax;
   double *partial = (double *)malloc(sizeof(double) * (size_t)G);
   double v[ORC_BLOCK];
   for (int g = 0; g < G; g++) {
+    int rb0 = g, stride = G, rend = nrb;
+    if (xcd && G % 8 == 0) {
+      const int x = g & 7, R = (nrb + 7) >> 3;
+      rb0 = x * R + (g >> 3);
+      stride = G >> 3;
+      rend = (x + 1) * R < nrb ? (x + 1) * R : nrb;
+    }
     for (int t = 0; t < ORC_BLOCK; t++) v[t] = 0.0;
-    for (int rb = g; rb < nrb; rb += G)
+    for (int rb = rb0; rb < rend; rb += stride)
       for (int t = 0; t < ORC_BLOCK; t++) {
         int r = starts[rb] + t;
         v[t] += r < starts[rb + 1] ? p[r] * q[r] : 0.0;
@@ -603,6 +616,7 @@ typedef struct {
  * y_i = scaled ? x_i + t : t. */
 typedef struct {
   int N, unit, nrb, gmax;
+  int xcd; /* DIA: the dot kernels walk row blocks XCD-contiguously */
   int *ptr, *index, *starts;
   float *value;
   float *value16; /* DIA: the values rounded to FP16 (Clenshaw steps), else NULL */
@@ -710,6 +724,7 @@ static int op32_build(const orc_problem *p, int scaled, op32 *o) {
         o->starts[b] = b * ORC_ROWS_MAX < N ? b * ORC_ROWS_MAX : N;
       if (dia) {
         o->gmax = ORC_DIA_GMAX;
+        o->xcd = 1;
         o->value16 = (float *)malloc(sizeof(float) * (size_t)(k > 0 ? k : 1));
         for (int e = 0; e < k; e++) o->value16[e] = orc_round_half(o->value[e]);
       }
@@ -846,7 +861,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
         f0[i] = (double)r32[i];                                               \
         f1[i] = (double)w[0][i];                                              \
       }                                                                       \
-      rz = orc_rowblock_dot_tree_g(o.nrb, o.starts, f0, f1, o.gmax);                    \
+      rz = orc_rowblock_dot_tree_walk(o.nrb, o.starts, f0, f1, o.gmax, o.xcd);                    \
     }                                                                         \
     if (first) {                                                              \
       nrm2_0 = sqrt(rt2);                                                     \
@@ -888,7 +903,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
     for (int i = 0; i < N; i++) q32[i] = op32_row(&o, p32, i);
     for (int i = 0; i < N; i++) f0[i] = (double)p32[i];
     for (int i = 0; i < N; i++) f1[i] = (double)q32[i];
-    double pq = orc_rowblock_dot_tree_g(o.nrb, o.starts, f0, f1, o.gmax);
+    double pq = orc_rowblock_dot_tree_walk(o.nrb, o.starts, f0, f1, o.gmax, o.xcd);
     double alpha = s.rho / pq;
     if (h_pq) h_pq[it] = pq;
     if (h_alpha) h_alpha[it] = alpha;
@@ -907,7 +922,7 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
         f0[i] = (double)r32[i];
         f1[i] = (double)w[0][i];
       }
-      rz = orc_rowblock_dot_tree_g(o.nrb, o.starts, f0, f1, o.gmax);
+      rz = orc_rowblock_dot_tree_walk(o.nrb, o.starts, f0, f1, o.gmax, o.xcd);
     }
     double resid = sqrt(rr) / inner0;
     double beta = rz / s.rho;

@@ -98,6 +98,13 @@ double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
 /* The same with at most gmax blocks (G = min(nrb, gmax)). */
 double orc_rowblock_dot_tree_g(int nrb, const int *starts, const double *p,
                                const double *q, int gmax);
+/* The same with xcd = 1: when G is a multiple of 8, block g walks the
+ * row blocks x R + (g >> 3), stepping by G / 8, below (x + 1) R, with
+ * x = g & 7 and R = ceil(nrb / 8) (the library's XCD-contiguous walk:
+ * each of the eight XCDs takes one contiguous eighth); otherwise, and with
+ * xcd = 0, block g takes g, g + G, ... */
+double orc_rowblock_dot_tree_walk(int nrb, const int *starts, const double *p,
+                                  const double *q, int gmax, int xcd);
 
 /* PCG (CG.cpp:284-385).
  * order 0: every dot left to right (SerialCG.cpp:145-155);
