@@ -91,6 +91,8 @@ def lib() -> C.CDLL:
         L.orc_crs_rowblocks.argtypes = [C.c_int, _i32p, _i32p]
         L.orc_rowblock_dot_tree.argtypes = [C.c_int, _i32p, _f64p, _f64p]
         L.orc_rowblock_dot_tree.restype = C.c_double
+        L.orc_rowblock_dot_tree_walk.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int, C.c_int]
+        L.orc_rowblock_dot_tree_walk.restype = C.c_double
         L.orc_pcg.argtypes = [C.POINTER(Problem), _f64p, _f64p, C.POINTER(Result)] + [C.c_void_p] * 5
         _lib = L
     return _lib

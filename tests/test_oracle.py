@@ -247,3 +247,25 @@ def test_fused_orders_agree_closely_with_serial(tmp_path):
     for order in (1, 2):
         assert abs(out[order][0] - out[0][0]) <= 1
         assert max_rel(out[order][1]["r2"], out[0][1]["r2"]) < 1e-6
+
+
+@pytest.mark.parametrize("xcd", [0, 1])
+@pytest.mark.parametrize("n,gmax", [(256 * 8192 + 77, 4096), (256 * 4094, 4096), (256 * 1000 + 5, 4096),
+                                    (256 * 13 + 200, 4096), (256 * 40, 16), (256 * 41 + 3, 24)])
+def test_rowblock_dot_tree_walk_covers_every_row_once(n, gmax, xcd):
+    """The DIA dot tree (256-row chunks, at most gmax blocks, round robin or
+    XCD-contiguous walk) adds every row exactly once: with integer products
+    below 2^53 every order gives the exact sum, and a weight of 2^k on
+    chunk k shows a missed or repeated chunk in the bits of the result."""
+    nrb = (n + 255) // 256
+    starts = np.minimum(np.arange(nrb + 1, dtype=np.int64) * 256, n).astype(np.int32)
+    rows = np.arange(n)
+    p = np.ones(n)
+    q = ((rows % 7) + 1).astype(np.float64)
+    want = float(q.sum())
+    got = oracle.lib().orc_rowblock_dot_tree_walk(nrb, starts, p, q, gmax, xcd)
+    assert got == want
+    if nrb <= 50:
+        q2 = np.ldexp(1.0, (rows // 256).astype(np.int32))
+        got2 = oracle.lib().orc_rowblock_dot_tree_walk(nrb, starts, p, q2, gmax, xcd)
+        assert got2 == float(q2.sum())
