@@ -2,8 +2,10 @@
 systems of the anisotropic diffusion family, measured on the GPU.
 
 For each held-out seed, five solves from x = 0 to FP64 true relative
-residual 1e-9, each timed after an untimed run of the same method on the
-same system:
+residual 1e-9.  Each method runs once untimed on the system, then twice
+timed, and the faster timed run counts: the next systems arrive from the
+generator processes on a background thread of this interpreter, which can
+delay a single run by tens of milliseconds.  The methods:
   learned  mixed precision, polynomial from the network (features and
            inference timed with the solve), Jacobi fallback
   fixed    mixed precision, Chebyshev on [theta_min(10) / rho*, b] (rho*
@@ -81,40 +83,48 @@ def main() -> int:
         ctx.upload("K", k)
         del ptr, idx, val, k, C
         r = {"seed": seed, "n": n}
-        # each method runs once untimed first, so every timed run finds its
-        # graphs and buffers for this system already built (as in bench.py)
-        learned.solve_learned(ctx, mlp, model, n)
-        lr = learned.solve_learned(ctx, mlp, model, n)
+
+        def best(run):
+            # one untimed run (graphs and buffers built, as in bench.py), then
+            # the faster of two timed runs
+            run()
+            a, b_ = run(), run()
+            return a if a["wall_s"] <= b_["wall_s"] else b_
+
+        lr = best(lambda: learned.solve_learned(ctx, mlp, model, n))
         r.update({f"learned_{key}": lr[key] for key in ("spmvs", "wall_s", "true_residual",
                                                            "poly_iterations", "jacobi_iterations")})
         r["learned_method"] = lr["method"]
-        f, b, theta10 = learned.system_features(ctx, n)
-        learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b)
-        t1 = time.perf_counter()
-        f, b, theta10 = learned.system_features(ctx, n)
-        fr = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b)
-        fr["wall_s"] = time.perf_counter() - t1
+
+        def chebyshev(rho_):
+            t1 = time.perf_counter()
+            f, b, theta10 = learned.system_features(ctx, n)
+            out = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho_), b)
+            out["wall_s"] = time.perf_counter() - t1
+            return out
+
+        fr = best(lambda: chebyshev(rho))
         r.update({f"fixed_{key}": fr[key] for key in ("spmvs", "wall_s", "true_residual",
                                                          "poly_iterations", "jacobi_iterations")})
         r["fixed_method"] = fr["method"]
-        learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, 1.0), b)
-        t1 = time.perf_counter()
-        f, b, theta10 = learned.system_features(ctx, n)
-        rr_ = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, 1.0), b)
-        rr_["wall_s"] = time.perf_counter() - t1
+        rr_ = best(lambda: chebyshev(1.0))
         r.update({f"ritz_{key}": rr_[key] for key in ("spmvs", "wall_s", "true_residual",
                                                          "poly_iterations", "jacobi_iterations")})
         r["ritz_method"] = rr_["method"]
-        ctx.fill("X", 0.0)
-        ctx.solve_mixed(precond="jacobi")
-        ctx.fill("X", 0.0)
-        jm = ctx.solve_mixed(precond="jacobi")
+
+        def mixed_jacobi():
+            ctx.fill("X", 0.0)
+            return ctx.solve_mixed(precond="jacobi")
+
+        jm = best(mixed_jacobi)
         r.update({"jacobi_spmvs": jm["iterations"], "jacobi_wall_s": jm["wall_s"],
                   "jacobi_true_residual": jm["true_residual"]})
-        ctx.fill("X", 0.0)
-        ctx.solve(precond="jacobi", max_iter=100000, tol=1e-9)
-        ctx.fill("X", 0.0)
-        f64 = ctx.solve(precond="jacobi", max_iter=100000, tol=1e-9)
+
+        def fp64():
+            ctx.fill("X", 0.0)
+            return ctx.solve(precond="jacobi", max_iter=100000, tol=1e-9)
+
+        f64 = best(fp64)
         nr, nb = ctx.true_residual("X", "K")
         r.update({"fp64_spmvs": f64["iterations"], "fp64_wall_s": f64["wall_s"],
                   "fp64_true_residual": nr / nb})
