@@ -280,7 +280,7 @@ class Context:
         self._chk(self.L.cghip_matrix_stats(self._h, out, out.size), "cghip_matrix_stats")
         return out
 
-    def solve_mixed(self, precond="jacobi", max_iter=100000, tol=1e-9, delta=0.1, batch=0,
+    def solve_mixed(self, precond="jacobi", max_iter=100000, tol=1e-9, delta=0.0, batch=0,
                     profile=False) -> dict:
         o = MixedOpts(PRECOND[precond], max_iter, tol, delta, batch, int(profile))
         info = MixedInfo()

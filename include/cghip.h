@@ -190,7 +190,8 @@ typedef struct {
   int precond;       /* CGHIP_PRECOND_* */
   int max_iter;      /* inner iterations, > 0 */
   double tol;        /* FP64 true relative residual */
-  double delta;      /* reliable-update factor in (0, 1); 0 = 0.1 */
+  double delta;      /* reliable-update factor in (0, 1); 0 = 0.01 with the
+                        polynomial, 0.1 with Jacobi or none */
   int batch;         /* inner iterations per graph launch; 0 = automatic */
   int profile;       /* CGHIP_PROFILE_* */
 } cghip_mixed_opts;

@@ -830,7 +830,9 @@ int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
     for (int j = 0; j < 3; j++) w[j] = (float *)calloc(nb, sizeof(float));
   const int deg = poly ? pl->degree : 0;
   const float ck = poly ? pl->c[deg] : 0.0f;
-  double delta2 = (delta > 0 ? delta : 0.1) * (delta > 0 ? delta : 0.1);
+  /* cghip_solve_mixed's default factor: 0.01 with the polynomial, else 0.1 */
+  const double dlt = delta > 0 ? delta : (poly ? 0.01 : 0.1);
+  double delta2 = dlt * dlt;
   double nrm2_0 = 0, inner0 = 0, true_res = 0;
   int updates = 0;
   mp_scalars s = {0, 0, 0, 0, 0, 0};

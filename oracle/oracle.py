@@ -228,7 +228,7 @@ def make_poly(coeffs, b):
 
 
 def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max_iter=100000,
-              delta=0.1, history=True, poly=None):
+              delta=0.0, history=True, poly=None):
     """The oracle's FP32-inner / FP64 reliable-update PCG (CRS).
 
     precond "jacobi" and "poly" run on the scaled operator
