@@ -321,7 +321,7 @@ def main() -> int:
                         "(log-normal kappa, sigma=1, corr=8, seed 20261202), k=A*1, x0=0, "
                         "FP64 true relative residual 1e-9",
             "method": f"mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 ({layout}, FP32 values), "
-                      "FP64 reliable updates (delta 0.1); the trained network gives the "
+                      "FP64 reliable updates (delta 0.01 with the polynomial, 0.1 with Jacobi); the trained network gives the "
                       "degree-8 polynomial's coefficients and predicted iteration counts, and the "
                       "polynomial runs only when its predicted cost beats Jacobi's (else Jacobi); "
                       "features, inference, choice and fallback inside the timed step",
