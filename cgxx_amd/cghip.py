@@ -24,6 +24,7 @@ SYMBOLS = [
     "cghip_create", "cghip_destroy", "cghip_last_error", "cghip_geometry",
     "cghip_set_size", "cghip_upload_crs", "cghip_upload_ell", "cghip_upload_jacobi",
     "cghip_set_format", "cghip_upload_vector", "cghip_download_vector", "cghip_set_dot_mode",
+    "cghip_vector_pointer",
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec", "cghip_time_hook",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
@@ -102,6 +103,7 @@ def load() -> C.CDLL:
     L.cghip_upload_vector.argtypes = [vp, C.c_int, _f64p]
     L.cghip_download_vector.argtypes = [vp, C.c_int, _f64p]
     L.cghip_set_dot_mode.argtypes = [vp, C.c_int]
+    L.cghip_vector_pointer.argtypes = [vp, C.c_int, C.POINTER(vp)]
     L.cghip_cpy.argtypes = [vp, C.c_int, C.c_int]
     L.cghip_matvec.argtypes = [vp, C.c_int, C.c_int]
     L.cghip_axpy.argtypes = [vp, C.c_double, C.c_int, C.c_int]
@@ -213,6 +215,12 @@ class Context:
         a = np.ascontiguousarray(a, np.float64)
         assert a.size == self.n
         self._chk(self.L.cghip_upload_vector(self._h, VEC[vec], a), "cghip_upload_vector")
+
+    def pointer(self, vec: str) -> int:
+        """Device address of a vector (cghip_vector_pointer)."""
+        out = C.c_void_p()
+        self._chk(self.L.cghip_vector_pointer(self._h, VEC[vec], C.byref(out)), "cghip_vector_pointer")
+        return out.value
 
     def download(self, vec: str) -> np.ndarray:
         out = np.empty(self.n, np.float64)

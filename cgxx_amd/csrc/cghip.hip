@@ -530,6 +530,14 @@ int cghip_download_vector(cghip_ctx *c, int id, double *host) {
   return sync(c);
 }
 
+int cghip_vector_pointer(cghip_ctx *c, int id, double **out) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (!valid_vec(id) || !out) return fail(c, CGHIP_ERR_ARG, "invalid vector id %d or NULL", id);
+  if (c->n <= 0) return fail(c, CGHIP_ERR_STATE, "no size set");
+  *out = c->vec[id];
+  return CGHIP_OK;
+}
+
 // ---- hooks ---------------------------------------------------------------
 
 int cghip_cpy(cghip_ctx *c, int dst, int src) {
@@ -629,6 +637,25 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
+// History buffers for a solve of at most max_iter iterations: min(max_iter
+// + 1, kHistCap) entries per array, in 64-bit arithmetic (max_iter may be
+// INT_MAX), reallocated only when a larger one is needed; the kernels write
+// entry i only while i < h->cap.
+int ensure_history(cghip_ctx *c, int max_iter, History *h) {
+  const int want = (int)std::min<int64_t>((int64_t)max_iter + 1, (int64_t)kHistCap);
+  if (c->hist_cap < want) {
+    drop_graph(c);
+    dfree(c->d_hist);
+    c->hist_cap = 0;
+    if (int rc = dalloc(c, &c->d_hist, 5 * (size_t)want)) return rc;
+    c->hist_cap = want;
+  }
+  const size_t hc = (size_t)c->hist_cap;
+  *h = History{c->d_hist,          c->d_hist + hc,     c->d_hist + 2 * hc,
+               c->d_hist + 3 * hc, c->d_hist + 4 * hc, c->hist_cap};
+  return CGHIP_OK;
+}
+
 }  // namespace
 
 int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info) {
@@ -643,15 +670,8 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
   HIPCHK(c, hipSetDevice(c->device));
   std::memset(info, 0, sizeof(*info));
 
-  if (c->hist_cap < o->max_iter + 1) {
-    drop_graph(c);
-    dfree(c->d_hist);
-    c->hist_cap = 0;
-    if (int rc = dalloc(c, &c->d_hist, 5 * ((size_t)o->max_iter + 1))) return rc;
-    c->hist_cap = o->max_iter + 1;
-  }
-  const size_t hc = (size_t)c->hist_cap;
-  History h{c->d_hist, c->d_hist + hc, c->d_hist + 2 * hc, c->d_hist + 3 * hc, c->d_hist + 4 * hc};
+  History h;
+  if (int rc = ensure_history(c, o->max_iter, &h)) return rc;
   const double *C = o->precond == CGHIP_PRECOND_JACOBI ? c->jac : nullptr;
 
   SolveState init{};
@@ -918,24 +938,6 @@ int cghip_set_poly(cghip_ctx *c, int degree, const double *coeffs, double b) {
   return CGHIP_OK;
 }
 
-namespace {
-
-int ensure_history(cghip_ctx *c, int max_iter, History *h) {
-  if (c->hist_cap < max_iter + 1) {
-    drop_graph(c);
-    dfree(c->d_hist);
-    c->hist_cap = 0;
-    if (int rc = dalloc(c, &c->d_hist, 5 * ((size_t)max_iter + 1))) return rc;
-    c->hist_cap = max_iter + 1;
-  }
-  const size_t hc = (size_t)c->hist_cap;
-  *h = History{c->d_hist, c->d_hist + hc, c->d_hist + 2 * hc, c->d_hist + 3 * hc,
-               c->d_hist + 4 * hc};
-  return CGHIP_OK;
-}
-
-}  // namespace
-
 int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info *info) {
   if (!c) return CGHIP_ERR_ARG;
   if (!o || !info) return fail(c, CGHIP_ERR_ARG, "opts/info is NULL");
@@ -1082,13 +1084,16 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
 
   const auto t0 = std::chrono::steady_clock::now();
   if (int rc = refresh(true)) return rc;
-  // Each pass either runs a batch (advancing the iteration count until
-  // max_iter stops it) or a reliable update (at most one per batch), so the
-  // loop ends; the bound below only guards against a broken state.
-  const long max_passes = 2 * ((long)o->max_iter / batch + 3) + 64;
-  for (long pass = 0;; pass++) {
+  // Each pass either runs a batch, which advances the iteration count by
+  // at least one (a reliable update can be requested after the batch's
+  // first iteration) until max_iter stops it, or a reliable update (at most
+  // one per batch), so the loop ends; the bound below, in iterations rather
+  // than batches, only guards against a broken state.
+  const int64_t max_passes = 2 * ((int64_t)o->max_iter + 3) + 64;
+  for (int64_t pass = 0;; pass++) {
     if (pass >= max_passes)
-      return fail(c, CGHIP_ERR_STATE, "mixed solver did not stop after %ld passes", pass);
+      return fail(c, CGHIP_ERR_STATE, "mixed solver did not stop after %lld passes",
+                  (long long)pass);
     const MixedState &s = *c->h_mstate;
     if (s.next.done == 1) break;  // true residual <= tol
     if (s.next.done == 2) {  // reliable update requested by the inner loop
@@ -1142,8 +1147,8 @@ int cghip_history(cghip_ctx *c, int which, double *out, int cap, int *written) {
   if (which < 0 || which > 4 || !out || cap < 0)
     return fail(c, CGHIP_ERR_ARG, "invalid history request");
   if (!c->d_hist) return fail(c, CGHIP_ERR_STATE, "no solve has run");
-  const int have = (which <= 1) ? c->last_iterations + 1 : c->last_iterations;
-  const int cnt = std::min(std::min(have, cap), c->hist_cap);
+  const int64_t have = (which <= 1) ? (int64_t)c->last_iterations + 1 : c->last_iterations;
+  const int cnt = (int)std::min<int64_t>(std::min<int64_t>(have, cap), c->hist_cap);
   if (cnt > 0)
     HIPCHK(c, hipMemcpy(out, c->d_hist + (size_t)which * c->hist_cap, sizeof(double) * (size_t)cnt,
                         hipMemcpyDeviceToHost));

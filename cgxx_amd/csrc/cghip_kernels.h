@@ -85,9 +85,14 @@ struct SolveState {
                  // read by the p update (which also does x += alpha p)
 };
 
-// Per-iteration history (nullable arrays of max_iter + 1 entries).
+// Per-iteration history: nullable arrays of `cap` entries each; entry i is
+// written only while i < cap, so a run longer than the buffer keeps its
+// first cap entries.  cap = min(max_iter + 1, kHistCap): the buffer stays
+// at most 5 x 8 MiB whatever CG_MAX_ITER is.
+constexpr int kHistCap = 1 << 20;
 struct History {
   double *r2, *rho, *pq, *alpha, *beta;
+  int cap;
 };
 
 // Per-block partial sums of the reductions.  A kernel writes its blocks'

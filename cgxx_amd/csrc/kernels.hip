@@ -292,8 +292,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_finish(Partials pa, int G, 
     st->next = s;
     st->cur = s;
     st->nrm2_0 = sqrt(r2);
-    if (h.r2) h.r2[0] = r2;
-    if (h.rho) h.rho[0] = s.rho;
+    if (h.r2 && h.cap > 0) h.r2[0] = r2;
+    if (h.rho && h.cap > 0) h.rho[0] = s.rho;
   }
 }
 
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *_
   const double a = c.rho / pq;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha = a;
-    if (h.pq) h.pq[c.iteration] = pq;
-    if (h.alpha) h.alpha[c.iteration] = a;
+    if (h.pq && c.iteration < h.cap) h.pq[c.iteration] = pq;
+    if (h.alpha && c.iteration < h.cap) h.alpha[c.iteration] = a;
   }
   const double na = -a;
   Pairs P(n);
@@ -384,14 +384,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
     s.r2 = r2;
     s.residual = res;
     s.iteration = it + 1;
-    if (h.r2) h.r2[it + 1] = r2;
+    if (h.r2 && it + 1 < h.cap) h.r2[it + 1] = r2;
+    // The reference computes rho and beta only when the loop goes on
+    // (CG.cpp:357-370): after the converging iteration their entries are NaN.
+    const double nan = __builtin_nan("");
+    if (h.rho && it + 1 < h.cap) h.rho[it + 1] = converged ? nan : rho;
+    if (h.beta && it < h.cap) h.beta[it] = converged ? nan : b;
     if (converged) {
       s.done = 1;
     } else {
       s.rho = rho;
       s.beta = b;
-      if (h.rho) h.rho[it + 1] = rho;
-      if (h.beta) h.beta[it] = b;
     }
     st->next = s;
   }

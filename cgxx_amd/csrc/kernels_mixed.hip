@@ -271,8 +271,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_fold(Partials prt, int gt
     st->next = s;
     st->cur = s;
     if (first) {
-      if (h.r2) h.r2[0] = rr;
-      if (h.rho) h.rho[0] = rz;
+      if (h.r2 && h.cap > 0) h.r2[0] = rr;
+      if (h.rho && h.cap > 0) h.rho[0] = rz;
     }
   }
 }
@@ -328,8 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__r
   const float na = (float)(-alpha);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha32 = (float)alpha;
-    if (h.pq) h.pq[c.iteration] = pq;
-    if (h.alpha) h.alpha[c.iteration] = alpha;
+    if (h.pq && c.iteration < h.cap) h.pq[c.iteration] = pq;
+    if (h.alpha && c.iteration < h.cap) h.alpha[c.iteration] = alpha;
   }
   Quads Q(n);
   double s_rr = 0.0;
@@ -411,9 +411,9 @@ __device__ inline void mp_advance(MixedState *st, const MixedScalars &c, double 
   s.residual = sqrt(rr) / st->inner0;
   if (s.residual <= st->tol || rr <= st->delta2 * c.r2_ref) s.done = 2;
   st->next = s;
-  if (h.r2) h.r2[it + 1] = rr;
-  if (h.rho) h.rho[it + 1] = rz;
-  if (h.beta) h.beta[it] = beta;
+  if (h.r2 && it + 1 < h.cap) h.r2[it + 1] = rr;
+  if (h.rho && it + 1 < h.cap) h.rho[it + 1] = rz;
+  if (h.beta && it < h.cap) h.beta[it] = beta;
 }
 
 // y32 += alpha p32 (alpha from the r update); beta = <r,r>_new / <r,r>;

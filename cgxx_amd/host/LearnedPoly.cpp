@@ -226,9 +226,15 @@ bool LearnedPoly::solve(int n, double tol, int maxIter, LearnedResult &res, std:
     if (cghip_solve_mixed(ctx_, &mo, &mi)) return fail("polynomial solve");
     res.polyIterations = mi.iterations;
     rel = mi.true_residual;
-    if (rel > tol) res.method = "poly+jacobi";
+    if (!(rel <= tol)) res.method = "poly+jacobi";
   }
-  if (rel > tol) {
+  // !(rel <= tol) also catches a NaN residual; a non-finite one leaves no
+  // usable X, so Jacobi then starts again from X = 0.
+  if (!(rel <= tol)) {
+    if (!std::isfinite(rel)) {
+      if (cghip_fill_vector(ctx_, CGHIP_VEC_X, 0.0)) return fail("fill");
+      rel = 1.0;
+    }
     cghip_mixed_opts mo{CGHIP_PRECOND_JACOBI, maxIter, tol / rel, 0.0, 0, CGHIP_PROFILE_NONE};
     mi = cghip_mixed_info{};
     if (cghip_solve_mixed(ctx_, &mo, &mi)) return fail("Jacobi solve");

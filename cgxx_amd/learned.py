@@ -294,9 +294,14 @@ def solve_poly_with_fallback(ctx, coeffs, b: float, tol: float = 1e-9, max_iter:
         res["poly_iterations"] = info["iterations"]
         res["updates"] += info["updates"]
         rel = info["true_residual"]
-    if rel > tol:
+    # `not rel <= tol` also catches a NaN residual; a non-finite one leaves
+    # no usable X, so Jacobi then starts again from X = 0.
+    if not rel <= tol:
         if ok:
             res["method"] = "poly+jacobi"
+        if not np.isfinite(rel):
+            ctx.fill("X", 0.0)
+            rel = 1.0
         info = ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=tol / rel)
         res["jacobi_iterations"] = info["iterations"]
         res["updates"] += info["updates"]

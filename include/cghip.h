@@ -93,6 +93,12 @@ int cghip_set_format(cghip_ctx *ctx, int format);
 int cghip_upload_vector(cghip_ctx *ctx, int id, const double *host);
 int cghip_download_vector(cghip_ctx *ctx, int id, double *host);
 int cghip_set_dot_mode(cghip_ctx *ctx, int mode);
+/* Device address of vector `id` (n doubles on the context's device), as
+ * CGCUDABase::Device::getVector maps a CG::Vector to its device array
+ * (cuda/CGCUDABase.h:74-91).  Valid until the next cghip_set_size with
+ * another n or cghip_destroy; work on it must be ordered with the context's
+ * calls by the caller. */
+int cghip_vector_pointer(cghip_ctx *ctx, int id, double **out);
 
 /* --- hooks (CG.h:224-237) ------------------------------------------------- */
 
@@ -217,7 +223,12 @@ int cghip_solve_mixed(cghip_ctx *ctx, const cghip_mixed_opts *opts,
 
 /* Per-iteration scalars of the last cghip_solve: which = 0 r2, 1 rho,
  * 2 dot_pq, 3 alpha, 4 beta.  r2/rho have iterations + 1 entries (index 0 is
- * before the loop), the others iterations.  Returns the count written. */
+ * before the loop), the others iterations.  When the loop stopped by
+ * convergence, rho[iterations] and beta[iterations - 1] are NaN: the
+ * reference computes them only when it goes on (CG.cpp:357-370).  At most
+ * the first 2^20 entries of each array are kept (the device buffer is
+ * min(max_iter + 1, 2^20) entries, whatever CG_MAX_ITER is).  Returns the
+ * count written. */
 int cghip_history(cghip_ctx *ctx, int which, double *out, int cap,
                   int *written);
 

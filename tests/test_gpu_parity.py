@@ -262,6 +262,35 @@ def test_fused_solve_is_deterministic(ctx, tmp_path):
     assert_bitwise(outs[0][2], outs[1][2], "r2")
 
 
+def test_huge_max_iter_and_history_after_convergence(ctx, tmp_path):
+    """CG_MAX_ITER = INT_MAX (accepted by CG.cpp:56): the history buffer stays
+    bounded and the solves match a run with a small limit bit for bit; after
+    the converging iteration rho and beta are NaN (the reference computes
+    them only when the loop goes on, CG.cpp:357-370)."""
+    fx = [f for f in load_golden() if f["case"] == "lap2d_64"][0]
+    case = Case(fx, str(tmp_path))
+    out = []
+    for maxit in (1000, 2**31 - 1):
+        _fresh(ctx, case, "CRS", "jacobi")
+        info = ctx.solve(precond="jacobi", max_iter=maxit)
+        it = info["iterations"]
+        out.append((it, ctx.download("X"), ctx.history("r2", it + 1), ctx.history("rho", it + 1),
+                    ctx.history("beta", it)))
+    assert out[0][0] == out[1][0] < 1000
+    for a, b in zip(out[0][1:], out[1][1:]):
+        assert_bitwise(a, b, "huge max_iter")
+    it, _, r2, rho, beta = out[1]
+    assert r2.size == it + 1 and np.all(np.isfinite(r2))
+    assert np.isnan(rho[it]) and np.all(np.isfinite(rho[:it]))
+    assert np.isnan(beta[it - 1]) and np.all(np.isfinite(beta[: it - 1]))
+    # the mixed solver with the same limit
+    ctx.prepare_fp32()
+    ctx.fill("X", 0.0)
+    m = ctx.solve_mixed(precond="jacobi", max_iter=2**31 - 1, tol=1e-9)
+    assert m["true_residual"] <= 1e-9
+    assert ctx.history("r2", m["iterations"] + 1).size == m["iterations"] + 1
+
+
 def test_fused_profile_and_batches_agree(ctx, tmp_path):
     fx = [f for f in load_golden() if f["case"] == "lap2d_64"][0]
     case = Case(fx, str(tmp_path))
