@@ -796,6 +796,27 @@ int orc_poly_apply(int N, const int *ptr, const int *index,
   return 0;
 }
 
+/* y = A_hat x in FP32 on the scaled operator (built as in orc_mixed_pcg_poly,
+ * implicit unit diagonal): the CG operator with half = 0 (FP32 values), the
+ * Clenshaw steps' operator with half = 1 (the FP16-rounded values when the
+ * library stores A_hat in DIA; returns 1 in *is_dia then, else 0 and the
+ * FP32 values are used).  Returns 0, or -3 without a positive diagonal. */
+int orc_op32_apply(int N, const int *ptr, const int *index, const double *value,
+                   const float *x, int half, float *y, int *is_dia) {
+  orc_problem p;
+  memset(&p, 0, sizeof(p));
+  p.N = N;
+  p.ptr = ptr, p.index = index, p.value = value;
+  op32 o;
+  int rc = op32_build(&p, 1, &o);
+  if (!rc) {
+    for (int i = 0; i < N; i++) y[i] = half ? op32_row_poly(&o, x, i) : op32_row(&o, x, i);
+    if (is_dia) *is_dia = o.value16 != NULL;
+  }
+  op32_free(&o);
+  return rc;
+}
+
 int orc_mixed_pcg(const orc_problem *p, const double *k, double *x,
                   double delta, orc_mixed_result *res, double *h_r2,
                   double *h_rho, double *h_pq, double *h_alpha,

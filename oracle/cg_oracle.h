@@ -206,6 +206,12 @@ int orc_poly_apply(int N, const int *ptr, const int *index,
                    const double *value, const float *r32, const orc_poly *pl,
                    float *z);
 
+/* y = A_hat x in FP32 on the scaled operator; half = 1: the FP16-rounded
+ * values of the Clenshaw steps when the library would store A_hat in DIA
+ * (*is_dia = 1), else the FP32 values. */
+int orc_op32_apply(int N, const int *ptr, const int *index, const double *value,
+                   const float *x, int half, float *y, int *is_dia);
+
 int orc_mixed_pcg_poly(const orc_problem *p, const orc_poly *pl,
                        const double *k, double *x, double delta,
                        orc_mixed_result *res, double *h_r2, double *h_rho,

@@ -290,3 +290,24 @@ def poly_apply(crs, r32, coeffs, b):
     if rc:
         raise ValueError(f"orc_poly_apply failed with code {rc}")
     return z
+
+
+def op32_apply(crs, x32, half=False):
+    """(y, is_dia): y = A_hat x in FP32 on the scaled operator
+    D^-1/2 A D^-1/2 (unit diagonal implicit), with the FP32 values or, with
+    half, the FP16-rounded ones the Clenshaw steps read when the library
+    stores A_hat in DIA."""
+    L = lib()
+    ptr = np.ascontiguousarray(crs[0], dtype=np.int32)
+    idx = np.ascontiguousarray(crs[1], dtype=np.int32)
+    val = np.ascontiguousarray(crs[2], dtype=np.float64)
+    n = ptr.size - 1
+    x = np.ascontiguousarray(x32, dtype=np.float32)
+    y = np.zeros(n, dtype=np.float32)
+    dia = C.c_int(0)
+    L.orc_op32_apply.argtypes = [C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p, C.c_void_p]
+    rc = L.orc_op32_apply(n, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, x.ctypes.data,
+                          int(half), y.ctypes.data, C.byref(dia))
+    if rc:
+        raise ValueError(f"orc_op32_apply failed with code {rc}")
+    return y, bool(dia.value)
