@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["cghip.hip", "kernels.hip", "kernels_mixed.hip", "cghip_mlp.hip", "kernels_mlp.hip",
-               "kernels_features.hip"]
+               "kernels_features.hip", "noalloc.hip"]
 HOST_SOURCES = ["CG.cpp", "Matrix.cpp", "Preconditioner.cpp", "CGHIP.cpp", "LearnedPoly.cpp", "main.cpp"]
 
 

@@ -24,7 +24,7 @@ SYMBOLS = [
     "cghip_create", "cghip_destroy", "cghip_last_error", "cghip_geometry",
     "cghip_set_size", "cghip_upload_crs", "cghip_upload_ell", "cghip_upload_jacobi",
     "cghip_set_format", "cghip_upload_vector", "cghip_download_vector", "cghip_set_dot_mode",
-    "cghip_vector_pointer",
+    "cghip_vector_pointer", "cghip_dot_noalloc",
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec", "cghip_time_hook",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
@@ -104,6 +104,7 @@ def load() -> C.CDLL:
     L.cghip_download_vector.argtypes = [vp, C.c_int, _f64p]
     L.cghip_set_dot_mode.argtypes = [vp, C.c_int]
     L.cghip_vector_pointer.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.cghip_dot_noalloc.argtypes = [vp, vp, C.c_int, vp, vp]
     L.cghip_cpy.argtypes = [vp, C.c_int, C.c_int]
     L.cghip_matvec.argtypes = [vp, C.c_int, C.c_int]
     L.cghip_axpy.argtypes = [vp, C.c_double, C.c_int, C.c_int]

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -111,6 +112,7 @@ struct cghip_ctx {
 namespace {
 
 std::string g_create_err = "no error";
+std::mutex g_create_mu;
 
 int fail(cghip_ctx *c, int code, const char *fmt, ...) {
   char buf[512];
@@ -272,6 +274,11 @@ int sync(cghip_ctx *c) {
 
 }  // namespace
 
+void cghip::set_global_error(const char *msg) {
+  std::lock_guard<std::mutex> lock(g_create_mu);
+  g_create_err = msg;
+}
+
 extern "C" {
 
 int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap, int *spmv_gmax,
@@ -306,6 +313,8 @@ int cghip_create(int device, cghip_ctx **out) {
   int rc = CGHIP_OK;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     rc = fail(nullptr, CGHIP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  if (!rc && (e = noalloc_prepare(device)) != hipSuccess)
+    rc = fail(nullptr, CGHIP_ERR_HIP, "scratch lookup: %s", hipGetErrorString(e));
   if (!rc) rc = dalloc(c, &c->partial, 4 * (size_t)kPartialSlot);
   if (!rc) rc = dalloc(c, &c->d_poly, 1);
   if (!rc) rc = dalloc(c, &c->dot_out, 8);

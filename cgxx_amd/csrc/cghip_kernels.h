@@ -125,6 +125,12 @@ struct Ell {
 // longer than kNnzCap is then streamed through LDS in chunks).
 int crs_rowblocks(int n, const int *ptr, int *starts);
 
+// Message returned by cghip_last_error(NULL) (errors without a context).
+void set_global_error(const char *msg);
+// Looks up the allocation-free entry point's static scratch array on
+// `device` (the current device); cghip_create calls it once per context.
+hipError_t noalloc_prepare(int device);
+
 // ---- launches (all on `s`; each returns the launch status) ----------------
 hipError_t matvec_crs(const Crs<double> &A, const double *x, double *y, hipStream_t s);
 hipError_t matvec_ell(const Ell<double> &A, const double *x, double *y, hipStream_t s);

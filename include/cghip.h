@@ -109,6 +109,22 @@ int cghip_xpay(cghip_ctx *ctx, int x, double a, int y);        /* CG.h:230 y = x
 int cghip_vector_dot(cghip_ctx *ctx, int a, int b, double *out); /* CG.h:232 */
 int cghip_apply_preconditioner(cghip_ctx *ctx, int x, int y);  /* CG.h:235 y = C o x */
 
+/* Allocation-free dot product.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  *out = <a, b> over n
+ * doubles, with the kernels and summation order of cghip_vector_dot in
+ * CGHIP_DOT_TREE mode (the same result bit for bit); a, b and out are device
+ * pointers on the current device; enqueued on `stream` (a hipStream_t; NULL
+ * = the null stream), asynchronously.  Calls no hipMalloc, hipFree or
+ * allocating library routine: the per-block partials (at most 2048 doubles,
+ * 16 KiB) live in a static device array that each GPU holds until the
+ * process ends, looked up per device by cghip_create or by the first call.
+ * Every partial a call folds is written by that call, so nothing needs
+ * clearing, but calls that overlap on one GPU race on the array.  Errors
+ * (n < 0, NULL pointers) are reported by cghip_last_error(NULL). */
+int cghip_dot_noalloc(const double *a, const double *b, int n, double *out,
+                      void *stream);
+
 /* --- fused solver ----------------------------------------------------------
  * The whole loop of CG::solve (CG.cpp:284-385) on the device: three kernels
  * per iteration (SpMV + <p,q>; r update + <r,r> + <r,z>; x += a p and

@@ -42,3 +42,14 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         assert "no CPU fallback" in str(e)
     else:
         raise AssertionError("load() must raise without the library")
+
+
+def test_dot_noalloc_rejects_bad_arguments_before_any_launch():
+    """The allocation-free dot refuses a negative size or NULL pointers on
+    the host, with a message for cghip_last_error(NULL), before any HIP
+    call (so this runs without a GPU)."""
+    L = cghip.load()
+    assert L.cghip_dot_noalloc(None, None, -5, None, None) == 1
+    assert b"negative" in L.cghip_last_error(None)
+    assert L.cghip_dot_noalloc(None, None, 10, None, None) == 1
+    assert b"NULL" in L.cghip_last_error(None)
