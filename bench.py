@@ -122,10 +122,14 @@ def main() -> int:
                     help="short run for rocprofv3 --pmc: one D2D copy of a vector (the "
                          "calibration kernel of known size), 20 FP64 CRS, 20 FP64 ELL, 20 mixed "
                          "Jacobi and 5 mixed polynomial iterations")
-    ap.add_argument("--workload", choices=("diffusion256", "poisson128"), default="diffusion256",
+    ap.add_argument("--workload", choices=("diffusion256", "poisson128", "executables"),
+                    default="diffusion256",
                     help="poisson128: the plain FP64 path on the 128^3 7-point Poisson system "
                          "(CRS and ELL, per-hook GB/s, the CPU ports' full solves beside it); "
-                         "a report of its own, not the benchmark line")
+                         "executables: the cg_hip executable on the 256^2 Laplacian and the 128^3 "
+                         "Poisson file (loops, COO, the reader's IO time) with the oracle's ports "
+                         "of the reference's CPU solvers and reader beside it; "
+                         "reports of their own, not the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -133,6 +137,8 @@ def main() -> int:
         return 2
     if args.workload == "poisson128":
         return poisson128(args)
+    if args.workload == "executables":
+        return executables(args)
 
     import numpy as np
 
@@ -446,6 +452,115 @@ def poisson128(args) -> int:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     return 0
+
+
+def executables(args) -> int:
+    """The cg_hip executable through its command line, with the oracle's
+    ports of the reference's CPU paths beside it on this host, on two files
+    written by cgxx_amd.problems.write_mm: the 256^2 5-point Laplacian (C1)
+    and the 128^3 7-point Poisson system.  C1: cg_hip's three loops (hook by
+    hook, fused with per-iteration kernel timing, fused from HIP graphs) and
+    full solves of the SerialCG port (1 core) and the CGOpenMP port (every
+    core of this host) from the file read by the oracle's reader.  128^3:
+    cg_hip in COO, CRS and ELL and the SerialCG port's COO solve; the
+    reader's IO time at 1, 4 and all host threads and with its binary cache,
+    beside the oracle's port of the reference's reader (one thread, then the
+    CRS conversion).  3 repetitions of each cg_hip run; one JSON object."""
+    import re
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from cgxx_amd import problems
+    from oracle import oracle
+
+    exe = os.path.join(REPO, "cgxx_amd", "bin", "cg_hip")
+    cores = len(os.sched_getaffinity(0))
+    out = {"host": host_info(), "cores_used_openmp": cores}
+
+    def cli(path, reps=3, **env):
+        e = dict(os.environ, **{k: str(v) for k, v in env.items()})
+        r = subprocess.run([exe, path, str(reps)], capture_output=True, text=True, env=e, timeout=600)
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        get = lambda label: [ln.split(":", 1)[1].strip() for ln in r.stdout.splitlines()
+                             if ln.startswith(label + ":")]
+        return {"iterations": [int(v) for v in get("Iterations")], "residual": get("Residual"),
+                "solve_s": [float(v) for v in get("Solve time")],
+                "io_s": [float(v) for v in get("IO time")],
+                "converting_s": [float(v) for v in get("Converting time")],
+                "matvec_gflops": [float(v) for v in get("MatVec GFLOP/s")],
+                "correct": r.stdout.count("Solution is correct!")}
+
+    def port(mm, fmt, order, threads, k, c, max_iter):
+        t0 = time.perf_counter()
+        kw = {"mm": mm} if fmt == "COO" else {"crs": mm.crs()}
+        x, it, res, _ = oracle.pcg(n=mm.N, fmt=fmt, precond="jacobi", order=order, nthreads=threads,
+                                   k=k, tol=1e-9, max_iter=max_iter, C_jac=c, history=False, **kw)
+        return {"seconds": time.perf_counter() - t0, "iterations": it, "residual": "%e" % res,
+                "correct": bool(np.all(np.abs(x - 1.0) <= 1e-5)), "cores": threads, "kind": "port"}
+
+    with tempfile.TemporaryDirectory() as tmp:
+        c1 = os.path.join(tmp, "c1.mtx")
+        problems.write_mm(c1, problems.laplace2d(256))
+        env = dict(CG_MATRIX_FORMAT="CRS", CG_PRECONDITIONER="jacobi", CG_MAX_ITER=1000)
+        r = {"hooks": cli(c1, CG_HIP_FUSED=0, **env), "fused_timed": cli(c1, **env),
+             "fused_graph": cli(c1, CG_HIP_GRAPH=1, **env)}
+        mm = oracle.MMFile(c1)
+        k, (c, _) = mm.rhs_ones(), mm.jacobi()
+        r["serial_port"] = [port(mm, "CRS", 0, 1, k, c, 1000) for _ in range(3)]
+        r["openmp_port"] = [port(mm, "CRS", 3, cores, k, c, 1000) for _ in range(3)]
+        out["c1_lap2d_256"] = r
+        log("c1: " + ", ".join(f"{key} {min(v['solve_s']) if isinstance(v, dict) else min(x['seconds'] for x in v):.4f} s"
+                               for key, v in r.items()))
+        del mm
+
+        c2 = os.path.join(tmp, "c2.mtx")
+        problems.write_mm(c2, problems.laplace3d(128))
+        env = dict(CG_PRECONDITIONER="jacobi", CG_MAX_ITER=1000)
+        r = {fmt: cli(c2, CG_MATRIX_FORMAT=fmt, **env) for fmt in ("COO", "CRS", "ELL")}
+        t0 = time.perf_counter()
+        mm = oracle.MMFile(c2)
+        t_read = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        crs = mm.crs()
+        t_conv = time.perf_counter() - t0
+        k, (c, _) = mm.rhs_ones(), mm.jacobi()
+        r["serial_port_coo"] = port(mm, "COO", 0, 1, k, c, 1000)
+        out["c2_poisson128"] = r
+        io = {f"threads_{t}": cli(c2, reps=1, CG_IO_THREADS=t, CG_MAX_ITER=1,
+                                  CG_PRECONDITIONER="jacobi") for t in (1, 4, cores)}
+        cache = os.path.join(tmp, "c2.cache")
+        io["cache_written"] = cli(c2, reps=1, CG_MATRIX_CACHE=cache, CG_MAX_ITER=1,
+                                  CG_PRECONDITIONER="jacobi")
+        io["cache_loaded"] = cli(c2, reps=1, CG_MATRIX_CACHE=cache, CG_MAX_ITER=1,
+                                 CG_PRECONDITIONER="jacobi")
+        io["oracle_reader_port"] = {"read_s": t_read, "crs_conversion_s": t_conv,
+                                    "entries": int(crs[0][-1]), "cores": 1, "kind": "port"}
+        out["reader_c2"] = io
+        log("c2: " + ", ".join(f"{fmt} {min(r[fmt]['solve_s']):.4f} s" for fmt in ("COO", "CRS", "ELL"))
+            + f", serial port COO {r['serial_port_coo']['seconds']:.2f} s; reader port {t_read:.3f} s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def host_info() -> dict:
+    """CPU model, sockets and logical CPUs of this host (lscpu), and the CPUs
+    this process may run on."""
+    import subprocess
+
+    info = {"nproc": len(os.sched_getaffinity(0))}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in txt.splitlines():
+            key, _, val = ln.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "CPU(s)", "Thread(s) per core",
+                               "Core(s) per socket"):
+                info[key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
 
 
 if __name__ == "__main__":

@@ -17,6 +17,7 @@
 #include <iomanip>
 #include <iostream>
 #include <sstream>
+#include <vector>
 
 namespace {
 
@@ -29,6 +30,20 @@ const char *kWorkDistribution = "CG_WORK_DISTRIBUTION";
 const char *kOverlappedGather = "CG_OVERLAPPED_GATHER";
 // Additions of this backend, no reference counterpart.
 const char *kMatrixCache = "CG_MATRIX_CACHE";
+const char *kRhs = "CG_RHS";
+
+// Uniform double in [0, 1) from SplitMix64 (Steele, Lea & Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014): the output for
+// state seed after i + 1 steps, so entry i depends on (seed, i) only and
+// rows can be filled in any order or in parallel.  The Python mirror is
+// cgxx_amd.problems.splitmix_uniform.
+double splitmixUniform(unsigned long long seed, unsigned long long i) {
+  unsigned long long z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
 
 const char *env(const char *name) {
   const char *v = std::getenv(name);
@@ -110,6 +125,20 @@ void CG::parseEnvironment() {
       fail("No support for overlapped gather!");
   }
   if (const char *v = env(kMatrixCache)) matrixCache = v;
+  if (const char *v = env(kRhs)) {
+    const std::string l = transformed(v, ::tolower);
+    if (l == "ones") {
+      rhsSeed = -1;
+    } else if (l.rfind("random:", 0) == 0 && l.size() > 7) {
+      errno = 0;
+      const long long seed = std::strtoll(l.c_str() + 7, &end, 10);
+      if (errno != 0 || *end != 0 || seed < 0)
+        fail(std::string("Invalid value for ") + kRhs + "! (ones, or random:<seed>)");
+      rhsSeed = seed;
+    } else {
+      fail(std::string("Invalid value for ") + kRhs + "! (ones, or random:<seed>)");
+    }
+  }
   hostThreads();  // validates CG_IO_THREADS before any work
 }
 
@@ -158,11 +187,16 @@ void CG::init(const char *matrixFile) {
   timing.converting = now() - startConverting;
   timing.io = now() - startIO;
 
-  // k = A * (1, ..., 1)^T, row sums accumulated in COO order (CG.cpp:256-261)
   allocateK();
-  std::memset(k, 0, sizeof(floatType) * (size_t)N);
-  const MatrixCOO &coo = *matrixCOO;
-  coo.forEachEntryByRow(hostThreads(), [&](int e) { k[coo.I[e]] += coo.V[e]; });
+  if (rhsSeed >= 0) {
+    // CG_RHS=random:<seed>: k_i = uniform(-1, 1), the solution unknown
+    for (int i = 0; i < N; i++) k[i] = 2.0 * splitmixUniform((unsigned long long)rhsSeed, i) - 1.0;
+  } else {
+    // k = A * (1, ..., 1)^T, row sums accumulated in COO order (CG.cpp:256-261)
+    std::memset(k, 0, sizeof(floatType) * (size_t)N);
+    const MatrixCOO &coo = *matrixCOO;
+    coo.forEachEntryByRow(hostThreads(), [&](int e) { k[coo.I[e]] += coo.V[e]; });
+  }
 
   allocateX();
   resetX();
@@ -229,10 +263,12 @@ void CG::solve() {
   std::cout << "Solving..." << std::endl;
   const time_point start = now();
 
+  residualHistory.clear();
   FusedResult fused;
   if (solveFused(maxIterations, tolerance, fused)) {
     iteration = fused.iterations;
     residual = fused.residual;
+    residualHistory = std::move(fused.residuals);
     timing.matvec += Timing::duration(fused.matvec);
     timing.axpy += Timing::duration(fused.axpy);
     timing.xpay += Timing::duration(fused.xpay);
@@ -254,6 +290,7 @@ void CG::solve() {
   floatType r2 = vectorDot(VectorR, VectorR);
   const floatType nrm2_0 = std::sqrt(r2);
   floatType rho = plain ? r2 : vectorDot(VectorP, VectorR);
+  residualHistory.push_back(1.0);
 
   for (iteration = 0; iteration < maxIterations; iteration++) {
     matvec(VectorP, VectorQ);                        // q = A p
@@ -262,6 +299,7 @@ void CG::solve() {
     axpy(-a, VectorQ, VectorR);                      // r -= a q
     r2 = vectorDot(VectorR, VectorR);
     residual = std::sqrt(r2) / nrm2_0;
+    residualHistory.push_back(residual);
     if (residual <= tolerance) {
       iteration++;  // this iteration counts
       break;
@@ -279,9 +317,57 @@ void CG::solve() {
   timing.solve = now() - start;
 }
 
+double CG::matvecBytes() const {
+  const double n = N, z = nz;
+  switch (matrixFormat) {
+  case MatrixFormatCRS: return 12.0 * z + 4.0 * (n + 1) + 16.0 * n;
+  case MatrixFormatELL: return 12.0 * z + 4.0 * n + 16.0 * n;
+  default: return 16.0 * z + 16.0 * n;  // COO: I, J, V, x, y
+  }
+}
+
+// ||k - A x|| / ||k|| in FP64 with the host's copy of the matrix (x0 = 0, so
+// ||k|| = ||r_0||).
+floatType CG::hostTrueResidual() const {
+  std::vector<floatType> y((size_t)N, 0.0);
+  if (matrixFormat == MatrixFormatCRS) {
+    for (int i = 0; i < N; i++)
+      for (int e = matrixCRS->ptr[i]; e < matrixCRS->ptr[i + 1]; e++)
+        y[i] += matrixCRS->value[e] * x[matrixCRS->index[e]];
+  } else if (matrixFormat == MatrixFormatELL) {
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < matrixELL->length[i]; j++) {
+        const long long s = (long long)j * N + i;
+        y[i] += matrixELL->data[s] * x[matrixELL->index[s]];
+      }
+  } else {
+    for (int e = 0; e < nz; e++) y[matrixCOO->I[e]] += matrixCOO->V[e] * x[matrixCOO->J[e]];
+  }
+  double rr = 0, kk = 0;
+  for (int i = 0; i < N; i++) {
+    const double r = k[i] - y[i];
+    rr += r * r;
+    kk += k[i] * k[i];
+  }
+  return kk > 0 ? std::sqrt(rr / kk) : std::sqrt(rr);
+}
+
 bool CG::check() {
   std::cout << "Checking solution..." << std::endl;
   const time_point start = now();
+  if (rhsSeed >= 0) {
+    // CG_RHS=random: the solution is not known; the FP64 true relative
+    // residual is judged against CG_CHECK_TOLERANCE instead.
+    trueResidual = hostTrueResidual();
+    timing.check = now() - start;
+    const bool ok = trueResidual <= checkTolerance;
+    checkErrors = ok ? 0 : 1;
+    std::ostringstream os;
+    os << std::scientific << trueResidual;
+    std::cout << "True relative residual: " << os.str() << std::endl;
+    std::cout << (ok ? "Solution is correct!" : "Solution is incorrect!") << std::endl;
+    return ok;
+  }
   int errors = 0;
   for (int i = 0; i < N; i++)
     if (std::abs(x[i] - 1.0) > checkTolerance) errors++;
@@ -395,7 +481,18 @@ bool CG::writeRecord(const char *path, int repetition) {
   jsonNumber(os, residual);
   os << ", \"check_tolerance\": ";
   jsonNumber(os, checkTolerance);
-  os << ", \"incorrect_elements\": " << checkErrors << ", \"times\": {";
+  os << ", \"incorrect_elements\": " << checkErrors << ", \"rhs\": \""
+     << (rhsSeed >= 0 ? "random:" + std::to_string(rhsSeed) : std::string("ones")) << "\"";
+  if (rhsSeed >= 0) {
+    os << ", \"true_relative_residual\": ";
+    jsonNumber(os, trueResidual);
+  }
+  os << ", \"residual_history\": [";
+  for (size_t i = 0; i < residualHistory.size(); i++) {
+    if (i) os << ", ";
+    jsonNumber(os, residualHistory[i]);
+  }
+  os << "], \"times\": {";
   const std::pair<const char *, double> times[] = {
       {"io", timing.io.count()},
       {"converting", timing.converting.count()},
@@ -417,6 +514,18 @@ bool CG::writeRecord(const char *path, int repetition) {
   }
   os << "}, \"matvec_gflops\": ";
   jsonNumber(os, 2.0 * (iteration + 1) * nz / 1e9 / timing.matvec.count());
+  // algorithmic bytes of one SpMV, and their rate over the solve's SpMVs
+  // (the initial A x0 included, as in the GFLOP/s figure)
+  os << ", \"matvec_bytes\": ";
+  jsonNumber(os, matvecBytes());
+  os << ", \"matvec_gbps\": ";
+  jsonNumber(os, matvecBytes() * (iteration + 1) / 1e9 / timing.matvec.count());
+  if (peakGBs() > 0) {
+    os << ", \"peak_gbps\": ";
+    jsonNumber(os, peakGBs());
+    os << ", \"matvec_peak_fraction\": ";
+    jsonNumber(os, matvecBytes() * (iteration + 1) / 1e9 / timing.matvec.count() / peakGBs());
+  }
   os << ", \"backend\": {";
   recordBackend(os);
   os << "}}\n";

@@ -23,6 +23,7 @@
 #include <memory>
 #include <ostream>
 #include <string>
+#include <vector>
 
 #include "Matrix.h"
 #include "Preconditioner.h"
@@ -51,9 +52,19 @@ private:
   Timing timing;
   std::string matrixFileName;
   int checkErrors = -1;  // elements the last check() found incorrect
+  // CG_RHS (no reference counterpart): -1 for the reference's k = A * 1,
+  // else the seed of k_i = uniform(-1, 1) (randomRhs); check() then judges
+  // the FP64 true relative residual instead of x = 1.
+  long long rhsSeed = -1;
+  floatType trueResidual = -1;  // ||k - A x|| / ||k|| of the last check (random k)
+  // Relative residual ||r_i|| / ||r_0|| before the loop and after each
+  // iteration of the last solve (the loop's own recursion).
+  std::vector<double> residualHistory;
 
   using time_point = Timing::clock::time_point;
   time_point now() const { return Timing::clock::now(); }
+
+  floatType hostTrueResidual() const;
 
   // Wall-clock wrappers of the hooks (the hooks are synchronous).
   void matvec(Vector in, Vector out);
@@ -118,6 +129,9 @@ protected:
     int iterations = 0;
     floatType residual = 0;
     double matvec = 0, axpy = 0, xpay = 0, vectorDot = 0, preconditioner = 0;
+    // ||r_i|| / ||r_0|| of the loop's recursion, i = 0 .. iterations (may
+    // be left empty or shorter)
+    std::vector<double> residuals;
   };
   virtual bool solveFused(int maxIterations, floatType tolerance, FusedResult &result) {
     (void)maxIterations;
@@ -131,6 +145,14 @@ protected:
   // Members of the backend's object in the solve record ("backend": {...}),
   // written without the braces; the default writes "name" only.
   virtual void recordBackend(std::ostream &os);
+  // Algorithmic bytes of one SpMV in the format the backend runs: values
+  // and indices once, x and y once (CRS also the row pointer, ELL the row
+  // lengths, COO the row indices).
+  virtual double matvecBytes() const;
+  // Peak memory bandwidth (GB/s) of the device the SpMV runs on; > 0 adds
+  // the SpMV's fraction of it to the solve record.
+  virtual double peakGBs() const { return 0; }
+  MatrixFormat getMatrixFormat() const { return matrixFormat; }
 
 public:
   virtual ~CG() {}

@@ -90,6 +90,21 @@ class CGHIP : public CG {
 
   bool solveMixed(int maxIterations, floatType tolerance, FusedResult &result);
   void recordBackend(std::ostream &os) override;
+  // COO runs on the device CRS; MI355X HBM3E peak 8 TB/s.
+  double matvecBytes() const override {
+    const double n = N, z = nz;
+    return getMatrixFormat() == MatrixFormatELL ? 12.0 * z + 20.0 * n
+                                                : 12.0 * z + 4.0 * (n + 1) + 16.0 * n;
+  }
+  double peakGBs() const override { return 8000.0; }
+  // ||r_i|| / ||r_0|| from the device's r2 history of the last solve.
+  void residualHistory(int iterations, std::vector<double> &out) {
+    std::vector<double> r2((size_t)iterations + 1);
+    int got = 0;
+    chk(cghip_history(ctx, 0, r2.data(), iterations + 1, &got), "history");
+    out.resize((size_t)got);
+    for (int i = 0; i < got; i++) out[i] = std::sqrt(r2[i] / r2[0]);
+  }
 
   void chk(int rc, const char *what) {
     if (rc != CGHIP_OK) hip_fail(ctx, what);
@@ -265,6 +280,7 @@ bool CGHIP::solveMixed(int maxIterations, floatType tolerance, FusedResult &resu
   lastMixed = info;
   result.iterations = info.iterations;
   result.residual = info.true_residual;
+  residualHistory(info.iterations, result.residuals);  // the inner recursion
   result.matvec = (info.ms_matvec + info.ms_refresh) * 1e-3;
   result.axpy = info.ms_update * 1e-3;
   result.xpay = info.ms_xpay * 1e-3;
@@ -288,6 +304,7 @@ bool CGHIP::solveFused(int maxIterations, floatType tolerance, FusedResult &resu
   chk(cghip_solve(ctx, &o, &info), "solve");
   result.iterations = info.iterations;
   result.residual = info.residual;
+  residualHistory(info.iterations, result.residuals);
   result.matvec = (info.ms_matvec + info.ms_init) * 1e-3;
   result.axpy = info.ms_update * 1e-3;
   result.xpay = info.ms_xpay * 1e-3;

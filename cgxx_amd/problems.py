@@ -262,6 +262,25 @@ def anisotropic_family(seed: int, sides=(64, 80, 96, 112, 128, 144, 160)) -> CRS
     return m
 
 
+def splitmix_uniform(n: int, seed: int) -> np.ndarray:
+    """Uniform doubles in [0, 1): entry i is SplitMix64's output for state
+    `seed` after i + 1 steps (Steele, Lea & Flood, OOPSLA 2014), top 53 bits.
+    The generator of the CG_RHS=random:<seed> right-hand side of the C++
+    driver (k_i = 2 u_i - 1, cgxx_amd/host/CG.cpp), bit for bit."""
+    i = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def random_rhs(n: int, seed: int) -> np.ndarray:
+    """The CG_RHS=random:<seed> right-hand side: k_i = 2 u_i - 1 in [-1, 1)."""
+    return 2.0 * splitmix_uniform(n, seed) - 1.0
+
+
 def zipf_spd(n: int = 1_000_000, alpha: float = 2.0, cap: int = 2000, seed: int = 20261202) -> CRS:
     """Heavy-tailed random SPD matrix with a symmetric pattern.
 

@@ -1,7 +1,9 @@
 """Shared helpers: rebuild a golden fixture's inputs and compare histories."""
 from __future__ import annotations
 
+import json
 import os
+import re
 
 import numpy as np
 
@@ -99,3 +101,54 @@ def max_rel(a, b) -> float:
         return 0.0
     d = np.abs(a[:n] - b[:n]) / np.maximum(np.abs(b[:n]), np.finfo(float).tiny)
     return float(d.max())
+
+
+# ---- the reference's printed report (tests/golden/cli) -------------------
+
+TIMED = re.compile(r"^(IO time|Converting time|Transfer to time|Solve time|Transfer from time|"
+                   r"Check time|Total time \(excl\. IO\)|MatVec time|MatVec GFLOP/s|axpy time|"
+                   r"xpay time|vectorDot time|Preconditioner time):")
+CLI_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cli",
+                           "reference_output.json")
+
+
+def load_cli_fixture() -> dict:
+    with open(CLI_FIXTURE) as f:
+        return json.load(f)
+
+
+def untimed_report(stdout: str, path: str) -> list[str]:
+    """A CG report's lines with every timing value blanked ("<t>"), the
+    device-only transfer lines dropped and the matrix path as <matrix>: the
+    form of the fixture written by tests/golden/make_golden.py cli."""
+    out = []
+    for ln in stdout.splitlines():
+        if ln.startswith("Transfer to time:") or ln.startswith("Transfer from time:"):
+            continue  # only device backends transfer
+        out.append(TIMED.match(ln).group(1) + ": <t>" if TIMED.match(ln) else ln)
+    return [ln.replace(path, "<matrix>") for ln in out]
+
+
+def write_cli_matrices(tmpdir: str) -> dict:
+    """The fixture's matrices as .mtx files (checked against their sha256)."""
+    files = {}
+    for name, case in load_cli_fixture()["cases"].items():
+        p = os.path.join(tmpdir, f"{name}.mtx")
+        m = getattr(problems, case["generator"])(**case["kwargs"])
+        assert problems.write_mm(p, m, symmetric=case["symmetric"]) == case["mtx_sha256"]
+        files[name] = p
+    return files
+
+
+# ---- sanitizer builds of the C++ host tools --------------------------------
+
+# Compiler flags of each build variant of the host test tools: the plain
+# build, AddressSanitizer + UndefinedBehaviorSanitizer (any report aborts the
+# run, so a test that expects exit 0 or a given message fails), and
+# ThreadSanitizer for the parallel reader.
+SANITIZERS = {
+    "plain": ["-O2"],
+    "asan-ubsan": ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                   "-fno-sanitize-recover=all"],
+    "tsan": ["-O1", "-g", "-fsanitize=thread"],
+}

@@ -2,8 +2,9 @@
 
 Run in the development container, where the reference sources exist:
 
-    make -C oracle ref            # builds oracle/_ref/cg_serial_trace
-    python tests/golden/make_golden.py
+    make -C oracle ref            # builds oracle/_ref/cg_serial_trace, cg_serial
+    python tests/golden/make_golden.py       # per-iteration histories (*.json)
+    python tests/golden/make_golden.py cli   # cli/reference_output.json
 
 ``cg_serial_trace`` is the reference's serial binary (serial/SerialCG.cpp +
 CG.cpp + Matrix.cpp + Preconditioner.cpp + WorkDistribution.cpp) compiled
@@ -13,6 +14,12 @@ this script writes the matrix with cgxx_amd.problems.write_mm, runs the
 binary under the given CG_* environment, and stores the full per-iteration
 history, the iteration count, the printed residual, the first ten entries of
 x and the check line as JSON (floats as C99 hex strings).
+
+With the argument ``cli`` it instead runs the plain reference binary
+(``cg_serial``) on three matrices in CRS, ELL and COO and stores its whole
+report with the timing values blanked, the fixture of the end-to-end tests
+of the ``cg_hip`` executable (which therefore never need the reference's
+binaries on a GPU machine).
 
 The fixtures are data: inputs are named by generator and parameters (plus the
 sha256 of the .mtx text), outputs are what the reference printed.
@@ -32,6 +39,8 @@ sys.path.insert(0, REPO)
 from cgxx_amd import problems  # noqa: E402
 
 BIN = os.path.join(REPO, "oracle", "_ref", "cg_serial_trace")
+SERIAL = os.path.join(REPO, "oracle", "_ref", "cg_serial")
+CLI_FIXTURE = os.path.join(HERE, "cli", "reference_output.json")
 
 # (case name, generator name, generator kwargs, symmetric file, runs)
 # runs: (CG_MATRIX_FORMAT, CG_PRECONDITIONER, extra env)
@@ -93,7 +102,57 @@ def parse(stdout: str) -> dict:
     return out
 
 
+# The reference binary's full report for the end-to-end tests of cg_hip:
+# every line of its stdout with the timing values replaced by "<t>" (the
+# untimed lines must be identical when cg_hip runs the reference's loop and
+# summation order).  (case name, generator, kwargs, symmetric file).
+CLI_CASES = [
+    ("lap2d_256", "laplace2d", {"side": 256}, False),
+    ("rand_sym_40", "random_spd_dense", {"n": 40, "density": 0.3, "seed": 7}, True),
+    ("diff3d_24", "diffusion3d", {"side": 24, "sigma": 1.0, "corr": 4.0, "seed": 5}, False),
+]
+CLI_ENV = {"CG_PRECONDITIONER": "jacobi", "CG_MAX_ITER": "5000"}
+TIMED_LABELS = ("IO time", "Converting time", "Solve time", "Check time", "Total time (excl. IO)",
+                "MatVec time", "MatVec GFLOP/s", "axpy time", "xpay time", "vectorDot time",
+                "Preconditioner time")
+
+
+def untimed_lines(stdout: str) -> list[str]:
+    out = []
+    for ln in stdout.splitlines():
+        label = ln.split(":", 1)[0]
+        out.append(label + ": <t>" if ":" in ln and label in TIMED_LABELS else ln)
+    return out
+
+
+def make_cli_fixture() -> None:
+    """Writes tests/golden/cli/reference_output.json from oracle/_ref/cg_serial."""
+    if not os.path.exists(SERIAL):
+        sys.exit(f"missing {SERIAL}: run `make -C oracle ref` first")
+    fixture = {"env": CLI_ENV, "cases": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, gen, kw, sym in CLI_CASES:
+            m = getattr(problems, gen)(**kw)
+            path = os.path.join(tmp, name + ".mtx")
+            sha = problems.write_mm(path, m, symmetric=sym)
+            runs = {}
+            for fmt in ("CRS", "ELL", "COO"):
+                env = dict(os.environ, CG_MATRIX_FORMAT=fmt, **CLI_ENV)
+                r = subprocess.run([SERIAL, path], env=env, capture_output=True, text=True, check=True)
+                # the file name is the test's own temporary path
+                runs[fmt] = [ln.replace(path, "<matrix>") for ln in untimed_lines(r.stdout)]
+                print(f"cli {name} {fmt}: {len(runs[fmt])} lines", flush=True)
+            fixture["cases"][name] = {"generator": gen, "kwargs": kw, "symmetric": sym,
+                                      "mtx_sha256": sha, "runs": runs}
+    os.makedirs(os.path.dirname(CLI_FIXTURE), exist_ok=True)
+    with open(CLI_FIXTURE, "w") as f:
+        json.dump(fixture, f, indent=1)
+
+
 def main() -> None:
+    if sys.argv[1:] == ["cli"]:
+        make_cli_fixture()
+        return
     if not os.path.exists(BIN):
         sys.exit(f"missing {BIN}: run `make -C oracle ref` first")
     only = set(sys.argv[1:])

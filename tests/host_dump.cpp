@@ -28,5 +28,9 @@ int main(int argc, char **argv) {
   std::fwrite(ell.data, sizeof(double), (size_t)ell.elements, f);
   std::fwrite(jac.C, sizeof(double), coo.N, f);
   std::fclose(f);
+  // explicit release, as CG::cleanup does (the matrices own raw arrays)
+  crs.deallocate();
+  ell.deallocate();
+  jac.deallocateC();
   return 0;
 }

@@ -1,29 +1,27 @@
-"""The cg_hip executable end to end, next to the reference's own serial binary.
+"""The cg_hip executable end to end, against the reference's own report.
 
-Both binaries read the same .mtx file.  The reference binary
-(oracle/_ref/cg_serial, built from the reference's sources by oracle/Makefile
-and shipped prebuilt) is the checker here; cg_hip is the product.  With
-CG_HIP_FUSED=0 CG_HIP_DOT=serial, cg_hip runs the reference's loop hook by
-hook in the reference's summation order, so every printed line except the
-timings must be identical.  The default fused device loop must reach the
-same iteration count and pass the same check.
+The reference's serial binary, built from its sources in the development
+container, printed its report for the three matrices below in CRS, ELL and
+COO; tests/golden/cli/reference_output.json holds every line with the
+timing values blanked (tests/golden/make_golden.py cli).  No reference
+binary is needed here.  With CG_HIP_FUSED=0 CG_HIP_DOT=serial, cg_hip runs
+the reference's loop hook by hook in the reference's summation order, so
+every printed line except the timings must be identical to those.  The
+default fused device loop must reach the same iteration count (+-1) and
+pass the same check.
 """
 import os
-import re
 import subprocess
 
 import pytest
 
-from cgxx_amd import problems
+from tests.cases import load_cli_fixture, untimed_report, write_cli_matrices
 from tests.conftest import REPO
 
 pytestmark = pytest.mark.gpu
 
 EXE = os.path.join(REPO, "cgxx_amd", "bin", "cg_hip")
-REF = os.path.join(REPO, "oracle", "_ref", "cg_serial")
-TIMED = re.compile(r"^(IO time|Converting time|Transfer to time|Solve time|Transfer from time|"
-                   r"Check time|Total time \(excl\. IO\)|MatVec time|MatVec GFLOP/s|axpy time|"
-                   r"xpay time|vectorDot time|Preconditioner time):")
+REFERENCE = load_cli_fixture()
 
 
 def run(exe, path, **env):
@@ -35,55 +33,42 @@ def run(exe, path, **env):
     return subprocess.run(args, capture_output=True, text=True, env=e, timeout=300)
 
 
-def untimed(stdout: str):
-    out = []
-    for ln in stdout.splitlines():
-        if ln.startswith("Transfer to time:") or ln.startswith("Transfer from time:"):
-            continue  # only device backends transfer
-        out.append(TIMED.sub(lambda m: m.group(1) + ": <t>", ln) if TIMED.match(ln) else ln)
-    return [re.sub(r"<t>.*", "<t>", ln) for ln in out]
+def reference_lines(name, fmt):
+    return REFERENCE["cases"][name]["runs"][fmt]
 
 
 @pytest.fixture(scope="module")
 def mtx(tmp_path_factory):
-    d = tmp_path_factory.mktemp("cli")
-    files = {}
-    for name, m, sym in [("lap2d_256", problems.laplace2d(256), False),
-                         ("rand_sym_40", problems.random_spd_dense(40, 0.3, 7), True),
-                         ("diff3d_24", problems.diffusion3d(24, 1.0, 4.0, 5), False)]:
-        p = str(d / f"{name}.mtx")
-        problems.write_mm(p, m, symmetric=sym)
-        files[name] = p
-    return files
+    return write_cli_matrices(str(tmp_path_factory.mktemp("cli")))
 
 
 @pytest.mark.parametrize("name", ["lap2d_256", "rand_sym_40", "diff3d_24"])
 @pytest.mark.parametrize("fmt", ["CRS", "ELL", "COO"])
 def test_reference_order_output_identical(mtx, name, fmt):
-    env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=5000)
-    ref = run(REF, mtx[name], **env)
+    env = dict(CG_MATRIX_FORMAT=fmt, **REFERENCE["env"])
     ours = run(EXE, mtx[name], CG_HIP_FUSED=0, CG_HIP_DOT="serial", **env)
-    assert ref.returncode == 0 and ours.returncode == 0, ours.stderr
-    assert untimed(ours.stdout) == untimed(ref.stdout)
+    assert ours.returncode == 0, ours.stderr
+    assert untimed_report(ours.stdout, mtx[name]) == reference_lines(name, fmt)
 
 
 @pytest.mark.parametrize("name", ["lap2d_256", "rand_sym_40", "diff3d_24"])
 @pytest.mark.parametrize("fmt", ["CRS", "ELL", "COO"])
 def test_fused_matches_reference_run(mtx, name, fmt):
-    env = dict(CG_MATRIX_FORMAT=fmt, CG_PRECONDITIONER="jacobi", CG_MAX_ITER=5000)
-    ref = run(REF, mtx[name], **env)
+    env = dict(CG_MATRIX_FORMAT=fmt, **REFERENCE["env"])
     ours = run(EXE, mtx[name], **env)
     assert ours.returncode == 0, ours.stderr
 
-    def field(out, label):
-        return [ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith(label)][0]
+    def field(lines, label):
+        return [ln.split(":", 1)[1].strip() for ln in lines if ln.startswith(label)][0]
 
-    assert abs(int(field(ours.stdout, "Iterations")) - int(field(ref.stdout, "Iterations"))) <= 1
+    ref_it = int(field(reference_lines(name, fmt), "Iterations"))
+    assert abs(int(field(ours.stdout.splitlines(), "Iterations")) - ref_it) <= 1
+    assert "Solution is correct!" in reference_lines(name, fmt)
     assert "Solution is correct!" in ours.stdout
     # every timing category is reported as a number
     for label in ("Solve time", "MatVec time", "axpy time", "xpay time", "MatVec GFLOP/s"):
-        float(field(ours.stdout, label))
-    assert float(field(ours.stdout, "MatVec time")) > 0
+        float(field(ours.stdout.splitlines(), label))
+    assert float(field(ours.stdout.splitlines(), "MatVec time")) > 0
 
 
 def test_repetitions_and_strict_check(mtx):
@@ -161,6 +146,15 @@ def test_solve_record(mtx, mode, tmp_path):
         assert r["incorrect_elements"] == 0
         assert abs(r["times"]["solve"] - float(field(ours.stdout, "Solve time", i))) <= 1e-6
         assert abs(r["times"]["io"] - float(field(ours.stdout, "IO time", i))) <= 1e-6
+        # per-iteration residuals of the loop's recursion and the SpMV rate
+        h = r["residual_history"]
+        if mode != "learned":
+            assert len(h) == r["iterations"] + 1 and h[0] == 1.0
+            assert all(v > 0 for v in h)
+        if mode in ("fused", "hooks"):
+            assert h[-1] == r["residual"]
+        assert r["rhs"] == "ones" and r["matvec_bytes"] > 0
+        assert r["peak_gbps"] == 8000.0 and 0 < r["matvec_peak_fraction"] < 2
         b = r["backend"]
         assert b["name"] == "hip" and b["dot"] == "tree"
         assert b["loop"] == {"fused": "fused", "hooks": "hooks", "mixed": "mixed", "learned": "mixed"}[mode]
@@ -177,3 +171,23 @@ def test_solve_record(mtx, mode, tmp_path):
 def test_solve_record_unwritable(mtx, tmp_path):
     ours = run(EXE, mtx["lap2d_256"], CG_REPORT_JSON=str(tmp_path / "no" / "such" / "dir.jsonl"))
     assert ours.returncode == 1 and "Could not write the solve record" in ours.stderr
+
+
+@pytest.mark.parametrize("mode", ["fused", "mixed"])
+def test_random_rhs_cli(mtx, mode, tmp_path):
+    """CG_RHS=random:<seed> on the device: the FP64 true relative residual
+    (computed on the host from the returned x) passes the check, and the
+    record names the right-hand side."""
+    rec = tmp_path / "solves.jsonl"
+    env = dict(CG_MATRIX_FORMAT="CRS", CG_PRECONDITIONER="jacobi", CG_RHS="random:7",
+               CG_REPORT_JSON=str(rec), CG_MAX_ITER=20000, CG_STRICT_CHECK=1)
+    if mode == "mixed":
+        env["CG_HIP_MIXED"] = 1
+    ours = run(EXE, mtx["diff3d_24"], **env)
+    assert ours.returncode == 0, ours.stderr + ours.stdout
+    tr = float(field(ours.stdout, "True relative residual"))
+    assert tr <= 1e-8 and "Solution is correct!" in ours.stdout
+    import json
+    r = json.loads(rec.read_text())
+    assert r["rhs"] == "random:7" and r["true_relative_residual"] == pytest.approx(tr, rel=1e-6)
+

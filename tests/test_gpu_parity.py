@@ -7,8 +7,11 @@ Tiers (all through the C ABI, include/cghip.h):
     against the reference's golden histories (r2, rho, dot_pq, a, b, x[:10]);
   * the same loop with tree dots: bit-exact against oracle order 2;
   * the fused device loop: bit-exact against oracle order 1, and within
-    1e-10 relative per iteration of the reference's history (iterations +-1,
-    CG_CHECK_TOLERANCE check passes) -- the tolerance the FP64 path states.
+    1e-10 relative per iteration of the reference's history while the
+    residual is above 1e-6 -- the tolerance the FP64 path states -- and down
+    to the stop test within 1e-10 or the measured worst of its tree-order
+    sums (GPU_R2_WORST; C1 1.5e-12, C2 7.7e-10), iterations +-1, the
+    CG_CHECK_TOLERANCE check passing.
 """
 import numpy as np
 import pytest
@@ -166,8 +169,6 @@ def test_reference_loop_serial_dots_bitwise(ctx, pair, tmp_path):
     from cgxx_amd.cghip import reference_loop
 
     fx, run = pair
-    if fx["N"] > 5000 and fx["case"] != "lap2d_256":
-        pytest.skip("serial-dot parity runs on the small fixtures and C1 only")
     case = Case(fx, str(tmp_path))
     fmt, pre, maxit, tol = run_env(run)
     g = golden_sequences(run)
@@ -205,6 +206,15 @@ def test_reference_loop_tree_dots_match_oracle(ctx, pair, tmp_path):
     assert_bitwise(ctx.download("X"), ox, "x")
 
 
+# Worst relative r2 deviation of the fused (tree-order) loop from the
+# reference's serial-order history, over every iteration with residual above
+# 1e-12, where it exceeds 1e-10 (measured; the loop is bit-exact with the
+# oracle's order 1, so these are the oracle's figures): lap3d_128 (C2, CRS
+# and ELL) 7.67e-10 (4.91e-11 above residual 1e-6), diff3d_24 1.41e-10,
+# lap2d_16 1.32e-10, rand_sym_40 4.56e-9.  C1 (lap2d_256): 1.51e-12.
+GPU_R2_WORST = {"lap3d_128": 8e-10, "diff3d_24": 1.5e-10, "lap2d_16": 1.4e-10, "rand_sym_40": 5e-9}
+
+
 @pytest.mark.parametrize("pair", ALL_DEVICE_RUNS, ids=[run_id(p) for p in ALL_DEVICE_RUNS])
 def test_fused_solve(ctx, pair, tmp_path):
     """Fused device loop: bit-exact vs oracle order 1; 1e-10 vs the reference."""
@@ -227,12 +237,14 @@ def test_fused_solve(ctx, pair, tmp_path):
         assert_bitwise(x, ox, "x")
         assert info["residual"] == ores
     # Against the reference's own run.  The fused loop sums its dots in a
-    # tree, SerialCG left to right, so the two recurrences round differently.
-    # Tolerance: 1e-10 relative per iteration on C1 (lap2d_256), and on every
-    # case while the residual is above 1e-6; in the last iterations, where r2
-    # is ~1e-18 of r2_0 and carries rounding noise of the reference's own
-    # sums, 1e-8 (measured worst case 6.1e-9, rand_sym_40); below a residual
-    # of 1e-12 r2 is pure rounding noise and is not compared.
+    # tree, SerialCG left to right, so the two recurrences round differently
+    # and drift apart as r2 falls.  Bounds, relative per iteration, over
+    # every iteration down to the stop test (CG.cpp:344-355): 1e-10 while
+    # the residual is above 1e-6 on every case, and through the last
+    # iteration 1e-10 wherever the measured worst allows it, else that
+    # measured worst rounded up (GPU_R2_WORST); below a residual of 1e-12
+    # (zipf_3000's single step lands at 1e-13) r2 is rounding noise of the
+    # reference's own sums and is not compared.
     assert abs(it - run["iterations"]) <= 1
     m = min(it, run["iterations"])
     a, b = h["r2"][: m + 1], g["r2"][: m + 1]
@@ -240,10 +252,9 @@ def test_fused_solve(ctx, pair, tmp_path):
     rel = np.abs(a - b) / np.abs(b)
     early = res > 1e-6
     assert rel[early].max() <= 1e-10, f"r2 deviates by {rel[early].max():.3e} above residual 1e-6"
-    if fx["case"] == "lap2d_256":
-        assert rel.max() <= 1e-10, f"C1 r2 deviates by {rel.max():.3e}"
     late = res > 1e-12
-    assert rel[late].max() <= 1e-8, f"r2 deviates by {rel[late].max():.3e}"
+    bound = GPU_R2_WORST.get(fx["case"], 1e-10)
+    assert rel[late].max() <= bound, f"r2 deviates by {rel[late].max():.3e} (bound {bound:.1e})"
     errors = int(np.sum(np.abs(x - 1.0) > 1e-5))
     expect_ok = run["check"] == "Solution is correct!"
     assert (errors == 0) == expect_ok

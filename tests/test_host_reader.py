@@ -1,7 +1,8 @@
 """The C++ host's Matrix Market reader and converters against the oracle's.
 
 Builds tests/host_dump.cpp with the product's Matrix.cpp/Preconditioner.cpp
-(g++, a few seconds) and compares every array bit for bit on the golden
+(g++, a few seconds; plain, with AddressSanitizer + UndefinedBehaviorSanitizer
+and with ThreadSanitizer, every test running against each) and compares every array bit for bit on the golden
 fixtures' matrices, including the symmetric one (mirroring order), at several
 thread counts (CG_IO_THREADS), plus the reader's error messages, its decoding
 of values against Python's correctly rounded float(), line-ending and
@@ -16,16 +17,18 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from tests.cases import Case
+from tests.cases import SANITIZERS, Case
 from tests.conftest import REPO, load_golden
 
 HOST = os.path.join(REPO, "cgxx_amd", "host")
 
 
-@pytest.fixture(scope="module")
-def dump_bin(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("hostdump") / "host_dump")
-    subprocess.run(["g++", "-std=c++17", "-O2", "-pthread", "-I" + HOST, "-o", out,
+@pytest.fixture(scope="module", params=["plain", "asan-ubsan", "tsan"])
+def dump_bin(request, tmp_path_factory):
+    """The reader tool, plain and under each sanitizer: every test below runs
+    against all three builds."""
+    out = str(tmp_path_factory.mktemp("hostdump") / f"host_dump_{request.param}")
+    subprocess.run(["g++", "-std=c++17", *SANITIZERS[request.param], "-pthread", "-I" + HOST, "-o", out,
                     os.path.join(REPO, "tests", "host_dump.cpp"),
                     os.path.join(HOST, "Matrix.cpp"), os.path.join(HOST, "Preconditioner.cpp")],
                    check=True)
