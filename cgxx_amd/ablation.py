@@ -6,11 +6,16 @@ residual 1e-9.  Each method runs once untimed on the system, then twice
 timed, and the faster timed run counts: the next systems arrive from the
 generator processes on a background thread of this interpreter, which can
 delay a single run by tens of milliseconds.  The methods:
-  learned  mixed precision, polynomial from the network (features and
-           inference timed with the solve), Jacobi fallback
-  fixed    mixed precision, Chebyshev on [theta_min(10) / rho*, b] (rho*
-           tuned on the training systems), the same fallback
-  ritz     the same with rho = 1: Chebyshev on the 10-step Ritz interval
+  learned  mixed precision, the network's choice from the matrix statistics:
+           Jacobi, or the polynomial of the degree it predicts cheapest on
+           [c_k theta_min_pred, b] (statistics and inference timed with the
+           solve), Jacobi fallback
+  fixed    mixed precision, 10 FP64 PCG steps for theta10 (timed with the
+           solve, and their 10 SpMVs counted), Chebyshev of one degree on
+           [theta10 / rho*, b] (degree and rho* tuned on the training
+           systems), the same fallback
+  ritz     as fixed, degree 8 with rho = 1: Chebyshev on the 10-step Ritz
+           interval
   jacobi   mixed precision, Jacobi
   fp64     the plain FP64 Jacobi-PCG loop (the reference's method)
 
@@ -56,6 +61,7 @@ def main() -> int:
 
     model = learned.load_model(args.model or learned.MODEL_PATH)
     rho = float(model["rho_star"])
+    fixed_degree = int(model["fixed_degree"])
     ctx = Context(0)
     mlp = Mlp(tuple(int(v) for v in model["dims"]), 1)
     mlp.set_params(model["params"])
@@ -95,19 +101,16 @@ def main() -> int:
         r.update({f"learned_{key}": lr[key] for key in ("spmvs", "wall_s", "true_residual",
                                                            "poly_iterations", "jacobi_iterations")})
         r["learned_method"] = lr["method"]
+        r["learned_degree"] = lr["decision"].get("degree", 0)
 
-        def chebyshev(rho_):
-            t1 = time.perf_counter()
-            f, b, theta10 = learned.system_features(ctx, n)
-            out = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho_), b)
-            out["wall_s"] = time.perf_counter() - t1
-            return out
+        def chebyshev(rho_, degree):
+            return learned.solve_fixed_rule(ctx, n, rho_, degree)
 
-        fr = best(lambda: chebyshev(rho))
+        fr = best(lambda: chebyshev(rho, fixed_degree))
         r.update({f"fixed_{key}": fr[key] for key in ("spmvs", "wall_s", "true_residual",
                                                          "poly_iterations", "jacobi_iterations")})
         r["fixed_method"] = fr["method"]
-        rr_ = best(lambda: chebyshev(1.0))
+        rr_ = best(lambda: chebyshev(1.0, 8))
         r.update({f"ritz_{key}": rr_[key] for key in ("spmvs", "wall_s", "true_residual",
                                                          "poly_iterations", "jacobi_iterations")})
         r["ritz_method"] = rr_["method"]
@@ -148,6 +151,9 @@ def main() -> int:
     summary = {
         "systems": len(rows),
         "rho_star": rho,
+        "fixed_degree": fixed_degree,
+        "learned_degrees": {str(k): int(np.sum(cols["learned_degree"] == k))
+                            for k in sorted(set(int(v) for v in cols["learned_degree"]))},
         "spmvs": {m: tot(f"{m}_spmvs") for m in ("learned", "fixed", "ritz", "jacobi", "fp64")},
         "seconds": {m: tot(f"{m}_wall_s") for m in ("learned", "fixed", "ritz", "jacobi", "fp64")},
         "learned_vs_fixed_spmvs": tot("fixed_spmvs") / tot("learned_spmvs"),

@@ -27,9 +27,11 @@
 //                   with Jacobi on the scaled operator D^-1/2 A D^-1/2); the
 //                   reported residual is the FP64 true relative residual.
 //   CG_HIP_POLY     none (default) | learned | fixed: with CG_HIP_MIXED=1 and
-//                   Jacobi, the degree-8 polynomial preconditioner from the
-//                   trained network (with its Jacobi choice and fallback) or
-//                   from the fixed Chebyshev rule.
+//                   Jacobi, the polynomial preconditioner chosen by the
+//                   trained network from the matrix statistics (degree and
+//                   interval from its predicted theta_min, or Jacobi) or by
+//                   the fixed Chebyshev rule (10 FP64 PCG steps for theta10),
+//                   both with the Jacobi fallback.
 //   CG_HIP_MODEL    model file for CG_HIP_POLY (default: ../data/poly_mlp.bin
 //                   next to this executable).
 #include <cerrno>
@@ -257,7 +259,8 @@ bool CGHIP::solveMixed(int maxIterations, floatType tolerance, FusedResult &resu
       std::cerr << "HIP error: " << err << std::endl;
       std::exit(1);
     }
-    std::cout << "HIP mixed precision, polynomial (" << polyMode << "): " << r.method << ", "
+    std::cout << "HIP mixed precision, polynomial (" << polyMode << "): " << r.method
+              << (r.degree ? ", degree " + std::to_string(r.degree) : std::string()) << ", "
               << r.polyIterations << " polynomial + " << r.jacobiIterations
               << " Jacobi inner iterations" << std::endl;
     lastLoop = "mixed";
@@ -325,7 +328,8 @@ void CGHIP::recordBackend(std::ostream &os) {
   };
   os << ", \"mixed\": {\"polynomial\": \"" << polyMode << "\"";
   if (learned) {
-    os << ", \"method\": \"" << lastLearned.method << "\", \"poly_iterations\": "
+    os << ", \"method\": \"" << lastLearned.method << "\", \"degree\": " << lastLearned.degree
+       << ", \"poly_iterations\": "
        << lastLearned.polyIterations << ", \"jacobi_iterations\": " << lastLearned.jacobiIterations
        << ", \"true_residual\": ";
     num(lastLearned.trueResidual);

@@ -88,24 +88,56 @@ bool PolyModel::load(const std::string &path, std::string &err) {
   f.read(magic, 8);
   int32_t hdr[6];
   f.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
-  if (!f || std::memcmp(magic, "CGHIPMLP", 8) != 0 || hdr[0] != 1) {
-    err = "not a version-1 CGHIPMLP model file: " + path;
+  if (!f || std::memcmp(magic, "CGHIPMLP", 8) != 0 || hdr[0] != 3) {
+    err = "not a version-3 CGHIPMLP model file: " + path;
     return false;
   }
-  in = hdr[1], h1 = hdr[2], h2 = hdr[3], out = hdr[4], degree = hdr[5];
-  if (in != 42 || out != degree + 3 || degree < 1 || degree > 16) {
-    err = "model shape does not match the 42 features and degree + 3 outputs";
+  in = hdr[1], h1 = hdr[2], h2 = hdr[3], out = hdr[4];
+  const int D = hdr[5];
+  if (D < 1 || D > 16) {
+    err = "truncated or inconsistent model file " + path;
+    return false;
+  }
+  std::vector<int32_t> deg(D + 1);
+  f.read(reinterpret_cast<char *>(deg.data()), sizeof(int32_t) * (D + 1));
+  degrees.assign(deg.begin(), deg.begin() + D);
+  fixedDegree = deg[D];
+  bool shape = f && in == kStats + 1 && out == 2 * D + 3 && fixedDegree >= 1 && fixedDegree <= 16;
+  for (int k : degrees) shape = shape && k >= 1 && k <= 16;
+  if (!shape) {
+    err = "model shape does not match the 37 features and 2 D + 3 outputs";
     return false;
   }
   f.read(reinterpret_cast<char *>(&rho_star), sizeof(double));
-  bool ok = readArray(f, feat_mean, in) && readArray(f, feat_std, in) &&
-            readArray(f, feat_lo, in) && readArray(f, feat_hi, in) &&
+  bool ok = readArray(f, lowerEnd, D) && readArray(f, feat_mean, in) &&
+            readArray(f, feat_std, in) && readArray(f, feat_lo, in) && readArray(f, feat_hi, in) &&
             readArray(f, out_mean, out) && readArray(f, out_std, out);
+  for (double c : lowerEnd) ok = ok && c > 0;
   int32_t np = 0;
   f.read(reinterpret_cast<char *>(&np), sizeof(np));
   ok = ok && f && np == cghip_mlp_param_count(in, h1, h2, out) && readArray(f, params, np);
   if (!ok) err = "truncated or inconsistent model file " + path;
   return ok;
+}
+
+LearnedDecision decide(const PolyModel &m, const float *yn) {
+  const int D = (int)m.degrees.size();
+  auto y = [&](int i) { return (double)yn[i] * m.out_std[i] + m.out_mean[i]; };
+  LearnedDecision d;
+  d.thetaMin = std::exp(y(0));
+  d.costJacobi = std::exp(y(1 + 2 * D) + y(2 + 2 * D));
+  double best = d.costJacobi;
+  d.cost.resize(D);
+  for (int j = 0; j < D; j++) {
+    d.cost[j] = std::exp(y(1 + j) + y(1 + D + j));
+    if (d.cost[j] < best) {
+      best = d.cost[j];
+      d.degree = m.degrees[j];
+      d.lowerEnd = m.lowerEnd[j] * d.thetaMin;
+      d.msPerIter = std::exp(y(1 + D + j));
+    }
+  }
+  return d;
 }
 
 void ritzExtremes(const double *alpha, const double *beta, int m, double &lo, double &hi) {
@@ -145,34 +177,31 @@ bool LearnedPoly::solve(int n, double tol, int maxIter, LearnedResult &res, std:
     err = std::string(what) + ": " + cghip_last_error(ctx_);
     return false;
   };
-  // ---- features: matrix statistics and 10 FP64 Jacobi-PCG steps ----
+  // ---- the network's inputs: matrix statistics and log N ----
   std::vector<double> f(model_.in, 0.0);
   if (cghip_matrix_stats(ctx_, f.data(), kStats)) return fail("matrix statistics");
   const double b = 1.0 + f[7];  // 1 + max Gershgorin radius of D^-1 A
-  if (cghip_fill_vector(ctx_, CGHIP_VEC_X, 0.0)) return fail("fill");
-  cghip_solve_opts so{CGHIP_PRECOND_JACOBI, kRitzSteps, 1e-300, 0, CGHIP_PROFILE_NONE};
-  cghip_solve_info si{};
-  if (cghip_solve(ctx_, &so, &si)) return fail("10 PCG steps");
-  double alpha[kRitzSteps], beta[kRitzSteps], r2[kRitzSteps + 1];
-  int w = 0;
-  if (si.iterations < kRitzSteps || cghip_history(ctx_, 3, alpha, kRitzSteps, &w) ||
-      cghip_history(ctx_, 4, beta, kRitzSteps, &w) || cghip_history(ctx_, 0, r2, kRitzSteps + 1, &w))
-    return fail("PCG history");
-  double lo10, hi10, lo5, hi5;
-  ritzExtremes(alpha, beta, kRitzSteps, lo10, hi10);
-  ritzExtremes(alpha, beta, 5, lo5, hi5);
   f[kStats] = std::log((double)n);
-  f[kStats + 1] = std::log(lo10);
-  f[kStats + 2] = std::log(hi10);
-  f[kStats + 3] = std::log(lo5);
-  f[kStats + 4] = std::log(hi5);
-  f[kStats + 5] = std::log10(std::max(r2[kRitzSteps], 1e-300) / r2[0]) / kRitzSteps;
 
-  // ---- coefficients ----
-  const int k = model_.degree;
+  // ---- choice and coefficients ----
+  int k = model_.fixedDegree;
   std::vector<double> coeffs;
-  double itJ = 0, itP = 0;
+  bool usePoly = true;
+  int budgetIters = std::max(1, maxIter / (k + 1));
   if (mode_ == "fixed") {
+    // the rule without the network estimates the spectrum: 10 FP64
+    // Jacobi-PCG steps for the smallest Ritz value
+    if (cghip_fill_vector(ctx_, CGHIP_VEC_X, 0.0)) return fail("fill");
+    cghip_solve_opts so{CGHIP_PRECOND_JACOBI, kRitzSteps, 1e-300, 0, CGHIP_PROFILE_NONE};
+    cghip_solve_info si{};
+    if (cghip_solve(ctx_, &so, &si)) return fail("10 PCG steps");
+    double alpha[kRitzSteps], beta[kRitzSteps];
+    int w = 0;
+    if (si.iterations < kRitzSteps || cghip_history(ctx_, 3, alpha, kRitzSteps, &w) ||
+        cghip_history(ctx_, 4, beta, kRitzSteps, &w))
+      return fail("PCG history");
+    double lo10, hi10;
+    ritzExtremes(alpha, beta, kRitzSteps, lo10, hi10);
     coeffs = chebResidualCoeffs(std::min(lo10 / model_.rho_star, 0.25 * b), b, k);
   } else {
     if (!mlp_ && cghip_mlp_create(device_, model_.in, model_.h1, model_.h2, model_.out, 1, &mlp_)) {
@@ -192,35 +221,24 @@ bool LearnedPoly::solve(int n, double tol, int maxIter, LearnedResult &res, std:
       err = std::string("network forward: ") + cghip_mlp_last_error(mlp_);
       return false;
     }
-    std::vector<double> pv(k + 1);
-    for (int j = 0; j <= k; j++) pv[j] = std::exp((double)yout[j] * model_.out_std[j] + model_.out_mean[j]);
-    coeffs = dct(pv);
-    itJ = std::exp((double)yout[k + 1] * model_.out_std[k + 1] + model_.out_mean[k + 1]);
-    itP = std::exp((double)yout[k + 2] * model_.out_std[k + 2] + model_.out_mean[k + 2]);
+    const LearnedDecision d = decide(model_, yout.data());
+    usePoly = d.degree > 0;
+    if (usePoly) {
+      k = d.degree;
+      coeffs = chebResidualCoeffs(std::min(d.lowerEnd, 0.25 * b), b, k);
+      budgetIters = std::max(1, (int)std::ceil(1.25 * d.costJacobi / d.msPerIter));
+    }
   }
-
-  // ---- choice and budget ----
-  const bool accepted = cghip_set_poly(ctx_, k, coeffs.data(), b) == CGHIP_OK;
-  int budgetIters = std::max(1, maxIter / (k + 1));
-  bool usePoly = accepted;
-  if (accepted && mode_ != "fixed") {
-    // the library's algorithmic bytes per iteration rank the two by time
-    double bytesJ = 0, bytesP = 0;
-    if (cghip_mixed_iteration_bytes(ctx_, CGHIP_PRECOND_JACOBI, &bytesJ) ||
-        cghip_mixed_iteration_bytes(ctx_, CGHIP_PRECOND_POLY, &bytesP))
-      return fail("iteration bytes");
-    const double costJ = bytesJ * itJ, costP = bytesP * itP;
-    usePoly = costP < costJ;
-    budgetIters = std::max(1, (int)std::ceil(1.25 * costJ / bytesP));
-  }
+  const bool accepted = usePoly && cghip_set_poly(ctx_, k, coeffs.data(), b) == CGHIP_OK;
 
   // ---- solve, with the Jacobi fallback ----
   if (cghip_fill_vector(ctx_, CGHIP_VEC_X, 0.0)) return fail("fill");
   double rel = 1.0;
   res = LearnedResult{};
-  res.method = accepted ? (usePoly ? "poly" : "jacobi-selected") : "jacobi";
+  res.method = !usePoly ? "jacobi-selected" : accepted ? "poly" : "jacobi (refused)";
+  res.degree = accepted ? k : 0;
   cghip_mixed_info mi{};
-  if (usePoly) {
+  if (accepted) {
     cghip_mixed_opts mo{CGHIP_PRECOND_POLY, std::min(budgetIters, maxIter), tol, 0.0, 0,
                         CGHIP_PROFILE_NONE};
     if (cghip_solve_mixed(ctx_, &mo, &mi)) return fail("polynomial solve");

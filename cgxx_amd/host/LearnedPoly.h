@@ -2,13 +2,15 @@
 // from C++ through the C ABI (include/cghip.h, include/cghip_mlp.h).
 //
 // The same sequence as cgxx_amd/learned.py (solve_learned): device matrix
-// statistics and 10 FP64 Jacobi-PCG steps for the 42 features, the network
-// on the GPU for log p at the Chebyshev points of [0, b] and the predicted
-// iteration counts, the discrete Chebyshev transform to coefficients, a
-// choice between the polynomial and Jacobi from the predicted iterations
-// times the library's bytes per iteration (cghip_mixed_iteration_bytes), and
-// the Jacobi fallback judged by the FP64 true residual.  No reference
-// counterpart.
+// statistics and log N (37 inputs, no solver steps), the network on the GPU
+// for the smallest eigenvalue theta_min of D^-1 A and, per candidate degree,
+// the iterations with Chebyshev on [c_k theta_min, b] and the time per
+// iteration, and Jacobi's iterations and time per iteration; the cheapest
+// predicted method runs, the polynomial within an iteration budget worth
+// 1.25x Jacobi's predicted cost, with the Jacobi fallback judged by the FP64
+// true residual.  The "fixed" mode is the rule without the network: 10 FP64
+// Jacobi-PCG steps for the smallest Ritz value theta10, Chebyshev of one
+// degree on [theta10 / rho*, b].  No reference counterpart.
 #pragma once
 
 #include <string>
@@ -18,17 +20,31 @@
 #include "cghip_mlp.h"
 
 struct PolyModel {
-  int in = 0, h1 = 0, h2 = 0, out = 0, degree = 0;
-  double rho_star = 16.0;
+  int in = 0, h1 = 0, h2 = 0, out = 0;
+  std::vector<int> degrees;      // candidate degrees
+  std::vector<double> lowerEnd;  // c_k: the degree's interval is [c_k theta_min, b]
+  int fixedDegree = 8;           // the fixed rule: Chebyshev of this degree on
+  double rho_star = 16.0;        // [theta10 / rho_star, b]
   std::vector<double> feat_mean, feat_std, feat_lo, feat_hi, out_mean, out_std;
   std::vector<float> params;
-  // Reads the binary written by cgxx_amd.learned.export_bin; false + err on
-  // a malformed file.
+  // Reads the version-3 binary written by cgxx_amd.learned.export_bin;
+  // false + err on a malformed file.
   bool load(const std::string &path, std::string &err);
 };
 
+// The choice from the network's standardised outputs (learned.decide):
+// degree 0 = Jacobi; cost[i] the predicted ms of degrees[i], costJacobi
+// Jacobi's; lowerEnd = c_k thetaMin for the chosen degree.
+struct LearnedDecision {
+  int degree = 0;
+  double thetaMin = 0, lowerEnd = 0, msPerIter = 0, costJacobi = 0;
+  std::vector<double> cost;
+};
+LearnedDecision decide(const PolyModel &model, const float *yStandardised);
+
 struct LearnedResult {
   std::string method;      // poly, poly+jacobi, jacobi (refused), jacobi-selected
+  int degree = 0;          // the polynomial's degree (0: none ran)
   int polyIterations = 0;  // inner iterations with the polynomial
   int jacobiIterations = 0;
   double trueResidual = 0;  // FP64 ||k - A x|| / ||k||
@@ -37,8 +53,9 @@ struct LearnedResult {
 
 class LearnedPoly {
  public:
-  // mode "learned": network coefficients and cost choice; "fixed":
-  // Chebyshev on [theta_min(10 steps) / rho*, b], always the polynomial.
+  // mode "learned": the network's spectrum estimate and cost choice;
+  // "fixed": Chebyshev on [theta_min(10 steps) / rho*, b], always the
+  // polynomial.
   LearnedPoly(cghip_ctx *ctx, int device, const PolyModel &model, const std::string &mode);
   ~LearnedPoly();
   // Solves from X = 0 (the context holds the CRS matrix, the Jacobi

@@ -1,33 +1,55 @@
-"""Learned polynomial preconditioner: features, coefficient encoding, labels.
+"""Learned polynomial preconditioner: features, labels, the model and its use.
 
-A small network (include/cghip_mlp.h) maps cheap features of a system to a
-degree-8 polynomial p for the preconditioner z = p(D^-1 A) D^-1 r.
+A small network (include/cghip_mlp.h, trained on the GPU by the library's
+own kernels) maps statistics of the matrix alone -- no solver steps -- to
+the bottom of the spectrum of D^-1 A and to what the mixed-precision solve
+would cost on this device with Jacobi and with the Chebyshev polynomial
+preconditioner of each candidate degree.  The solve then runs the cheapest
+choice, the polynomial on [c_k theta_min, b] with theta_min predicted.  The
+method without the network must first estimate the spectrum: 10 FP64
+Jacobi-PCG steps for the smallest Ritz value theta10, then Chebyshev of one
+tuned degree on [theta10 / rho*, b].
 
-Features (42):
+Inputs (37):
   0..35   cghip_matrix_stats: mean/std/min/max of log a_ii, the Gershgorin
           radius sum_{j!=i}|a_ij|/a_ii, log strongest and weakest coupling
           |a_ij|/a_ii and their ratio; the row-length histogram
   36      log N
-  37..41  from the first 10 steps of FP64 Jacobi-PCG (the reference's loop,
-          CG.cpp:284-385): log of the smallest and largest Ritz values after
-          5 and 10 steps, and log10(r2_10 / r2_0) / 10
-Outputs (11): log p at the 9 Chebyshev points of [0, b], b = 1 + max
-Gershgorin radius (an upper bound of the spectrum of D^-1 A), then the log of
-the iteration counts predicted for Jacobi-PCG and for the polynomial PCG.
-The coefficients follow by the discrete Chebyshev transform, so p > 0 at
-those points by construction; the library checks p > 0 on all of [0, b]
-before use.  The iteration predictions, with the per-iteration costs timed
-on the device, decide whether the polynomial is used at all.
+The harvest also records 5 numbers of the first 10 FP64 Jacobi-PCG steps
+(the reference's loop, CG.cpp:284-385; features 37..41: log of the smallest
+and largest Ritz values after 10 and 5 steps, log10(r2_10 / r2_0) / 10):
+the fixed rule's theta10, not an input of the network.
+Outputs (2 D + 3, D = len(DEGREES)): log theta_min (the smallest eigenvalue
+of D^-1 A); for each candidate degree k the log of the mixed-precision PCG
+iterations with Chebyshev on [c_k theta_min, b] and the log of its time per
+iteration (ms) on this device; the log of Jacobi's iterations and of its
+time per iteration.  b = 1 + max Gershgorin radius bounds the spectrum from
+above; c_k (one per degree) is tuned on the training systems.
 
 Labels come from the spectrum the FP64 solver itself reveals: the Lanczos
 tridiagonal of a converged Jacobi-PCG run (its alpha/beta) gives Ritz nodes
 theta_i and Gauss weights w_i of the initial residual's spectral measure
-(Golub & Meurant, Matrices, Moments and Quadrature, 2010, ch. 6).  The
-polynomial-preconditioned PCG acts on the same eigenvectors with eigenvalues
-theta_i p(theta_i) and weights p(theta_i) w_i, so its convergence is
-simulated by CG on that diagonal system.  The label is the Chebyshev
-polynomial (on [a, b]) whose simulated iteration count is smallest over a
-grid of lower ends a.
+(Golub & Meurant, Matrices, Moments and Quadrature, 2010, ch. 6); its
+smallest node is theta_min.  PCG with the polynomial acts on the same
+eigenvectors with eigenvalues theta_i p(theta_i) and weights p(theta_i) w_i,
+so its iterations are those of CG on that diagonal system
+(simulate_iterations), which matched the device's iteration counts within
+one on 256^3 and eight family members for degrees 6-16.  The m-node measure
+of an m-step run reproduces the moments of a polynomial PCG run while
+iterations x (degree + 1) stay below about 2 m; far below the best lower
+end the simulated counts fall short of the true ones
+(tests/test_spectral_labels.py), so a label is exact where it matters, near
+the best interval.  The times per iteration are measured on the device when
+the system is harvested (cgxx_amd.harvest).  On this family the simulated
+cost is flat near its optimum: per system, the best degree and lower end
+save 0.2-0.3% against one tuned degree and ratio, so the network's gain is
+the spectral estimate it replaces, not a per-system choice of interval.
+
+Use (solve_learned): predicted cost = predicted iterations x predicted time
+per iteration, for Jacobi and every degree; the cheapest runs.  The
+polynomial gets an iteration budget worth 1.25x Jacobi's predicted cost and
+the library refuses a polynomial that is not positive on [0, b]; in either
+case Jacobi continues from the current x to the FP64 true residual.
 """
 from __future__ import annotations
 
@@ -38,16 +60,30 @@ import numpy as np
 
 from . import spectrum
 
+# Candidate degrees of the polynomial.  Lower ends of the Chebyshev
+# interval [a, b] searched: for the network, a = c theta_min with c on
+# LOWER_END_GRID (quarter octaves, 2^-2 .. 2^8); for the fixed rule,
+# a = theta10 r with r on RATIO_GRID (quarter octaves, 2^0.5 .. 2^-6).
+DEGREES = (4, 6, 8, 12, 16)
+LOWER_END_GRID = 2.0 ** (np.arange(-8, 33) / 4.0)
+RATIO_GRID = 2.0 ** (-np.arange(-2, 25) / 4.0)
+# the degree of the fixed rule before it is tuned on a training set
 DEGREE = 8
 N_STATS = 36
-N_FEATURES = N_STATS + 1 + 5
-# outputs: log p at the DEGREE + 1 Chebyshev points, then the log of the
-# predicted iteration counts of Jacobi-PCG and of the polynomial PCG
-DIMS = (N_FEATURES, 128, 128, DEGREE + 3)
+N_FEATURES = N_STATS + 1
+N_HARVESTED = N_FEATURES + 5
+N_OUT = 2 * len(DEGREES) + 3
+DIMS = (N_FEATURES, 128, 128, N_OUT)
 RITZ_STEPS = 10
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "poly_mlp.npz")
-# lower ends a = theta_1 * 2^(s/2), s = 0..A_STEPS-1, theta_1 = smallest Ritz value
-A_STEPS = 17
+
+
+def out_index(kind: str, j: int = 0) -> int:
+    """Position of an output: "theta_min"; "iters" or "ms" of degree
+    DEGREES[j]; "jacobi_iters" / "jacobi_ms"."""
+    D = len(DEGREES)
+    return {"theta_min": 0, "iters": 1 + j, "ms": 1 + D + j, "jacobi_iters": 1 + 2 * D,
+            "jacobi_ms": 2 + 2 * D}[kind]
 
 
 def init_params(dims, seed: int) -> np.ndarray:
@@ -60,33 +96,6 @@ def init_params(dims, seed: int) -> np.ndarray:
         parts.append(rng.uniform(-lim, lim, (fan_in, fan_out)).ravel())
         parts.append(np.zeros(fan_out))
     return np.concatenate(parts).astype(np.float32)
-
-
-# ---- coefficients <-> log p at Chebyshev points --------------------------
-
-def cheb_nodes(degree: int, b: float) -> np.ndarray:
-    y = np.cos(np.pi * (np.arange(degree + 1) + 0.5) / (degree + 1))
-    return (y + 1.0) * 0.5 * b
-
-
-def decode_coeffs(y, b: float) -> np.ndarray:
-    """Chebyshev coefficients on [0, b] of the degree-k polynomial with
-    p(node_j) = exp(y_j) (discrete Chebyshev transform)."""
-    y = np.asarray(y, dtype=np.float64)
-    k = y.size - 1
-    pv = np.exp(y)
-    ang = np.pi * (np.arange(k + 1) + 0.5) / (k + 1)
-    c = np.array([np.sum(pv * np.cos(m * ang)) for m in range(k + 1)]) * (2.0 / (k + 1))
-    c[0] *= 0.5
-    return c
-
-
-def encode_coeffs(c, b: float) -> np.ndarray:
-    lam = cheb_nodes(len(c) - 1, b)
-    pv = spectrum.poly_eval(c, b, lam)
-    if np.any(pv <= 0):
-        raise ValueError("polynomial not positive at the Chebyshev points")
-    return np.log(pv)
 
 
 # ---- features -------------------------------------------------------------
@@ -104,6 +113,7 @@ def cg_statistics(alpha, beta, r2) -> np.ndarray:
 
 
 def features(stats, n: int, alpha, beta, r2) -> np.ndarray:
+    """The 42 harvested features: 36 statistics, log N, cg_statistics."""
     stats = np.asarray(stats, np.float64)
     if stats.size != N_STATS:
         raise ValueError(f"expected {N_STATS} matrix statistics")
@@ -115,18 +125,34 @@ def spectrum_bound(stats) -> float:
     return 1.0 + float(stats[7])
 
 
-def system_features(ctx, n: int):
-    """(features, b, ritz10) of the system uploaded to a cghip Context (CRS,
-    Jacobi, K set): matrix statistics on the device and 10 FP64 Jacobi-PCG
-    steps from x = 0.  Overwrites X."""
-    stats = ctx.matrix_stats()
+def matrix_features(ctx, n: int):
+    """(features, b) from the matrix alone: the device matrix statistics and
+    log N (the network's inputs), and b = 1 + max Gershgorin radius."""
+    stats = np.asarray(ctx.matrix_stats(), np.float64)
+    if stats.size != N_STATS:
+        raise ValueError(f"expected {N_STATS} matrix statistics")
+    return np.concatenate([stats, [np.log(n)]]), spectrum_bound(stats)
+
+
+def ritz_features(ctx):
+    """(5 features, theta10) of 10 FP64 Jacobi-PCG steps from x = 0 on the
+    uploaded system (CRS, Jacobi, K set): what a rule without the network
+    needs for its interval.  Overwrites X."""
     ctx.fill("X", 0.0)
     ctx.solve(precond="jacobi", max_iter=RITZ_STEPS, tol=1e-300)
     a = ctx.history("alpha", RITZ_STEPS)
     bt = ctx.history("beta", RITZ_STEPS)
     r2 = ctx.history("r2", RITZ_STEPS + 1)
-    f = features(stats, n, a, bt, r2)
-    return f, spectrum_bound(stats), np.exp(f[37])
+    c = cg_statistics(a, bt, r2)
+    return c, float(np.exp(c[0]))
+
+
+def system_features(ctx, n: int):
+    """(all 42 harvested features, b, theta10): matrix_features then
+    ritz_features.  Overwrites X."""
+    f, b = matrix_features(ctx, n)
+    c, theta10 = ritz_features(ctx)
+    return np.concatenate([f, c]), b, theta10
 
 
 # ---- labels from the harvested spectrum ----------------------------------
@@ -168,36 +194,93 @@ def simulate_iterations(theta, w, pvals, tol=1e-9, max_iter=20000) -> int:
     return max_iter
 
 
-def candidate_lower_ends(theta_min: float, b: float) -> np.ndarray:
-    a = theta_min * 2.0 ** (0.5 * np.arange(A_STEPS))
-    return a[a < 0.5 * b]
+def grid_counts(theta, w, lower_ends, b: float, degree: int, tol: float = 1e-9) -> np.ndarray:
+    """Simulated PCG iterations (simulate_iterations) of the Chebyshev
+    polynomial of the given degree on [a, b] for every a of lower_ends
+    (capped at b / 4); -1 where the polynomial is not positive on the
+    nodes."""
+    its = np.full(len(lower_ends), -1)
+    for i, a in enumerate(lower_ends):
+        c = spectrum.cheb_residual_coeffs(min(a, 0.25 * b), b, degree)
+        pv = spectrum.poly_eval(c, b, theta)
+        if np.all(pv > 0):
+            its[i] = simulate_iterations(theta, w, pv, tol=tol)
+    return its
 
 
-def label(alpha, beta, b: float, degree: int = DEGREE):
-    """Best Chebyshev lower end on the harvested spectrum.
+def best_lower_end(theta, w, theta10: float, b: float, degree: int, tol: float = 1e-9):
+    """(ratio r, simulated iterations, counts on RATIO_GRID) of the
+    Chebyshev polynomial of the given degree on [theta10 r, b] with the
+    fewest simulated PCG iterations on the spectral measure (theta, w); -1
+    where the polynomial is not positive on the Ritz nodes.  Ties go to the
+    larger r."""
+    its = grid_counts(theta, w, theta10 * RATIO_GRID, b, degree, tol)
+    ok = its >= 0
+    if not ok.any():
+        raise ValueError("no positive Chebyshev polynomial on the grid")
+    i = int(np.argmin(np.where(ok, its, np.iinfo(its.dtype).max)))
+    return float(RATIO_GRID[i]), int(its[i]), its
 
-    Returns dict: a_best, y (log p at the Chebyshev points), the candidate
-    ends and their simulated iteration counts, and the simulated Jacobi
-    count (p = 1)."""
+
+def label_degrees(alpha, beta, theta10: float, b: float, degrees=DEGREES, tol: float = 1e-9) -> dict:
+    """Labels of one harvested system: for each degree the best lower-end
+    ratio relative to theta10, its simulated iterations and the counts on
+    the whole ratio grid (best_lower_end, for the fixed rule); the counts
+    with a = c theta_min on LOWER_END_GRID (lower_end_iters, for the
+    network); the simulated Jacobi iterations (p = 1); the extreme Ritz
+    values of the full run."""
     theta, w = spectral_measure(alpha, beta)
     theta = np.clip(theta, 1e-300, None)
-    cands = candidate_lower_ends(theta[0], b)
-    its = []
-    for a in cands:
-        c = spectrum.cheb_residual_coeffs(a, b, degree)
-        its.append(simulate_iterations(theta, w, spectrum.poly_eval(c, b, theta)))
-    its = np.array(its)
-    k = int(np.argmin(its))
-    c = spectrum.cheb_residual_coeffs(cands[k], b, degree)
-    return {
-        "a_best": float(cands[k]),
-        "y": encode_coeffs(c, b),
-        "cands": cands,
-        "iters": its,
-        "jacobi_iters": simulate_iterations(theta, w, np.ones_like(theta)),
-        "theta_min": float(theta[0]),
-        "theta_max": float(theta[-1]),
-    }
+    out = {"ratio": [], "iters": [], "grid_iters": [], "lower_end_iters": []}
+    for k in degrees:
+        r, it, its = best_lower_end(theta, w, theta10, b, k, tol)
+        out["ratio"].append(r)
+        out["iters"].append(it)
+        out["grid_iters"].append(its)
+        out["lower_end_iters"].append(grid_counts(theta, w, theta[0] * LOWER_END_GRID, b, k, tol))
+    out["jacobi_iters"] = simulate_iterations(theta, w, np.ones_like(theta), tol=tol)
+    out["theta_min"] = float(theta[0])
+    out["theta_max"] = float(theta[-1])
+    return out
+
+
+def _cost(iters, ms):
+    it = np.asarray(iters, np.float64).copy()
+    it[it < 0] = np.inf
+    return it * ms
+
+
+def tune_lower_ends(d: dict) -> np.ndarray:
+    """Per degree the index on LOWER_END_GRID of the multiplier c (a =
+    c theta_min) with the least summed cost (simulated iterations x measured
+    ms per iteration) over labelled systems."""
+    cost = _cost(d["lower_end_iters"], d["poly_ms_per_iter"][:, :, None]).sum(0)
+    return np.argmin(cost, axis=1)
+
+
+def targets(d: dict, lower_end_index) -> np.ndarray:
+    """Network targets of labelled systems (cgxx_amd.label output): log
+    theta_min; per degree log iterations at a = c_k theta_min (c_k =
+    LOWER_END_GRID[lower_end_index[k]]) and log ms per iteration; log Jacobi
+    iterations and log Jacobi ms per iteration."""
+    S = d["lower_end_iters"].shape[0]
+    it = np.stack([d["lower_end_iters"][:, j, g] for j, g in enumerate(lower_end_index)], 1)
+    if np.any(it <= 0):
+        raise ValueError("a labelled count at the tuned lower end is not positive")
+    return np.concatenate([np.log(d["theta_min"]).reshape(S, 1), np.log(it.astype(np.float64)),
+                           np.log(d["poly_ms_per_iter"]),
+                           np.log(d["jacobi_iters"].astype(np.float64))[:, None],
+                           np.log(d["jacobi_ms_per_iter"])[:, None]], axis=1)
+
+
+def tune_fixed_rule(d: dict) -> tuple[int, float]:
+    """The method without the network, tuned on labelled systems: the one
+    degree and ratio r (Chebyshev on [theta10 r, b]) with the least summed
+    cost (simulated iterations x measured ms per iteration).  Returns
+    (degree, rho = 1 / r)."""
+    cost = _cost(d["grid_iters"], d["poly_ms_per_iter"][:, :, None]).sum(0)
+    j, g = np.unravel_index(int(np.argmin(cost)), cost.shape)
+    return int(DEGREES[j]), float(1.0 / RATIO_GRID[g])
 
 
 # ---- model file and inference ---------------------------------------------
@@ -205,7 +288,7 @@ def label(alpha, beta, b: float, degree: int = DEGREE):
 def save_model(path, params, feat_mean, feat_std, out_mean, out_std, **meta):
     np.savez(path, params=np.asarray(params, np.float32), dims=np.array(DIMS),
              feat_mean=feat_mean, feat_std=feat_std, out_mean=out_mean, out_std=out_std,
-             degree=DEGREE, **{k: np.asarray(v) for k, v in meta.items()})
+             degrees=np.array(DEGREES), **{k: np.asarray(v) for k, v in meta.items()})
 
 
 def load_model(path=MODEL_PATH) -> dict:
@@ -214,52 +297,47 @@ def load_model(path=MODEL_PATH) -> dict:
 
 
 def normalize(f, model) -> np.ndarray:
-    """Standardised features, first clamped to the range seen in training
-    (a system outside the family, e.g. a larger grid, is mapped to its
-    nearest training edge instead of being extrapolated)."""
-    f = np.atleast_2d(np.asarray(f, np.float64))
-    if "feat_lo" in model:
-        f = np.clip(f, model["feat_lo"], model["feat_hi"])
+    """Standardised network inputs (the first N_FEATURES of f), first
+    clamped to the range seen in training (a system outside the family, e.g.
+    a larger grid, is mapped to its nearest training edge instead of being
+    extrapolated)."""
+    f = np.atleast_2d(np.asarray(f, np.float64))[:, :len(model["feat_mean"])]
+    f = np.clip(f, model["feat_lo"], model["feat_hi"])
     return ((f - model["feat_mean"]) / model["feat_std"]).astype(np.float32)
 
 
-def predict(mlp, model, f, b: float) -> dict:
-    """Network output for features f (on the GPU): the coefficients on
-    [0, b] and, for a model with 11 outputs, the predicted Jacobi and
-    polynomial iteration counts."""
-    yn = mlp.forward(normalize(f, model))[0].astype(np.float64)
-    y = yn * model["out_std"] + model["out_mean"]
-    k = int(model["degree"])
-    out = {"coeffs": decode_coeffs(y[: k + 1], b)}
-    if y.size >= k + 3:
-        out["it_jacobi"] = float(np.exp(y[k + 1]))
-        out["it_poly"] = float(np.exp(y[k + 2]))
+def decide(y, model) -> dict:
+    """The choice from de-standardised network outputs y (one system):
+    predicted theta_min, predicted cost (ms) of Jacobi and of every degree,
+    and the cheapest with its lower end a = c_k theta_min."""
+    degrees = [int(v) for v in model["degrees"]]
+    D = len(degrees)
+    cost = {"jacobi": float(np.exp(y[1 + 2 * D] + y[2 + 2 * D]))}
+    for j, k in enumerate(degrees):
+        cost[k] = float(np.exp(y[1 + j] + y[1 + D + j]))
+    best = min(cost, key=cost.get)
+    theta_min = float(np.exp(y[0]))
+    out = {"cost_ms": cost, "choice": best, "theta_min": theta_min,
+           "jacobi_ms_per_iter": float(np.exp(y[2 + 2 * D]))}
+    if best != "jacobi":
+        j = degrees.index(best)
+        out.update(degree=best, lower_end=float(model["lower_end"][j]) * theta_min,
+                   iters=float(np.exp(y[1 + j])), ms_per_iter=float(np.exp(y[1 + D + j])))
     return out
 
 
-def predict_coeffs(mlp, model, f, b: float) -> np.ndarray:
-    """Coefficients on [0, b] from features f, with the network on the GPU."""
-    return predict(mlp, model, f, b)["coeffs"]
+def predict(mlp, model, f) -> dict:
+    """Network outputs for features f (on the GPU), de-standardised, and the
+    decision (decide)."""
+    yn = mlp.forward(normalize(f, model))[0].astype(np.float64)
+    y = yn * model["out_std"] + model["out_mean"]
+    return dict(decide(y, model), y=y)
 
 
-def iteration_costs(ctx, coeffs, b: float) -> tuple[float, float] | None:
-    """Algorithmic bytes of one mixed-precision iteration with Jacobi and
-    with the polynomial on the uploaded system (the library's byte model,
-    cghip_mixed_iteration_bytes: its loop kernels stream at similar rates,
-    so bytes rank the two by time).  None if the library refuses the
-    polynomial."""
-    from .cghip import CGHIPError
-
-    try:
-        ctx.set_poly(coeffs, b)
-    except CGHIPError:
-        return None
-    return ctx.mixed_iteration_bytes("jacobi"), ctx.mixed_iteration_bytes("poly")
-
-
-def fixed_rule_coeffs(theta10: float, b: float, rho: float) -> np.ndarray:
+def fixed_rule_coeffs(theta10: float, b: float, rho: float, degree: int = None) -> np.ndarray:
     """The method without the network: Chebyshev on [theta_min(10) / rho, b]."""
-    return spectrum.cheb_residual_coeffs(min(theta10 / rho, 0.25 * b), b, DEGREE)
+    return spectrum.cheb_residual_coeffs(min(theta10 / rho, 0.25 * b), b,
+                                         DEGREE if degree is None else degree)
 
 
 def solve_poly_with_fallback(ctx, coeffs, b: float, tol: float = 1e-9, max_iter: int = 100000,
@@ -312,46 +390,54 @@ def solve_poly_with_fallback(ctx, coeffs, b: float, tol: float = 1e-9, max_iter:
     return res
 
 
-def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 100000,
-                  budget: int | None = None) -> dict:
-    """Features on the device, the network on the GPU, then the solve.
+def solve_learned(ctx, mlp, model, n: int, tol: float = 1e-9, max_iter: int = 100000) -> dict:
+    """Matrix statistics on the device, the network on the GPU, the cheapest
+    predicted method, then the solve.
 
-    With iteration predictions (11-output model) the predicted costs decide:
-    the polynomial runs only if (predicted polynomial iterations) x (its
-    bytes per iteration) is below the same product for Jacobi, and then with
-    an SpMV budget worth 1.25x Jacobi's predicted cost; past it, or if the
-    library refuses the polynomial, Jacobi takes over
-    (solve_poly_with_fallback).  Otherwise Jacobi runs from the start.
-    The per-iteration costs are the library's algorithmic bytes per
-    iteration (iteration_costs).  The wall time includes features and
-    inference."""
+    Jacobi runs from the start when its predicted cost is the lowest;
+    otherwise the polynomial of the chosen degree on [c_k theta_min, b]
+    (theta_min predicted) runs with an iteration budget worth 1.25x Jacobi's
+    predicted cost (solve_poly_with_fallback: past it, or if the library
+    refuses the polynomial, Jacobi takes over).  The wall time includes the
+    statistics and inference."""
     t0 = time.perf_counter()
-    f, b, _ = system_features(ctx, n)
-    pr = predict(mlp, model, f, b)
-    c = pr["coeffs"]
-    k = len(c) - 1
-    decision = None
-    if "it_jacobi" in pr and budget is None:
-        costs = iteration_costs(ctx, c, b)
-        if costs is not None:
-            bj, bp = costs
-            cost_j, cost_p = bj * pr["it_jacobi"], bp * pr["it_poly"]
-            decision = {"it_jacobi_pred": pr["it_jacobi"], "it_poly_pred": pr["it_poly"],
-                        "bytes_jacobi_iter": bj, "bytes_poly_iter": bp,
-                        "use_poly": cost_p < cost_j}
-            budget = int(np.ceil(1.25 * cost_j / bp)) * (k + 1)
-    if decision is not None and not decision["use_poly"]:
+    f, b = matrix_features(ctx, n)
+    pr = predict(mlp, model, f)
+    if pr["choice"] == "jacobi":
         ctx.fill("X", 0.0)
         info = ctx.solve_mixed(precond="jacobi", max_iter=max_iter, tol=tol)
         res = {"method": "jacobi-selected", "poly_iterations": 0,
                "jacobi_iterations": info["iterations"], "updates": info["updates"],
                "true_residual": info["true_residual"], "spmvs": info["iterations"]}
+        c = None
     else:
-        res = solve_poly_with_fallback(ctx, c, b, tol, max_iter, budget)
+        k = pr["degree"]
+        c = spectrum.cheb_residual_coeffs(min(pr["lower_end"], 0.25 * b), b, k)
+        budget_iters = int(np.ceil(1.25 * pr["cost_ms"]["jacobi"] / pr["ms_per_iter"]))
+        res = solve_poly_with_fallback(ctx, c, b, tol, max_iter, budget=budget_iters * (k + 1))
     res["wall_s"] = time.perf_counter() - t0
     res["coeffs"] = c
     res["b"] = b
-    res["decision"] = decision
+    res["decision"] = {key: pr[key] for key in pr if key != "y"}
+    return res
+
+
+def solve_fixed_rule(ctx, n: int, rho: float, degree: int, tol: float = 1e-9,
+                     max_iter: int = 100000) -> dict:
+    """The method without the network: b from the matrix statistics,
+    theta10 from 10 FP64 Jacobi-PCG steps, then Chebyshev of the given
+    degree on [theta10 / rho, b] (solve_poly_with_fallback).  The wall time
+    includes both spectral estimates."""
+    t0 = time.perf_counter()
+    _, b = matrix_features(ctx, n)
+    _, theta10 = ritz_features(ctx)
+    c = fixed_rule_coeffs(theta10, b, rho, degree)
+    res = solve_poly_with_fallback(ctx, c, b, tol, max_iter)
+    res["spmvs"] += RITZ_STEPS  # the FP64 steps of the estimate
+    res["wall_s"] = time.perf_counter() - t0
+    res["coeffs"] = c
+    res["b"] = b
+    res["theta10"] = theta10
     return res
 
 
@@ -360,15 +446,18 @@ BIN_MAGIC = b"CGHIPMLP"
 
 def export_bin(model: dict, path: str) -> None:
     """Writes the model for the C++ host (cgxx_amd/host/LearnedPoly.cpp), all
-    little-endian: magic "CGHIPMLP", int32 version 1, int32 in, h1, h2, out,
-    degree, float64 rho_star, float64 feat_mean[in], feat_std[in],
+    little-endian: magic "CGHIPMLP", int32 version 3, int32 in, h1, h2, out,
+    int32 D, int32 degrees[D], int32 fixed degree, float64 fixed rho,
+    float64 lower_end[D] (c_k), float64 feat_mean[in], feat_std[in],
     feat_lo[in], feat_hi[in], out_mean[out], out_std[out], int32 nparams,
     float32 params[nparams]."""
     dims = [int(v) for v in model["dims"]]
+    degrees = [int(v) for v in model["degrees"]]
     with open(path, "wb") as f:
         f.write(BIN_MAGIC)
-        np.array([1, *dims, int(model["degree"])], "<i4").tofile(f)
+        np.array([3, *dims, len(degrees), *degrees, int(model["fixed_degree"])], "<i4").tofile(f)
         np.array([float(model["rho_star"])], "<f8").tofile(f)
+        np.asarray(model["lower_end"], "<f8").tofile(f)
         for key in ("feat_mean", "feat_std", "feat_lo", "feat_hi", "out_mean", "out_std"):
             np.asarray(model[key], "<f8").tofile(f)
         params = np.asarray(model["params"], "<f4")

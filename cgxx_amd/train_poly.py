@@ -1,14 +1,19 @@
-"""Train the coefficient network on harvested data, on the GPU.
+"""Train the learned preconditioner's network on labelled systems, on the GPU.
 
-    python -m cgxx_amd.train_poly --data train.npz [--heldout heldout.npz]
+    python -m cgxx_amd.train_poly --data labels_train.npz [...] [--heldout labels_heldout.npz ...]
         [--out cgxx_amd/data/poly_mlp.npz] [--steps 6000]
 
-Inputs are the harvest files of cgxx_amd.harvest.  Features and targets
-(log p at the Chebyshev points of the best Chebyshev polynomial) are
-standardised; the network (learned.DIMS) is trained by full-batch Adam with
-the library's HIP kernels.  Also picks rho*, the single ratio of the fixed
-rule a = theta_min(10 steps) / rho with the fewest simulated iterations
-over the training systems: the method without the network.
+Inputs are the label files of cgxx_amd.label.  First the per-degree lower
+ends a = c_k theta_min with the least summed cost on the training systems
+(learned.tune_lower_ends).  Inputs (the 37 matrix features) and targets
+(learned.targets: log theta_min; per candidate degree the log of the
+iterations at c_k theta_min and of the device's ms per iteration; the log
+of Jacobi's iterations and ms per iteration) are standardised; the network
+(learned.DIMS) is trained by full-batch Adam with the library's HIP
+kernels.  Also tunes the method without the network on the same systems:
+the one degree and ratio to theta10 with the least summed cost
+(learned.tune_fixed_rule).  Prints a JSON summary with, for the held-out
+files, the RMSE of each output group against predicting the training mean.
 """
 from __future__ import annotations
 
@@ -22,28 +27,25 @@ import numpy as np
 
 from . import learned
 
-
-def load(path):
-    with np.load(path, allow_pickle=False) as z:
-        return {k: z[k] for k in z.files}
+FIELDS = ("features", "grid_iters", "lower_end_iters", "poly_ms_per_iter", "jacobi_iters",
+          "jacobi_ms_per_iter", "theta_min", "n", "seed")
 
 
-def targets(d) -> np.ndarray:
-    """log p at the Chebyshev points of the best Chebyshev polynomial, the
-    log of the FP64 Jacobi-PCG iteration count, the log of the best
-    simulated polynomial iteration count."""
-    it = d["sim_iters"].astype(np.float64)
-    it[it < 0] = np.inf
-    return np.concatenate([d["y"].astype(np.float64),
-                           np.log(d["fp64_iterations"].astype(np.float64))[:, None],
-                           np.log(it.min(1))[:, None]], axis=1)
+def load(paths) -> dict:
+    parts = []
+    for p in paths:
+        with np.load(p, allow_pickle=False) as z:
+            parts.append({k: z[k] for k in FIELDS})
+    return {k: np.concatenate([p[k] for p in parts]) for k in FIELDS}
 
 
-def best_rho(d) -> float:
-    it = d["rho_iters"].astype(np.float64)
-    it[it < 0] = np.inf
-    tot = it.sum(0)
-    return float(d["rho"][int(np.argmin(tot))])
+D = len(learned.DEGREES)
+GROUPS = {"theta_min": slice(0, 1), "iters": slice(1, 1 + D), "ms_per_iter": slice(1 + D, 1 + 2 * D),
+          "jacobi": slice(1 + 2 * D, 3 + 2 * D)}
+
+
+def group_rmse(pred, Y) -> dict:
+    return {g: float(np.sqrt(np.mean((pred[:, s] - Y[:, s]) ** 2))) for g, s in GROUPS.items()}
 
 
 def main() -> int:
@@ -56,18 +58,17 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
 
-    parts = [load(p) for p in args.data]
-    d = {k: np.concatenate([p[k] for p in parts])
-         for k in ("features", "y", "rho_iters", "fp64_iterations", "sim_iters")}
-    d["rho"] = parts[0]["rho"]
-    X = d["features"].astype(np.float64)
-    Y = targets(d)
+    d = load(args.data)
+    X = d["features"][:, :learned.N_FEATURES].astype(np.float64)
+    lower = learned.tune_lower_ends(d)
+    Y = learned.targets(d, lower)
     fm, fs = X.mean(0), X.std(0)
     fs = np.where(fs < 1e-6, 1.0, fs)
     om, os_ = Y.mean(0), Y.std(0)
     os_ = np.where(os_ < 1e-6, 1.0, os_)
     Xn = ((X - fm) / fs).astype(np.float32)
     Yn = ((Y - om) / os_).astype(np.float32)
+    fixed_degree, rho_star = learned.tune_fixed_rule(d)
 
     from .cghip import Mlp
 
@@ -81,25 +82,21 @@ def main() -> int:
     params = mlp.get_params()
     train_s = time.perf_counter() - t0
     pred = mlp.forward(Xn).astype(np.float64) * os_ + om
-    train_rmse = float(np.sqrt(np.mean((pred[:, :learned.DEGREE + 1] - Y[:, :learned.DEGREE + 1]) ** 2)))
     summary = {"n_train": int(X.shape[0]), "steps": args.steps, "lr": args.lr,
                "train_seconds": train_s, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
-               "train_rmse_log_p": train_rmse, "rho_star": best_rho(d)}
+               "train_rmse": group_rmse(pred, Y), "fixed_degree": fixed_degree, "rho_star": rho_star,
+               "lower_end": {int(k): float(learned.LOWER_END_GRID[g]) for k, g in zip(learned.DEGREES, lower)}}
     if args.heldout:
-        hp = [load(p) for p in args.heldout]
-        Xh = np.concatenate([p["features"] for p in hp]).astype(np.float64)
-        Yh = targets({k: np.concatenate([p[k] for p in hp])
-                      for k in ("y", "fp64_iterations", "sim_iters")})
-        Xc = np.clip(Xh, X.min(0), X.max(0))  # as inference clamps to the training range
+        h = load(args.heldout)
+        Yh = learned.targets(h, lower)
+        Xc = np.clip(h["features"][:, :learned.N_FEATURES].astype(np.float64), X.min(0), X.max(0))  # as inference clamps
         ph = mlp.forward(((Xc - fm) / fs).astype(np.float32)).astype(np.float64) * os_ + om
-        summary["n_heldout"] = int(Xh.shape[0])
-        k = learned.DEGREE + 1
-        summary["heldout_rmse_log_p"] = float(np.sqrt(np.mean((ph[:, :k] - Yh[:, :k]) ** 2)))
-        summary["heldout_rmse_mean_predictor"] = float(np.sqrt(np.mean((om[:k] - Yh[:, :k]) ** 2)))
-        summary["heldout_iterations_rel_err"] = [
-            float(np.mean(np.abs(np.exp(ph[:, k + i] - Yh[:, k + i]) - 1))) for i in (0, 1)]
+        summary["n_heldout"] = int(Yh.shape[0])
+        summary["heldout_rmse"] = group_rmse(ph, Yh)
+        summary["heldout_rmse_mean_predictor"] = group_rmse(np.broadcast_to(om, Yh.shape), Yh)
     mlp.close()
-    learned.save_model(args.out, params, fm, fs, om, os_, rho_star=summary["rho_star"],
+    learned.save_model(args.out, params, fm, fs, om, os_, rho_star=rho_star, fixed_degree=fixed_degree,
+                       lower_end=learned.LOWER_END_GRID[lower], lower_end_index=lower,
                        feat_lo=X.min(0), feat_hi=X.max(0),
                        loss_history=hist[:: max(1, len(hist) // 200)])
     learned.export_bin(learned.load_model(args.out), os.path.splitext(args.out)[0] + ".bin")

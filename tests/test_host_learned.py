@@ -57,10 +57,52 @@ def test_model_file_round_trip(tool, tmp_path):
     out = run(tool, "model", os.path.join(REPO, "cgxx_amd", "data", "poly_mlp.bin"))
     model = learned.load_model()
     dims = [int(v) for v in model["dims"]]
-    assert [int(v) for v in out[:5]] == dims + [int(model["degree"])]
-    assert int(out[5]) == model["params"].size
-    assert float(out[6]) == float(model["rho_star"])
+    assert [int(v) for v in out[:4]] == dims
+    assert int(out[4]) == len(model["degrees"]) and int(out[5]) == int(model["fixed_degree"])
+    assert int(out[6]) == model["params"].size
+    assert float(out[7]) == float(model["rho_star"])
+    assert [int(v) for v in out[8::2]] == [int(v) for v in model["degrees"]]
+    assert [float(v) for v in out[9::2]] == [float(v) for v in model["lower_end"]]
     bad = tmp_path / "bad.bin"
     bad.write_bytes(open(os.path.join(REPO, "cgxx_amd", "data", "poly_mlp.bin"), "rb").read()[:100])
     r = subprocess.run([tool, "model", str(bad)], capture_output=True, text=True)
     assert r.returncode == 1 and "truncated" in r.stdout
+
+
+def _synthetic_model(rng):
+    D = len(learned.DEGREES)
+    out = learned.N_OUT
+    F = learned.N_FEATURES
+    return {"dims": np.array(learned.DIMS), "degrees": np.array(learned.DEGREES), "fixed_degree": 8,
+            "rho_star": 16.0, "lower_end": rng.uniform(2.0, 16.0, D),
+            "feat_mean": np.zeros(F), "feat_std": np.ones(F), "feat_lo": -np.ones(F), "feat_hi": np.ones(F),
+            "out_mean": np.concatenate([[rng.uniform(-8, -4)], rng.uniform(3, 6, D), rng.uniform(-3, 0, D),
+                                        [rng.uniform(5, 8), rng.uniform(-3, -1)]]),
+            "out_std": rng.uniform(0.1, 1.0, out),
+            "params": np.zeros(sum(a * b + b for a, b in zip(learned.DIMS[:-1], learned.DIMS[1:])),
+                               np.float32)}
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_decision_matches_python(tool, seed, tmp_path):
+    """The C++ driver's choice (LearnedPoly.cpp decide) equals learned.decide
+    on the same standardised outputs: the cheapest of Jacobi and each degree
+    by predicted iterations x predicted ms per iteration, and the lower end
+    c_k theta_min."""
+    rng = np.random.default_rng(seed)
+    model = _synthetic_model(rng)
+    path = str(tmp_path / "m.bin")
+    learned.export_bin(model, path)
+    for _ in range(20):
+        yn = rng.standard_normal(learned.N_OUT).astype(np.float32)
+        got = run(tool, "decide", path, *[repr(float(v)) for v in yn])
+        want = learned.decide(yn.astype(np.float64) * model["out_std"] + model["out_mean"], model)
+        deg = int(got[0])
+        assert deg == (0 if want["choice"] == "jacobi" else want["degree"])
+        assert float(got[3]) == pytest.approx(want["cost_ms"]["jacobi"], rel=1e-6)
+        assert float(got[4]) == pytest.approx(want["theta_min"], rel=1e-6)
+        costs = [float(v) for v in got[5:]]
+        assert costs == pytest.approx([want["cost_ms"][k] for k in learned.DEGREES], rel=1e-6)
+        if deg:
+            assert float(got[1]) == pytest.approx(want["lower_end"], rel=1e-6)
+            assert float(got[2]) == pytest.approx(want["ms_per_iter"], rel=1e-6)

@@ -79,16 +79,3 @@ def test_adam_descends_quadratic():
     for t in range(1, 3001):
         p, m, v = mo.adam_step(p, m, v, 2 * (p - c), t, lr=1e-2)
     np.testing.assert_allclose(p, c, atol=1e-3)
-
-
-def test_coefficient_decoding_positive_at_nodes():
-    """Network outputs are log p at the Chebyshev points of [0, b]; the
-    decoded coefficients reproduce those values."""
-    y = np.array([2.0, 1.5, 1.0, 0.7, 0.2, -0.1, -0.3, -0.5, -0.6])
-    b = 2.0
-    c = learned.decode_coeffs(y, b)
-    from cgxx_amd import spectrum
-
-    lam = learned.cheb_nodes(len(y) - 1, b)
-    np.testing.assert_allclose(spectrum.poly_eval(c, b, lam), np.exp(y), rtol=1e-12)
-    np.testing.assert_allclose(learned.encode_coeffs(c, b), y, rtol=1e-12)
