@@ -55,58 +55,237 @@ def ell_bytes(n: int, nnz: int, value_bytes: int, vec_bytes: int) -> int:
     return value_bytes * nnz + 4 * nnz + 4 * n + 2 * vec_bytes * n
 
 
-def traffic_from_profiles(name: str):
-    """HBM bytes per launch of a kernel from the committed PMC summary."""
+def traffic_from_profiles(prefix: str):
+    """HBM bytes per launch of the first kernel whose name starts with
+    prefix, from the committed PMC summary."""
     path = os.path.join(REPO, "profiles", "traffic_c4.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        k = json.load(f).get("kernels", {}).get(name)
-    return None if k is None else k.get("bytes_per_launch")
+        kernels = json.load(f).get("kernels", {})
+    for name, k in kernels.items():
+        if name.startswith(prefix):
+            return k.get("bytes_per_launch")
+    return None
+
+
+def kernel_short_name(name: str) -> str:
+    """'void cghip::(anonymous namespace)::k_x<a, b>(args...)' -> 'k_x<a, b>'."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    name = name.replace("cghip::(anonymous namespace)::", "").replace("cghip::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def algorithmic_bytes(short: str, meta: dict):
+    """Bytes a launch must move at least (each array once, x once per SpMV)."""
+    n, nnz, D = meta["n"], meta["nnz"], meta["ndiag"]
+    table = [("k_pcg_matvec_dot_crs", crs_bytes(n, nnz, 8, 8)), ("k_matvec_crs<double", crs_bytes(n, nnz, 8, 8)),
+             ("k_pcg_matvec_dot_ell", ell_bytes(n, nnz, 8, 8)), ("k_matvec_ell<double", ell_bytes(n, nnz, 8, 8)),
+             ("k_pcg_update<true>", 32 * n), ("k_pcg_update<false>", 24 * n), ("k_pcg_xpay<true>", 48 * n),
+             ("k_pcg_xpay<false>", 40 * n), ("k_mp_matvec_dot<2>", 4 * D * n + n + 8 * n),
+             ("k_mp_update", 12 * n), ("k_mp_xpay_poly", 20 * n), ("k_mp_xpay", 20 * n),
+             ("k_poly_step<false, 2, 1,", 2 * D * n + n + 8 * n),
+             ("k_poly_step<false, 2, 2,", 2 * D * n + n + 12 * n),
+             ("k_poly_step<false, 2, 0,", 2 * D * n + n + 16 * n),
+             ("k_poly_step<true, 2, 0,", 2 * D * n + n + 16 * n)]
+    for prefix, by in table:
+        if short.startswith(prefix):
+            return by
+    return None
+
+
+def summarize_trace(trace_dir: str, bench_json: str, out_stem: str) -> int:
+    """profiles/<out_stem>.md from a rocprofv3 --kernel-trace --stats run of
+    bench.py (csv output): per kernel the launches and average, min and max
+    duration, and for the kernels with a byte model the algorithmic GB/s and
+    fraction of the 8 TB/s HBM peak (sizes from bench_json, the bench line
+    of a run on the same workload)."""
+    import csv
+    import glob
+
+    with open(bench_json) as f:
+        line = [ln for ln in f if ln.strip().startswith("{")][-1]
+    cfg = json.loads(line)["config"]
+    fmt = cfg.get("format", "")
+    ndiag = int(fmt.split("(")[1].split()[0]) if "diagonals" in fmt else 0
+    meta = {"n": cfg["n"], "nnz": cfg["nnz"], "ndiag": ndiag}
+    rows = []
+    for path in glob.glob(os.path.join(trace_dir, "**", "*kernel_stats.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                rows.append(row)
+    if not rows:
+        raise SystemExit("no kernel_stats.csv under " + trace_dir)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = []
+    for r in rows:
+        name = kernel_short_name(r["Name"])
+        avg = float(r["AverageNs"]) / 1e3
+        alg = algorithmic_bytes(name, meta)
+        rate = f"{alg / (avg * 1e-6) / 1e9:.0f} | {alg / (avg * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f}" \
+            if alg else "— | —"
+        lines.append(f"| {name} | {r['Calls']} | {avg:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} | {rate} |")
+    out = os.path.join(REPO, "profiles", out_stem + ".md")
+    with open(out, "w") as f:
+        f.write(f"# Kernel trace: {cfg['workload']}\n\n"
+                "Commands (from the repository root):\n\n"
+                "    rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/ktrace -o kt -- "
+                "python3 bench.py --trace > gpurun_out/trace.json\n"
+                f"    python3 bench.py --summarize-trace gpurun_out/ktrace gpurun_out/trace.json {out_stem}\n\n"
+                f"N = {meta['n']:,}, nnz = {meta['nnz']:,}, {ndiag} diagonals in the scaled FP32 operator. "
+                "`bench.py --trace` launches the bench's kernels directly, not from HIP graphs: 100 FP64 "
+                "Jacobi-PCG iterations in CRS and in ELL, 20 plain SpMVs each, 100 mixed Jacobi and 20 "
+                "mixed polynomial iterations (the network's degree), kernel by kernel, and 20 "
+                "back-to-back polynomial applications. Durations in microseconds over every launch; "
+                "GB/s = algorithmic bytes per launch / average duration.\n\n"
+                "| kernel | launches | avg us | min us | max us | % of GPU time | algorithmic GB/s | "
+                "of 8 TB/s |\n|---|---|---|---|---|---|---|---|\n" + "\n".join(lines) + "\n")
+    print(f"wrote {out} ({len(rows)} kernels)")
+    return 0
+
+
+def summarize_pmc(fetch_dir: str, write_dir: str, meta_path: str, out_stem: str) -> int:
+    """profiles/<out_stem>.json and .md from two rocprofv3 --pmc runs of
+    `bench.py --pmc` (FETCH_SIZE, then WRITE_SIZE; csv output): per kernel
+    the median counter per launch in KiB, scaled by what the counters read
+    for the D2D copy of 8N bytes (__amd_rocclr_copyBuffer), against the
+    kernel's algorithmic bytes."""
+    import csv
+    import glob
+    import statistics
+
+    with open(meta_path) as f:
+        meta = json.load(f)
+
+    def load(d, counter):
+        vals = {}
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    vals.setdefault(kernel_short_name(row["Kernel_Name"]), []).append(
+                        float(row["Counter_Value"]))
+        return vals
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k]
+    if not copy:
+        raise SystemExit("no copy kernel in the FETCH_SIZE run")
+    # the runtime's copy kernel also moves small buffers: the 8N copy is its
+    # largest launch
+    ref = 8 * meta["n"]
+    copy_f, copy_w = max(fetch[copy[0]]), max(write[copy[0]])
+    cal_f = copy_f * 1024 / ref
+    cal_w = copy_w * 1024 / ref
+    kernels, rows = {}, []
+    for name in fetch:
+        if name not in write or "copyBuffer" in name:
+            continue
+        alg = algorithmic_bytes(name, meta)
+        if alg is None:
+            continue
+        fk, wk = statistics.median(fetch[name]), statistics.median(write[name])
+        rd, wr = fk * 1024 / cal_f, wk * 1024 / cal_w
+        kernels[name] = {"bytes_read": rd, "bytes_written": wr, "bytes_per_launch": rd + wr,
+                         "algorithmic": alg, "launches": len(fetch[name])}
+        rows.append(f"| {name} | {len(fetch[name])} | {fk:.1f} | {wk:.1f} | {rd:.4g} | {wr:.4g} | "
+                    f"{(rd + wr) / alg:.3f} |")
+    out = os.path.join(REPO, "profiles", out_stem)
+    with open(out + ".json", "w") as f:
+        json.dump({"source": f"profiles/{out_stem}.md", "calibration_fetch": cal_f,
+                   "calibration_write": cal_w, "meta": meta, "kernels": kernels}, f, indent=1)
+    with open(out + ".md", "w") as f:
+        f.write(f"# HBM traffic per kernel (PMC): {meta['side']}^3 diffusion, bench kernels\n\n"
+                "Commands (from the repository root; one counter per pass):\n\n"
+                "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o f -- "
+                "python3 bench.py --pmc\n"
+                "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o w -- "
+                "python3 bench.py --pmc\n"
+                f"    python3 bench.py --summarize-pmc gpurun_out/pmc_fetch gpurun_out/pmc_write "
+                f"gpurun_out/pmc_meta.json {out_stem}\n\n"
+                f"Calibration: the D2D copy of 8N = {ref:,} bytes (the largest launch of the "
+                f"runtime's copy kernel) reads FETCH_SIZE = {copy_f:.0f} KiB and WRITE_SIZE = "
+                f"{copy_w:.0f} KiB, so the counters are scaled by "
+                f"1/{cal_f:.4f} and 1/{cal_w:.4f}. Medians over the launches. N = {meta['n']:,}, "
+                f"nnz = {meta['nnz']:,}, {meta['ndiag']} diagonals in the scaled FP32 operator "
+                f"({meta['layout']}).\n\n"
+                "| kernel | launches | FETCH_SIZE KiB | WRITE_SIZE KiB | bytes read | bytes written | "
+                "traffic / algorithmic |\n|---|---|---|---|---|---|---|\n" + "\n".join(rows) + "\n")
+    print(f"wrote {out}.json and .md ({len(kernels)} kernels)")
+    return 0
 
 
 def application_traffic(kind: int, degree: int):
     """PMC bytes per Clenshaw step averaged over one application: the first
-    step (template argument 1), the second (2), the plain ones (0) and the
-    last (final, 0)."""
-    first = traffic_from_profiles(f"k_poly_step<false, {kind}, 1>")
-    second = traffic_from_profiles(f"k_poly_step<false, {kind}, 2>")
-    plain = traffic_from_profiles(f"k_poly_step<false, {kind}, 0>")
-    last = traffic_from_profiles(f"k_poly_step<true, {kind}, 0>")
+    step (template arguments <false, kind, 1, ...>), the second (2), the
+    plain ones (0) and the last (final, 0)."""
+    first = traffic_from_profiles(f"k_poly_step<false, {kind}, 1,")
+    second = traffic_from_profiles(f"k_poly_step<false, {kind}, 2,")
+    plain = traffic_from_profiles(f"k_poly_step<false, {kind}, 0,")
+    last = traffic_from_profiles(f"k_poly_step<true, {kind}, 0,")
     if degree < 3 or None in (first, second, plain, last):
         return None
     return (first + second + (degree - 3) * plain + last) / degree
 
 
+def cpu_threads() -> int:
+    """Threads for the OpenMP port: the CPU share this process is given
+    (OMP_NUM_THREADS when set, as on a leased GPU machine), else every CPU
+    it may run on."""
+    try:
+        return max(1, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(m, k, c, iterations: int, sample_iters: int) -> dict:
-    """The oracle's ports of the reference's CPU solvers on this host: the
-    OpenMP loop (CGOpenMP.cpp:188-265, order 3) on up to 16 threads, and
-    SerialCG (order 0) on one core, each over `sample_iters` iterations of
-    the workload and scaled to the FP64 iteration count."""
+    """The oracle's ports of the reference's CPU solvers on this host: one
+    full solve of the OpenMP loop (CGOpenMP.cpp:188-265, order 3) on the
+    process's CPU share, threads bound to cores (OMP_PROC_BIND=close,
+    OMP_PLACES=cores), to the same stop test; and SerialCG (order 0) on one
+    core over `sample_iters` iterations, scaled to the FP64 iteration count
+    (a full serial solve takes minutes)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle
 
-    def run(order, threads, iters):
-        t0 = time.perf_counter()
-        oracle.pcg(n=m.n, fmt="CRS", precond="jacobi", order=order, nthreads=threads, k=k,
-                   tol=1e-300, max_iter=iters, crs=(m.ptr, m.index, m.value), C_jac=c,
-                   history=False)
-        dt = time.perf_counter() - t0
-        return dt, dt / (iters + 1) * (iterations + 1)  # + the set-up pass
-
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    dt_omp, v_omp = run(3, threads, 4 * sample_iters)
-    dt_ser, v_ser = run(0, 1, sample_iters)
+    threads = cpu_threads()
+    host = host_info()
+    affinity = os.sched_getaffinity(0)
+    t0 = time.perf_counter()
+    _, it, res, _ = oracle.pcg(n=m.n, fmt="CRS", precond="jacobi", order=3, nthreads=threads, k=k,
+                               tol=1e-9, max_iter=100000, crs=(m.ptr, m.index, m.value), C_jac=c,
+                               history=False)
+    v_omp = time.perf_counter() - t0
+    # with OMP_PROC_BIND the OpenMP runtime binds this thread to one core
+    os.sched_setaffinity(0, affinity)
+    t0 = time.perf_counter()
+    oracle.pcg(n=m.n, fmt="CRS", precond="jacobi", order=0, nthreads=1, k=k, tol=1e-300,
+               max_iter=sample_iters, crs=(m.ptr, m.index, m.value), C_jac=c, history=False)
+    dt_ser = time.perf_counter() - t0
+    v_ser = dt_ser / (sample_iters + 1) * (iterations + 1)  # + the set-up pass
     return {
         "value": v_omp,
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"OpenMP port of CGOpenMP (oracle/cg_oracle.c, order 3, {threads} threads), FP64 "
-                   f"Jacobi-PCG on this workload: {4 * sample_iters} iterations in {dt_omp:.2f} s, "
-                   f"scaled to the {iterations} iterations of the FP64 solve"),
-        "serial": {"value": v_ser, "cores": 1,
+        "sample": (f"one full FP64 Jacobi-PCG solve of this workload by the OpenMP port of CGOpenMP "
+                   f"(oracle/cg_oracle.c, order 3) on {threads} threads bound to cores: {it} "
+                   f"iterations, relative residual {res:.3e}"),
+        "host": host,
+        "serial": {"value": v_ser, "cores": 1, "kind": "port", "scaled_sample": True,
                    "sample": f"SerialCG port (order 0): {sample_iters} iterations in {dt_ser:.2f} s, "
-                             f"scaled likewise"},
+                             f"scaled to the {iterations} iterations of the FP64 solve"},
     }
 
 
@@ -122,6 +301,12 @@ def main() -> int:
                     help="short run for rocprofv3 --pmc: one D2D copy of a vector (the "
                          "calibration kernel of known size), 20 FP64 CRS, 20 FP64 ELL, 20 mixed "
                          "Jacobi and 5 mixed polynomial iterations")
+    ap.add_argument("--trace", action="store_true",
+                    help="launch the bench's kernels directly (no HIP graph replays) for a kernel trace")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "META", "STEM"),
+                    help="write profiles/STEM.json/.md from two rocprofv3 --pmc runs of --pmc")
+    ap.add_argument("--summarize-trace", nargs=3, metavar=("TRACE_DIR", "BENCH_JSON", "STEM"),
+                    help="write profiles/STEM.md from a rocprofv3 --kernel-trace --stats run")
     ap.add_argument("--workload", choices=("diffusion256", "poisson128", "executables"),
                     default="diffusion256",
                     help="poisson128: the plain FP64 path on the 128^3 7-point Poisson system "
@@ -135,6 +320,10 @@ def main() -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         return 2
+    if args.summarize_pmc:
+        return summarize_pmc(*args.summarize_pmc)
+    if args.summarize_trace:
+        return summarize_trace(*args.summarize_trace)
     if args.workload == "poisson128":
         return poisson128(args)
     if args.workload == "executables":
@@ -153,11 +342,16 @@ def main() -> int:
     log(f"generated {args.side}^3 diffusion: N={n} nnz={nnz} in {time.perf_counter() - t:.1f} s")
 
     ctx = Context(0)
+    setup = {}
+    t = time.perf_counter()
     ctx.upload_crs(m.ptr, m.index, m.value)
+    setup["upload_crs_s"] = time.perf_counter() - t  # host validation + H2D
     ctx.upload_ell(*m.to_ell())
     ctx.upload_jacobi(c)
     ctx.set_format("CRS")
+    t = time.perf_counter()
     ctx.prepare_fp32()
+    setup["prepare_fp32_s"] = time.perf_counter() - t  # FP32 copy, scaled DIA/ELL/CRS operator
     ctx.upload("K", k)
     nnz_nd = nnz - n  # the scaled operator stores no diagonal
     layout = ctx.fp32_layout()
@@ -178,8 +372,41 @@ def main() -> int:
     mlp = Mlp(tuple(int(v) for v in model["dims"]), 1)
     mlp.set_params(model["params"])
     rho = float(model["rho_star"])
+    fixed_degree = int(model["fixed_degree"])
 
     max_iter = 100000
+    if args.trace:
+        # the kernels of the bench, each launched directly (no graph
+        # replays), for a profiler's kernel trace
+        for fmt in ("CRS", "ELL"):
+            ctx.set_format(fmt)
+            ctx.fill("X", 0.0)
+            ctx.solve(precond="jacobi", max_iter=100, tol=1e-9, profile=1)
+            ctx.fill("P", 1.0)
+            ctx.time_matvec("P", "Q", reps=20)
+        ctx.set_format("CRS")
+        ctx.fill("X", 0.0)
+        ctx.solve_mixed(precond="jacobi", max_iter=100, tol=1e-9, profile=1)
+        f, b = learned.matrix_features(ctx, n)
+        pr = learned.predict(mlp, model, f)
+        k_deg = pr["degree"] if pr["choice"] != "jacobi" else fixed_degree
+        from cgxx_amd import spectrum
+        lower = pr["lower_end"] if pr["choice"] != "jacobi" else learned.ritz_features(ctx)[1] / rho
+        ctx.set_poly(spectrum.cheb_residual_coeffs(min(lower, 0.25 * b), b, k_deg), b)
+        ctx.fill("X", 0.0)
+        ctx.solve_mixed(precond="poly", max_iter=20, tol=1e-9, profile=1)
+        step_ms = ctx.time_poly_step(reps=20)
+        log(f"trace run: 100 FP64 CRS and ELL, 20+20 plain SpMVs, 100 mixed Jacobi and 20 mixed "
+            f"degree-{k_deg} polynomial iterations kernel by kernel, 20 applications; Clenshaw step "
+            f"{step_ms:.4f} ms")
+        print(json.dumps({"config": {"workload": f"3D 7-point variable-coefficient diffusion {args.side}^3",
+                                     "n": n, "nnz": nnz,
+                                     "format": f"scaled FP32 operator in {layout}"
+                                               + (f" ({ndiag} diagonals)" if layout == "DIA" else "")},
+                          "degree": k_deg, "clenshaw_step_ms": step_ms}), flush=True)
+        mlp.close()
+        ctx.close()
+        return 0
     if args.pmc:
         ctx.fill("P", 1.0)
         ctx.cpy("R", "P")  # reads and writes exactly 8*N bytes each
@@ -190,18 +417,32 @@ def main() -> int:
         ctx.set_format("CRS")
         ctx.fill("X", 0.0)
         ctx.solve_mixed(precond="jacobi", max_iter=20, tol=1e-9, batch=1)
-        f, b, theta10 = learned.system_features(ctx, n)
-        ctx.set_poly(learned.fixed_rule_coeffs(theta10, b, rho), b)  # kernel traffic only
+        _, b = learned.matrix_features(ctx, n)
+        _, theta10 = learned.ritz_features(ctx)
+        ctx.set_poly(learned.fixed_rule_coeffs(theta10, b, rho, fixed_degree), b)  # traffic only
         ctx.fill("X", 0.0)
         ctx.solve_mixed(precond="poly", max_iter=5, tol=1e-9, batch=1)
-        log(f"pmc run: copy of {8 * n} bytes, 20+20 FP64, 20 mixed Jacobi, 5 mixed poly iterations")
+        for fmt in ("CRS", "ELL"):
+            ctx.set_format(fmt)
+            ctx.time_matvec("P", "Q", reps=5)
+        os.makedirs(os.path.join(REPO, "gpurun_out"), exist_ok=True)
+        with open(os.path.join(REPO, "gpurun_out", "pmc_meta.json"), "w") as f:
+            json.dump({"n": n, "nnz": nnz, "ndiag": ndiag, "layout": layout, "kind": kind,
+                       "side": args.side}, f)
+        log(f"pmc run: copy of {8 * n} bytes, 20+20 FP64, 20 mixed Jacobi, 5 mixed poly iterations, "
+            "5 plain CRS and ELL SpMVs")
         mlp.close()
         ctx.close()
         return 0
 
+    first_call = {}
+
     def timed(fn, label):
-        for i in range(args.warmup):
+        for i in range(max(args.warmup, 1)):
+            t0 = time.perf_counter()
             info = fn()
+            if i == 0:
+                first_call[label] = time.perf_counter() - t0
             log(f"{label} warmup {i}: {info['wall_s']:.4f} s")
         t0 = time.perf_counter()
         infos = [fn() for _ in range(args.steps)]
@@ -274,13 +515,16 @@ def main() -> int:
     nrl, nbl = ctx.true_residual("X", "K")
     xl = ctx.download("X")
     prof_coeffs, prof_src = infol["coeffs"], "learned"
-    if infol["method"] == "jacobi":  # refused: time the kernels with the fixed rule's polynomial
-        f, b, theta10 = learned.system_features(ctx, n)
-        prof_coeffs, prof_src = learned.fixed_rule_coeffs(theta10, b, rho), "fixed rule"
+    if infol["coeffs"] is None or infol["method"] == "jacobi":
+        # Jacobi chosen or the polynomial refused: time the kernels with the fixed rule's
+        _, b = learned.matrix_features(ctx, n)
+        _, theta10 = learned.ritz_features(ctx)
+        prof_coeffs = learned.fixed_rule_coeffs(theta10, b, rho, fixed_degree)
+        prof_src = "fixed rule"
     ctx.set_poly(prof_coeffs, infol["b"])
     ctx.fill("X", 0.0)
     profl = ctx.solve_mixed(precond="poly", max_iter=max_iter, tol=1e-9, profile=1)
-    deg = learned.DEGREE
+    deg = len(prof_coeffs) - 1
     step_ms_events = per(profl, "poly")  # kernel by kernel, events between iterations
     step_ms = ctx.time_poly_step(reps=5)  # steady state: applications back to back
     # Clenshaw steps, averaged over one application: each streams the
@@ -293,16 +537,21 @@ def main() -> int:
         f"{infol['jacobi_iterations']} Jacobi iterations ({infol['spmvs']} FP32 SpMVs), true residual "
         f"{nrl / nbl:.3e}, {secl:.4f} s; Clenshaw step {step_ms:.4f} ms = {step_gbps:.0f} GB/s")
 
-    # ---- the method without the network: Chebyshev on [theta10 / rho*, b] ----
+    # ---- the method without the network: one degree on [theta10 / rho*, b] ----
     def fixed_solve():
-        t0 = time.perf_counter()
-        f, b, theta10 = learned.system_features(ctx, n)
-        r = learned.solve_poly_with_fallback(ctx, learned.fixed_rule_coeffs(theta10, b, rho), b,
-                                             tol=1e-9, max_iter=max_iter)
-        r["wall_s"] = time.perf_counter() - t0
-        return r
+        return learned.solve_fixed_rule(ctx, n, rho, fixed_degree, tol=1e-9, max_iter=max_iter)
 
     secf, infof = timed(fixed_solve, "mixed+fixed-rule poly")
+
+    # ---- the measured method with the polynomial on the FP32 operator values ----
+    ctx.set_poly_values(32)
+    secl32, infol32 = timed(learned_solve, "mixed+learned, FP32 values")
+    ctx.set_poly(infol32["coeffs"], infol32["b"])
+    step32_ms = ctx.time_poly_step(reps=5)
+    ctx.set_poly_values(16)
+    step32_bytes = op_bytes + (8 * n + 12 * n + (deg - 2) * 16 * n) / deg
+    log(f"mixed+learned, FP32 values: {infol32['spmvs']} FP32 SpMVs, {secl32:.4f} s; Clenshaw step "
+        f"{step32_ms:.4f} ms = {step32_bytes / (step32_ms * 1e-3) / 1e9:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -320,17 +569,20 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f16+f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"3D 7-point variable-coefficient diffusion {args.side}^3 "
                         "(log-normal kappa, sigma=1, corr=8, seed 20261202), k=A*1, x0=0, "
                         "FP64 true relative residual 1e-9",
-            "method": f"mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 ({layout}, FP32 values), "
-                      "FP64 reliable updates (delta 0.01 with the polynomial, 0.1 with Jacobi); the trained network gives the "
-                      "degree-8 polynomial's coefficients and predicted iteration counts, and the "
-                      "polynomial runs only when its predicted cost beats Jacobi's (else Jacobi); "
-                      "features, inference, choice and fallback inside the timed step",
+            "method": f"mixed+learned: FP32-inner CG on D^-1/2 A D^-1/2 ({layout}, FP32 values in the CG "
+                      "operator), polynomial preconditioner applied by Clenshaw steps reading the "
+                      + ("FP16 copy of the DIA values" if layout == "DIA" else "FP32 values")
+                      + ", dots accumulated in FP64, FP64 reliable updates and true residual; "
+                      "the trained network predicts theta_min and each method's cost from the matrix "
+                      "statistics alone and chooses the polynomial (degree, interval "
+                      "[c_k theta_min, b]) or Jacobi; statistics, inference, choice and fallback "
+                      "inside the timed step",
             "n": n,
             "nnz": nnz,
             "format": f"CRS upload, scaled FP32 operator in {layout}"
@@ -339,6 +591,8 @@ def main() -> int:
         "iterations": infol["poly_iterations"] + infol["jacobi_iterations"],
         "fp32_spmvs": infol["spmvs"],
         "fallback_method": infol["method"],
+        "learned_choice": {k: v for k, v in infol["decision"].items() if k != "cost_ms"},
+        "learned_predicted_cost_ms": {str(k): v for k, v in infol["decision"]["cost_ms"].items()},
         "reliable_updates": infol["updates"],
         "true_relative_residual": nrl / nbl,
         "max_abs_error_vs_ones": float(np.abs(xl - 1.0).max()),
@@ -347,9 +601,21 @@ def main() -> int:
         "fp64_baseline_format": base_fmt,
         "fp64_jacobi": fp64,
         "mixed_jacobi": mixed_j,
-        "mixed_fixed_rule_poly": {"seconds": secf, "rho": rho, "fp32_spmvs": infof["spmvs"],
+        "mixed_fixed_rule_poly": {"seconds": secf, "rho": rho, "degree": fixed_degree,
+                                  "spmvs": infof["spmvs"], "theta10": infof["theta10"],
                                   "true_relative_residual": infof["true_residual"],
                                   "speedup_vs_fp64": sec64 / secf},
+        "mixed_learned_fp32_values": {"seconds": secl32, "fp32_spmvs": infol32["spmvs"],
+                                      "true_relative_residual": infol32["true_residual"],
+                                      "clenshaw_step_ms": step32_ms,
+                                      "clenshaw_step_gbps": step32_bytes / (step32_ms * 1e-3) / 1e9,
+                                      "speedup_vs_fp64": sec64 / secl32},
+        "setup_s": dict(setup, first_call_extra_s={
+            label: first_call[label] - sec for label, sec in
+            (("mixed+learned", secl), ("mixed jacobi", secmj), ("mixed+fixed-rule poly", secf))},
+            note="outside value (the reference's Converting convention): upload with host "
+                 "validation, the FP32/scaled-operator build, and each method's first solve "
+                 "beyond a steady one (HIP graph capture, buffers)"),
         "kernel_ms": {"clenshaw_step": step_ms, "clenshaw_step_in_profiled_solve": step_ms_events,
                       "spmv_dot_fp32": per(profl, "matvec"),
                       "update_fp32": per(profl, "update"), "xpay_fp32": per(profl, "xpay"),

@@ -28,7 +28,7 @@ SYMBOLS = [
     "cghip_cpy", "cghip_matvec", "cghip_axpy", "cghip_xpay", "cghip_vector_dot",
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec", "cghip_time_hook",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
-    "cghip_set_poly", "cghip_matrix_stats", "cghip_time_poly_step", "cghip_fp32_layout",
+    "cghip_set_poly", "cghip_set_poly_values", "cghip_matrix_stats", "cghip_time_poly_step", "cghip_fp32_layout",
     "cghip_mixed_iteration_bytes",
     "cghip_mlp_param_count", "cghip_mlp_create", "cghip_mlp_destroy", "cghip_mlp_last_error",
     "cghip_mlp_set_params", "cghip_mlp_get_params", "cghip_mlp_forward", "cghip_mlp_gradient",
@@ -119,6 +119,7 @@ def load() -> C.CDLL:
     L.cghip_true_residual.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.cghip_prepare_fp32.argtypes = [vp]
     L.cghip_set_poly.argtypes = [vp, C.c_int, _f64p, C.c_double]
+    L.cghip_set_poly_values.argtypes = [vp, C.c_int]
     L.cghip_matrix_stats.argtypes = [vp, _f64p, C.c_int]
     L.cghip_time_poly_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.cghip_fp32_layout.argtypes = [vp, C.POINTER(C.c_int)]
@@ -276,6 +277,11 @@ class Context:
         """Polynomial preconditioner p(D^-1 A) D^-1, p = sum_j coeffs[j] T_j(2 l / b - 1)."""
         c = np.ascontiguousarray(coeffs, dtype=np.float64)
         self._chk(self.L.cghip_set_poly(self._h, int(c.size - 1), c, float(b)), "cghip_set_poly")
+
+    def set_poly_values(self, bits: int):
+        """Precision (16 or 32) of the DIA operator values the Clenshaw
+        steps read; drops the polynomial (set it again)."""
+        self._chk(self.L.cghip_set_poly_values(self._h, int(bits)), "cghip_set_poly_values")
 
     def time_poly_step(self, reps: int = 5) -> float:
         """Average ms of one Clenshaw step (back-to-back applications)."""

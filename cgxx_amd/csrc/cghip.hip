@@ -52,6 +52,11 @@ struct cghip_ctx {
   // reduction scratch
   // 4 x kPartialSlot: <p,q> | <r,r> | <r,z> | ||k - A x||^2
   double *partial = nullptr;
+  // the FP64 fused loop's <p,q>: one partial per row block of the SpMV,
+  // then their group sums (fold_groups) after them; sized at upload for
+  // the larger of the uploaded matrices
+  double *pq1 = nullptr;
+  size_t pq1_cap = 0;
   double *dot_out = nullptr;
 
   // solver
@@ -96,13 +101,16 @@ struct cghip_ctx {
   MixedState *d_mstate = nullptr;
   MixedState *h_mstate = nullptr;  // pinned
   hipGraphExec_t graph_mp = nullptr;
-  int graph_mp_batch = 0, graph_mp_precond = -1, graph_mp_degree = 0;
+  int graph_mp_batch = 0, graph_mp_precond = -1, graph_mp_degree = 0, graph_mp_half = -1;
 
   // polynomial preconditioner: coefficients on the device, the three
   // Clenshaw work vectors (allocated by cghip_set_poly, freed with the FP32
   // data)
   bool has_poly = false;
   int poly_degree = 0;
+  // the Clenshaw steps read the FP16 copy of the DIA values (default) or
+  // the FP32 values (cghip_set_poly_values)
+  bool poly_fp16 = true;
   PolyParams *d_poly = nullptr;
   float *w32[3] = {};
 
@@ -214,8 +222,22 @@ Partials pr_of(cghip_ctx *c) {
 }
 Partials pt_of(cghip_ctx *c) { return Partials{c->partial + 3 * (size_t)kPartialSlot, nullptr}; }
 
+// Blocks of the FP64 SpMV + <p,q> kernel: one per row block (CRS) or
+// 256-row chunk (ELL).
 int spmv_blocks(cghip_ctx *c) {
-  return c->format == CGHIP_FORMAT_CRS ? spmv_grid(c->crs_nrb) : ell64_grid(c->n);
+  const int g = c->format == CGHIP_FORMAT_CRS ? c->crs_nrb : ell_chunks(c->n);
+  return g > 0 ? g : 1;
+}
+
+// Room in pq1 for G partials and their fold_groups(G) group sums.
+int ensure_pq1(cghip_ctx *c, int G) {
+  const size_t need = (size_t)G + (size_t)fold_groups(G);
+  if (c->pq1_cap >= need) return CGHIP_OK;
+  dfree(c->pq1);
+  c->pq1_cap = 0;
+  if (int rc = dalloc(c, &c->pq1, need)) return rc;
+  c->pq1_cap = need;
+  return CGHIP_OK;
 }
 
 Crs<double> crs_of(cghip_ctx *c) {
@@ -227,8 +249,10 @@ Ell<double> ell_of(cghip_ctx *c) {
 }
 
 Dia dia_of(cghip_ctx *c) {
-  Dia d{c->dia_val, c->dia_val16, c->dia_mask, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1};
+  Dia d{c->dia_val, c->dia_val16, c->dia_mask, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1,
+        0, 0};
   for (int k = 0; k < kDiaMax; k++) d.off[k] = c->dia_off[k];
+  dia_strips(d);
   return d;
 }
 
@@ -346,6 +370,7 @@ int cghip_destroy(cghip_ctx *c) {
   free_ell(c);
   dfree(c->jac);
   dfree(c->partial);
+  dfree(c->pq1);
   dfree(c->d_poly);
   dfree(c->dot_out);
   dfree(c->d_state);
@@ -438,6 +463,10 @@ int cghip_upload_crs(cghip_ctx *c, int n, int nnz, const int32_t *ptr, const int
     HIPCHK(c, hipMemcpy(c->crs_value, value, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice));
   }
   HIPCHK(c, hipMemcpy(c->crs_rb, rb.data(), sizeof(int) * ((size_t)nrb + 1), hipMemcpyHostToDevice));
+  if (int e = ensure_pq1(c, nrb > 0 ? nrb : 1)) {
+    free_crs(c);
+    return e;
+  }
   c->crs_nnz = nnz;
   c->crs_nrb = nrb;
   c->dia_ndiag = ndiag;
@@ -478,6 +507,10 @@ int cghip_upload_ell(cghip_ctx *c, int n, int width, const int32_t *length, cons
   if (slots > 0) {
     HIPCHK(c, hipMemcpy(c->ell_index, index, sizeof(int) * (size_t)slots, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->ell_data, data, sizeof(double) * (size_t)slots, hipMemcpyHostToDevice));
+  }
+  if (int e = ensure_pq1(c, std::max(ell_chunks(n), 1))) {
+    free_ell(c);
+    return e;
   }
   c->ell_width = width;
   c->has_ell = true;
@@ -613,13 +646,14 @@ int enqueue_iteration(cghip_ctx *c, const double *C, History h, hipEvent_t *ev) 
   double *p = c->vec[CGHIP_VEC_P], *q = c->vec[CGHIP_VEC_Q], *r = c->vec[CGHIP_VEC_R],
          *x = c->vec[CGHIP_VEC_X];
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  const int G = spmv_blocks(c);
+  const Partials pq1{c->pq1, nullptr}, pq2{c->pq1 + G, nullptr};
   if (c->format == CGHIP_FORMAT_CRS)
-    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, pq_of(c), c->d_state, c->stream));
+    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, pq1, pq2.a, c->d_state, c->stream));
   else
-    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, pq_of(c), c->d_state, c->stream));
+    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, pq1, pq2.a, c->d_state, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  HIPCHK(c, pcg_update(c->n, spmv_blocks(c), r, q, C, pq_of(c), pr_of(c), c->d_state, h,
-                       c->stream));
+  HIPCHK(c, pcg_update(c->n, fold_groups(G), r, q, C, pq2, pr_of(c), c->d_state, h, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   HIPCHK(c, pcg_xpay(c->n, x, p, r, C, pr_of(c), c->d_state, h, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));
@@ -694,7 +728,11 @@ int cghip_solve(cghip_ctx *c, const cghip_solve_opts *o, cghip_solve_info *info)
   if (o->profile != CGHIP_PROFILE_NONE && o->profile != CGHIP_PROFILE_KERNELS)
     return fail(c, CGHIP_ERR_ARG, "unknown profile mode %d", o->profile);
   const int mode = o->profile;
+  // Iterations per graph launch; never more than max_iter (the kernels
+  // after CG_MAX_ITER exit at once but still cost a launch each, e.g. the
+  // 10-step solves of the learned preconditioner's features).
   int batch = o->batch > 0 ? o->batch : (c->n >= (1 << 20) ? 8 : 32);
+  if (o->batch <= 0) batch = std::min(batch, o->max_iter);
   if (mode == CGHIP_PROFILE_KERNELS) batch = 1;
   // events[0..1] bracket the set-up; events[2..5] the kernels of an iteration.
   if (mode != CGHIP_PROFILE_NONE && ensure_events(c, 6)) return CGHIP_ERR_HIP;
@@ -887,6 +925,17 @@ int cghip_prepare_fp32(cghip_ctx *c) {
   return CGHIP_OK;
 }
 
+int cghip_set_poly_values(cghip_ctx *c, int bits) {
+  if (!c) return CGHIP_ERR_ARG;
+  if (bits != 16 && bits != 32)
+    return fail(c, CGHIP_ERR_ARG, "value precision must be 16 or 32 bits (got %d)", bits);
+  if (c->poly_fp16 != (bits == 16)) {
+    c->poly_fp16 = bits == 16;
+    c->has_poly = false;  // its positivity check depended on the values
+  }
+  return CGHIP_OK;
+}
+
 int cghip_fp32_layout(const cghip_ctx *c, int *layout) {
   if (!c || !layout) return CGHIP_ERR_ARG;
   *layout = !c->has_scaled ? -1
@@ -921,7 +970,7 @@ int cghip_set_poly(cghip_ctx *c, int degree, const double *coeffs, double b) {
   // entries differ by at most 2^-11 relative (|a_ij| <= 1 after scaling), so
   // its spectrum lies in [-2^-10 b, (1 + 2^-10) b]: p is checked there.
   constexpr int kGrid = 4096;
-  const double ext = c->dia_nd ? 2.0 * std::ldexp(1.0, -10) : 0.0;  // in y units
+  const double ext = c->dia_nd && c->poly_fp16 ? 2.0 * std::ldexp(1.0, -10) : 0.0;  // in y units
   for (int i = 0; i < kGrid; i++) {
     const double y = -1.0 - ext + 2.0 * (1.0 + ext) * i / (kGrid - 1);
     double b1 = 0.0, b2 = 0.0;
@@ -1010,10 +1059,15 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   // the long polynomial iterations of large systems take small batches (256^3:
   // 155.1-155.3 ms at 4-8, 158.3 ms at 32), the short Jacobi ones large batches
   // (271.8 ms at 8, 270.0 ms at 32).
-  const int batch = profile ? 1
-                    : o->batch > 0 ? o->batch
-                    : (poly && n >= (1 << 20)) ? 8
-                                               : 32;
+  // With the polynomial an iteration is degree + 3 kernels and a reliable
+  // update falls due every few iterations, after which the rest of the
+  // batch still launches (each kernel exits at once): the batch holds about
+  // 64 kernels, 3-8 iterations (512k rows, degree 16: 9.4 ms at 32 per
+  // graph against 7.6 ms for degree 8).  Jacobi iterations are 3 kernels:
+  // 8 per graph from 2^20 rows, else 32.
+  const int auto_batch = poly ? std::max(3, std::min(8, 64 / (degree + 3)))
+                              : (n >= (1 << 20) ? 8 : 32);
+  const int batch = profile ? 1 : o->batch > 0 ? o->batch : std::min(auto_batch, o->max_iter);
   if (profile && ensure_events(c, 7)) return CGHIP_ERR_HIP;
   hipEvent_t *ev = profile ? c->events.data() : nullptr;
 
@@ -1032,7 +1086,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
     if (poly) {
       HIPCHK(c, mp_restart(n, r64, scale, r32, y32, p32, false, prr, c->stream));
       HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, first ? p32 : nullptr, prz, nullptr,
-                              c->stream));
+                              c->stream, c->poly_fp16));
     } else {
       HIPCHK(c, mp_restart(n, r64, scale, r32, y32, p32, first, prr, c->stream));
     }
@@ -1053,7 +1107,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
       HIPCHK(c, mp_update(n, gpq, r32, q32, pa, prr, c->d_mstate, h, c->stream));
       if (e) HIPCHK(c, hipEventRecord(e[2], c->stream));
       HIPCHK(c, mp_poly_apply(A32, r32, c->d_poly, degree, w, nullptr, prz, c->d_mstate,
-                              c->stream));
+                              c->stream, c->poly_fp16));
       if (e) HIPCHK(c, hipEventRecord(e[3], c->stream));
       HIPCHK(c, mp_xpay_poly(n, y32, p32, w[0], prr, prz, gz, c->d_mstate, h, c->stream));
     } else {
@@ -1067,7 +1121,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
   };
 
   if (!profile && (!c->graph_mp || c->graph_mp_batch != batch || c->graph_mp_precond != o->precond ||
-                   c->graph_mp_degree != degree)) {
+                   c->graph_mp_degree != degree || c->graph_mp_half != (int)c->poly_fp16)) {
     if (c->graph_mp) (void)hipGraphExecDestroy(c->graph_mp);
     c->graph_mp = nullptr;
     hipGraph_t g = nullptr;
@@ -1089,6 +1143,7 @@ int cghip_solve_mixed(cghip_ctx *c, const cghip_mixed_opts *o, cghip_mixed_info 
     c->graph_mp_batch = batch;
     c->graph_mp_precond = o->precond;
     c->graph_mp_degree = degree;
+    c->graph_mp_half = (int)c->poly_fp16;
   }
 
   const auto t0 = std::chrono::steady_clock::now();
@@ -1254,7 +1309,7 @@ int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) 
   // steps read DIA's FP16 copy
   const double op32 = c->dia_nd ? 4.0 * d * n + n : c->ell_nd ? 8.0 * nnz + 4.0 * n
                                                               : 8.0 * nnz + 4.0 * (n + 1);
-  const double op16 = c->dia_nd ? 2.0 * d * n + n : op32;
+  const double op16 = c->dia_nd && c->poly_fp16 ? 2.0 * d * n + n : op32;
   const double spmv = op32 + 8.0 * n;
   if (precond == CGHIP_PRECOND_JACOBI) {
     *bytes = spmv + 12.0 * n + 20.0 * n;  // + r update (r, q; r) + y, p update (y, p, r; y, p)
@@ -1284,11 +1339,11 @@ int cghip_time_poly_step(cghip_ctx *c, int reps, double *avg_ms) {
   HIPCHK(c, fill32(c->n, 1.0f, c->v32[1], c->stream));
   for (auto &v : c->w32) HIPCHK(c, fill32(c->n, 0.0f, v, c->stream));
   HIPCHK(c, mp_poly_apply(A, c->v32[1], c->d_poly, c->poly_degree, w, nullptr, prz, nullptr,
-                          c->stream));  // warm-up
+                          c->stream, c->poly_fp16));  // warm-up
   HIPCHK(c, hipEventRecord(c->events[0], c->stream));
   for (int i = 0; i < reps; i++)
     HIPCHK(c, mp_poly_apply(A, c->v32[1], c->d_poly, c->poly_degree, w, nullptr, prz, nullptr,
-                            c->stream));
+                            c->stream, c->poly_fp16));
   HIPCHK(c, hipEventRecord(c->events[1], c->stream));
   HIPCHK(c, hipEventSynchronize(c->events[1]));
   *avg_ms = elapsed(c->events[0], c->events[1]) / ((double)reps * c->poly_degree);

@@ -17,7 +17,7 @@ constexpr int kVecGMax = 2048;   // max blocks of grid-stride vector kernels
 constexpr int kRowsMax = 256;    // max rows in one CRS row block (one per thread)
 constexpr int kNnzCap = 2048;    // max nonzeros staged in LDS per row block
 constexpr int kSpmvGMax = 2048;  // max blocks of the CRS SpMV and FP32 ELL kernels
-constexpr int kEllGMax = 8192;   // max blocks of the FP64 ELL SpMV kernels
+constexpr int kEllGMax = 0;      // the FP64 SpMV kernels are not capped: one block per row block
 constexpr int kDiaGMax = 4096;   // max blocks of the FP32 DIA kernels (scaled operator)
 constexpr int kPartialSlot = 8192;  // entries per partial-sum slot (>= every grid above)
 constexpr int kDiaMax = 8;       // max diagonals of the FP32 DIA operator (one mask byte per row)
@@ -48,15 +48,11 @@ inline int spmv_grid(int nrb) {
   return (int)g;
 }
 
-// ELL kernels: 256-row chunks, one row per thread; FP64 with up to 8192
-// blocks (more waves in flight for the 12-byte slots), FP32 up to 2048.
+// ELL kernels: 256-row chunks, one row per thread; the FP64 ones one block
+// per chunk, the FP32 ones at most 2048 blocks.
 __host__ __device__ inline int ell_chunks(int n) { return (int)(((int64_t)n + kBlock - 1) / kBlock); }
-inline int ell64_grid(int n) {
-  int64_t g = ell_chunks(n);
-  if (g > kEllGMax) g = kEllGMax;
-  if (g < 1) g = 1;
-  return (int)g;
-}
+// Groups of 256 partials folded by k_fold_groups.
+inline int fold_groups(int G) { return (G + kBlock - 1) / kBlock; }
 inline int ell32_grid(int n) { return spmv_grid(ell_chunks(n)); }
 // DIA kernels: the same chunks, up to 4096 blocks (256^3 Clenshaw step: 83.5 us
 // at 4096, 85.2 us at 8192, 86.8 us at 2048 blocks).
@@ -149,10 +145,13 @@ hipError_t residual_norms(int n, const double *b, const double *q, Partials pa, 
 // Fused PCG loop kernels (one iteration = matvec_dot, update, xpay).
 hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *C, Partials pa,
                     SolveState *st, History h, hipStream_t s);
+// q = A p with one partial of <p,q> per row block (CRS) or 256-row chunk
+// (ELL) in pq.a, then those partials folded in groups of 256 into
+// pq2[0 .. fold_groups(G)), G the number of row blocks / chunks.
 hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Partials pq,
-                              SolveState *st, hipStream_t s);
+                              double *pq2, SolveState *st, hipStream_t s);
 hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
-                              SolveState *st, hipStream_t s);
+                              double *pq2, SolveState *st, hipStream_t s);
 // alpha = rho / <p,q>; r -= alpha q; partials of <r,r> and <r,Cr>
 hipError_t pcg_update(int n, int gpq, double *r, const double *q, const double *C, Partials pq,
                       Partials pr, SolveState *st, History h, hipStream_t s);
@@ -246,7 +245,27 @@ struct Dia {
   int n, ndiag;
   int off[kDiaMax];
   int unit;  // 1: implicit unit diagonal
+  // Strip walk of the Clenshaw steps without a dot (dia_strips): the
+  // largest offset P is a plane of plane_chunks 256-row chunks, cut into
+  // strips of strip_chunks; 0 = not a plane-aligned stencil.
+  int plane_chunks, strip_chunks;
 };
+// Strip walk geometry of a DIA operator whose largest offset P is a whole
+// number of chunks dividing n (a 3D stencil whose x-y plane is P rows):
+// strips of the largest divisor of P / 256 not above 32 chunks.  Sets both
+// to 0 when the operator does not qualify.
+inline void dia_strips(Dia &d) {
+  d.plane_chunks = d.strip_chunks = 0;
+  if (d.ndiag < 1) return;
+  const int64_t P = d.off[d.ndiag - 1];
+  if (P <= 0 || P % kBlock != 0 || d.n % P != 0 || d.n / P < 2) return;
+  const int pc = (int)(P / kBlock);
+  if (pc < 16) return;
+  int w = 32;
+  while (pc % w) w--;
+  d.plane_chunks = pc;
+  d.strip_chunks = w;
+}
 inline bool dia_fits(int64_t ndiag, int64_t n) {
   return ndiag >= 1 && ndiag <= kDiaMax && ndiag * n * 4 <= (1ll << 32) && n * 4 <= (1ll << 32);
 }
@@ -309,9 +328,10 @@ hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, 
 // Polynomial variant (its r update and restart are mp_update and mp_restart).
 // Work buffers w[0..2] hold the Clenshaw B_j (B_j in w[j % 3]; B_k = c_k r is
 // recomputed where it is read, not stored); z ends in w[0].
+// half: with the DIA layout, the Clenshaw steps read the FP16 values.
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
-                         hipStream_t s);
+                         hipStream_t s, bool half);
 hipError_t mp_xpay_poly(int n, float *y32, float *p32, const float *z32, Partials prr, Partials prz,
                         int gz, MixedState *st, History h, hipStream_t s);
 hipError_t mp_matvec_dot(const Op32 &A, const float *p, float *q, Partials pq, MixedState *st,

@@ -64,16 +64,21 @@ __device__ inline bool begin_iteration(SolveState *st) {
   return !stop;
 }
 
+// One block per row block (not a persistent grid: the hardware keeps new
+// blocks streaming while others finish, 334 against 376 us at 256^3 for the
+// CRS kernel), one partial of <p,q> per block; k_fold_groups then folds
+// them in groups of 256 before the r update reads them.  Consecutive row
+// blocks on consecutive XCDs (the dispatch order) measured faster than
+// giving each XCD a contiguous range of row blocks (382 vs 402 us at
+// 256^3; ELL 338 vs 359 us).
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_crs(Crs<double> A, const double *__restrict__ p,
                                                                double *__restrict__ q, Partials pq,
                                                                SolveState *st) {
   __shared__ double prod[kNnzCap];
   __shared__ double lds[4];
   if (!begin_iteration(st)) return;
-  double tpart = 0.0;
-  for (int rb = blockIdx.x; rb < A.nrb; rb += gridDim.x)
-    tpart += crs_rowblock<double, double, true>(A, p, q, rb, prod);
-  const double bp = block_fold(tpart, lds);
+  const double t = crs_rowblock<double, double, true>(A, p, q, blockIdx.x, prod);
+  const double bp = block_fold(t, lds);
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
 }
 
@@ -83,12 +88,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_dot_ell(Ell<double> A, co
                                                                SolveState *st) {
   __shared__ double lds[4];
   if (!begin_iteration(st)) return;
-  const int nrb = (A.n + kBlock - 1) / kBlock;
-  double tpart = 0.0;
-  for (int rb = blockIdx.x; rb < nrb; rb += gridDim.x)
-    tpart += ell_rowblock<double, double, true, O32>(A, p, q, rb);
-  const double bp = block_fold(tpart, lds);
+  const double t = ell_rowblock<double, double, true, O32>(A, p, q, blockIdx.x);
+  const double bp = block_fold(t, lds);
   if (threadIdx.x == 0) pq.a[blockIdx.x] = bp;
+}
+
+// Group fold: block g folds partials g*256 .. g*256+255 (zero past G) into
+// out[g], so that the next kernel folds ceil(G / 256) values instead of G.
+__global__ __launch_bounds__(kBlock) void k_fold_groups(const double *__restrict__ in, int G,
+                                                        double *__restrict__ out) {
+  __shared__ double lds[4];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const double s = block_fold(i < G ? in[i] : 0.0, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +321,8 @@ __device__ inline void update_elem(double &rv, double qv, double cv, double na, 
 // the x += a p of CG.cpp:340 is done by the last kernel, which reads p
 // anyway); partials of <r,r> and <r,z> with z = C o r (CG.cpp:344, 363-366).
 template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *__restrict__ r,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pcg_update(
+    int n, int gpq, double *__restrict__ r,
                                                        const double *__restrict__ q,
                                                        const double *__restrict__ C, Partials ppq,
                                                        Partials prr, SolveState *st, History h) {
@@ -332,8 +345,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int gpq, double *_
     for (int u = 0; u < kUnroll; u++) {
       const int64_t j = P.q + u * P.T;
       rv[u] = v2(r)[j];
-      qv[u] = v2(q)[j];
-      if (JAC) cv[u] = v2(C)[j];
+      qv[u] = v2_stream(q, j);
+      if (JAC) cv[u] = v2_stream(C, j);
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
@@ -414,8 +427,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
       const int64_t j = P.q + u * P.T;
       xv[u] = v2(x)[j];
       pv[u] = v2(p)[j];
-      rv[u] = v2(r)[j];
-      if (JAC) cv[u] = v2(C)[j];
+      rv[u] = v2_stream(r, j);
+      if (JAC) cv[u] = v2_stream(C, j);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -539,21 +552,24 @@ hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *
   return hipGetLastError();
 }
 
+// SpMV + <p,q> partials into pq.a (one per row block), then their group
+// sums into pq2 (fold_groups(G) entries).
 hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Partials pq,
-                              SolveState *st, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_matvec_dot_crs, dim3(spmv_grid(A.nrb)), dim3(kBlock), 0, s, A, p, q,
-                     pq, st);
+                              double *pq2, SolveState *st, hipStream_t s) {
+  const int G = A.nrb > 0 ? A.nrb : 1;
+  hipLaunchKernelGGL(k_pcg_matvec_dot_crs, dim3(G), dim3(kBlock), 0, s, A, p, q, pq, st);
+  hipLaunchKernelGGL(k_fold_groups, dim3(fold_groups(G)), dim3(kBlock), 0, s, pq.a, G, pq2);
   return hipGetLastError();
 }
 
 hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
-                              SolveState *st, hipStream_t s) {
+                              double *pq2, SolveState *st, hipStream_t s) {
+  const int G = ell_chunks(A.n) > 0 ? ell_chunks(A.n) : 1;
   if (ell_o32(A))
-    hipLaunchKernelGGL(k_pcg_matvec_dot_ell<true>, dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A,
-                       p, q, pq, st);
+    hipLaunchKernelGGL(k_pcg_matvec_dot_ell<true>, dim3(G), dim3(kBlock), 0, s, A, p, q, pq, st);
   else
-    hipLaunchKernelGGL(k_pcg_matvec_dot_ell<false>, dim3(ell64_grid(A.n)), dim3(kBlock), 0, s, A,
-                       p, q, pq, st);
+    hipLaunchKernelGGL(k_pcg_matvec_dot_ell<false>, dim3(G), dim3(kBlock), 0, s, A, p, q, pq, st);
+  hipLaunchKernelGGL(k_fold_groups, dim3(fold_groups(G)), dim3(kBlock), 0, s, pq.a, G, pq2);
   return hipGetLastError();
 }
 

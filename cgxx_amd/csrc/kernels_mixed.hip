@@ -159,12 +159,41 @@ __device__ inline void chunk_walk(int nrb, int &rb0, int &stride, int &rend) {
   }
 }
 
-template <int K, bool HALF = false, bool XCD = false, typename Pre, typename Epi,
-          typename XF = XIdent>
+// Strip walk (DIA steps without a dot, when dia_strips found a plane of Pc
+// chunks cut into strips of Wc): the chunks in the order strip-major, then
+// plane, then chunk within the strip, and that sequence split into eighths
+// per XCD as above.  An XCD's resident blocks then cover a few planes of
+// one strip instead of two whole planes, so x[r +- P] is read again a few
+// hundred KB of streaming later instead of about one L2's worth.  Each row
+// is computed alone, so the walk changes no result.
+__device__ inline bool strip_walk(const Dia &d, int &q0, int &stride, int &qend) {
+  if (d.strip_chunks <= 0 || (gridDim.x & 7) != 0) return false;
+  const int nrb = ell_chunks(d.n), x = blockIdx.x & 7, R = (nrb + 7) >> 3;
+  q0 = x * R + (blockIdx.x >> 3);
+  stride = gridDim.x >> 3;
+  qend = min(nrb, (x + 1) * R);
+  return true;
+}
+__device__ inline int strip_chunk(const Dia &d, int q) {
+  const int Pc = d.plane_chunks, Wc = d.strip_chunks;
+  const int per = (ell_chunks(d.n) / Pc) * Wc;  // chunks of one strip over all planes
+  const int s = q / per, rem = q - s * per;
+  const int z = rem / Wc;
+  return z * Pc + s * Wc + (rem - z * Wc);
+}
+
+template <int K, bool HALF = false, bool XCD = false, bool STRIP = false, typename Pre,
+          typename Epi, typename XF = XIdent>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
                                    Epi epi, XF xf = XF()) {
   double tpart = 0.0;
-  if (K == kOpDia) {
+  int q0, qs, qend;
+  if (K == kOpDia && STRIP && strip_walk(A.dia, q0, qs, qend)) {
+    for (int q = q0; q < qend; q += qs) {
+      const int r = strip_chunk(A.dia, q) * kBlock + (int)threadIdx.x;
+      tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi, xf);
+    }
+  } else if (K == kOpDia) {
     int rb0, stride, rend;
     chunk_walk<XCD>(ell_chunks(A.dia.n), rb0, stride, rend);
     for (int rb = rb0; rb < rend; rb += stride) {
@@ -447,12 +476,15 @@ enum { kStepPlain = 0, kStepFirst = 1, kStepSecond = 2 };
 // and the second (kStepSecond, j = k - 2) takes c_k r[i] as its B_{j+2},
 // the same products a stored B_k would give.  st == nullptr runs the step
 // unconditionally (restart); otherwise it is skipped once the iteration is
-// off.  A_hat is the scaled operator: its unit diagonal is implicit (the
-// launcher refuses any other).  The kernel keeps at most 80 SGPRs: above
+// off.  HALF (DIA only): the operator values come from the FP16 copy.
+// A_hat is the scaled operator: its unit diagonal is implicit (the launcher
+// refuses any other).  NT: r and B_{j+2}, read once per row, are loaded
+// non-temporally (poly_stream picks it by size).  The kernel keeps at most
+// 80 SGPRs: above
 // that, fewer than 8 blocks of 256 fit on a CU of gfx950 and the grid of
 // 2048 blocks runs in two rounds (295 us instead of 200 us per step at
 // 256^3).
-template <bool FINAL, int K, int MODE>
+template <bool FINAL, int K, int MODE, bool HALF, bool NT = false>
 __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
                                                       const float *__restrict__ r,
@@ -466,13 +498,13 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
   const float ck = poly->c[poly->degree];
-  const double tpart = op32_rows<K, K == kOpDia, K == kOpDia || !FINAL>(
+  const double tpart = op32_rows<K, HALF, K == kOpDia || !FINAL, !FINAL>(
       A, MODE == kStepFirst ? r : b1, prod,
       [&](int i) {
-        const float ri = r[i];
+        const float ri = NT ? __builtin_nontemporal_load(r + i) : r[i];
         if (MODE == kStepFirst) return float4{ck * ri, ri, 0.0f, 0.0f};
         if (MODE == kStepSecond) return float4{b1[i], ri, ck * ri, 0.0f};
-        return float4{b1[i], ri, b2 ? b2[i] : 0.0f, 0.0f};
+        return float4{b1[i], ri, b2 ? (NT ? __builtin_nontemporal_load(b2 + i) : b2[i]) : 0.0f, 0.0f};
       },
       [&](int i, float t0, float4 pv) -> double {
         const float bi = pv.x, ri = pv.y, bb = pv.z;
@@ -588,7 +620,7 @@ hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, 
   return hipGetLastError();
 }
 
-template <int K>
+template <int K, bool HALF, bool NT>
 static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
@@ -597,41 +629,52 @@ static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly
   for (int j = degree - 1; j >= 1; j--) {
     const float *b1 = w[(j + 1) % 3], *b2 = (j + 2 <= degree - 1) ? w[(j + 2) % 3] : nullptr;
     if (j == degree - 1)
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst>), grid, block, 0, s, A, nullptr,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst, HALF, NT>), grid, block, 0, s, A, nullptr,
                          nullptr, r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
     else if (j == degree - 2)
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond>), grid, block, 0, s, A, b1, nullptr,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond, HALF, NT>), grid, block, 0, s, A, b1, nullptr,
                          r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
     else
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain>), grid, block, 0, s, A, b1, b2, r32,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain, HALF, NT>), grid, block, 0, s, A, b1, b2, r32,
                          poly, j, w[j % 3], (float *)nullptr, prz, st);
   }
   if (degree == 1)
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst>), grid, block, 0, s, A, nullptr, nullptr,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst, HALF, NT>), grid, block, 0, s, A, nullptr, nullptr,
                        r32, poly, 0, w[0], pcopy, prz, st);
   else if (degree == 2)
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond>), grid, block, 0, s, A, w[1], nullptr,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond, HALF, NT>), grid, block, 0, s, A, w[1], nullptr,
                        r32, poly, 0, w[0], pcopy, prz, st);
   else
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain>), grid, block, 0, s, A, w[1], w[2], r32,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain, HALF, NT>), grid, block, 0, s, A, w[1], w[2], r32,
                        poly, 0, w[0], pcopy, prz, st);
 }
 
+// Non-temporal r and B_{j+2} in the Clenshaw steps (FP16 DIA values) above
+// 2^24 rows, where the step's vectors no longer stay in the 256 MB
+// last-level cache between launches: 320^3 step 202 -> 177 us; at 256^3
+// they would cost 83 -> 87 us.
+static bool poly_stream(int n) { return n > (1 << 24); }
+
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
-                         hipStream_t s) {
+                         hipStream_t s, bool half) {
   switch (A.kind) {
     case kOpDia:
       if (A.dia.unit != 1) return hipErrorInvalidValue;
-      poly_apply_t<kOpDia>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      if (half && poly_stream(A.dia.n))
+        poly_apply_t<kOpDia, true, true>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      else if (half)
+        poly_apply_t<kOpDia, true, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      else
+        poly_apply_t<kOpDia, false, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
       break;
     case kOpEll:
       if (A.ell.unit != 1) return hipErrorInvalidValue;
-      poly_apply_t<kOpEll>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      poly_apply_t<kOpEll, false, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
       break;
     default:
       if (A.crs.unit != 1) return hipErrorInvalidValue;
-      poly_apply_t<kOpCrs>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      poly_apply_t<kOpCrs, false, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
   }
   return hipGetLastError();
 }

@@ -70,7 +70,8 @@ int cghip_destroy(cghip_ctx *ctx);
  * cghip_create). Never NULL. */
 const char *cghip_last_error(const cghip_ctx *ctx);
 /* Reduction geometry of the kernels (the summation order is a function of
- * these and of the vector length / CRS row pointer only). */
+ * these and of the vector length / CRS row pointer only).  ell_gmax is 0:
+ * the FP64 SpMV kernels run one block per row block, uncapped. */
 int cghip_geometry(int *block, int *vec_gmax, int *rows_max, int *nnz_cap,
                    int *spmv_gmax, int *ell_gmax, int *dia_max);
 
@@ -207,6 +208,15 @@ int cghip_fp32_layout(const cghip_ctx *ctx, int *layout);
  * degree with the recurrence fused into the SpMV kernel.  Needs the scaled
  * operator of cghip_prepare_fp32; dropped by the next matrix upload. */
 int cghip_set_poly(cghip_ctx *ctx, int degree, const double *coeffs, double b);
+
+/* Precision of the operator values the polynomial's Clenshaw steps read
+ * when the scaled operator is stored in DIA: 16 (default: the FP16 copy of
+ * the values, 2 bytes per entry) or 32 (the FP32 values the CG operator
+ * reads).  With 16 the preconditioner is the polynomial of the FP16-rounded
+ * operator, still SPD when p > 0 on its spectrum (cghip_set_poly then checks
+ * p on [-2^-10 b, (1 + 2^-10) b]).  Changing it drops the polynomial: call
+ * cghip_set_poly again.  ELL and CRS layouts always read FP32 values. */
+int cghip_set_poly_values(cghip_ctx *ctx, int bits);
 
 typedef struct {
   int precond;       /* CGHIP_PRECOND_* */

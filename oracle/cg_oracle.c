@@ -426,6 +426,37 @@ double orc_rowblock_dot_tree_walk(int nrb, const int *starts, const double *p,
   return r;
 }
 
+/* The FP64 fused loop's <p,q>: one block per row block (thread t owns row
+ * starts[g] + t), the block's fold; those partials folded in groups of 256
+ * by one block each (zero past the last); the group sums folded like any
+ * block partials (final_fold). */
+double orc_rowblock_dot_tree_groups(int nrb, const int *starts, const double *p,
+                                    const double *q) {
+  if (nrb <= 0) return 0.0;
+  int G2 = (nrb + ORC_BLOCK - 1) / ORC_BLOCK;
+  double *part = (double *)malloc(sizeof(double) * (size_t)nrb);
+  double *group = (double *)malloc(sizeof(double) * (size_t)G2);
+  double v[ORC_BLOCK];
+  for (int g = 0; g < nrb; g++) {
+    for (int t = 0; t < ORC_BLOCK; t++) {
+      int r = starts[g] + t;
+      v[t] = r < starts[g + 1] ? p[r] * q[r] : 0.0;
+    }
+    part[g] = block_fold(v);
+  }
+  for (int h = 0; h < G2; h++) {
+    for (int t = 0; t < ORC_BLOCK; t++) {
+      int i = h * ORC_BLOCK + t;
+      v[t] = i < nrb ? part[i] : 0.0;
+    }
+    group[h] = block_fold(v);
+  }
+  double r = final_fold(group, G2);
+  free(part);
+  free(group);
+  return r;
+}
+
 /* ------------------------------------------------------------------------ */
 /* PCG: CG.cpp:284-385.                                                      */
 
@@ -536,9 +567,7 @@ int orc_pcg(const orc_problem *p, const double *k, double *x, orc_result *res,
 
   for (it = 0; it < p->max_iter; it++) {
     spmv(&c, P, Q);
-    dot_pq = fused ? orc_rowblock_dot_tree_g(c.nrb, c.starts, P, Q,
-                                             p->format == 2 ? ORC_ELL_GMAX : ORC_SPMV_GMAX)
-                   : dot(&c, P, Q);
+    dot_pq = fused ? orc_rowblock_dot_tree_groups(c.nrb, c.starts, P, Q) : dot(&c, P, Q);
     a = rho / dot_pq;
     if (h_pq) h_pq[it] = dot_pq;
     if (h_alpha) h_alpha[it] = a;
@@ -762,7 +791,7 @@ static void poly_clenshaw(const op32 *o, const float *u, const orc_poly *pl,
     float *b0 = w[j % 3];
     const float cj = pl->c[j];
     for (int i = 0; i < o->N; i++) {
-      const float t = op32_row_poly(o, b1, i);
+      const float t = pl->fp32_values ? op32_row(o, b1, i) : op32_row_poly(o, b1, i);
       const float sv = tb * t - b1[i];
       const float bb = b2 ? b2[i] : 0.0f;
       b0[i] = j > 0 ? (cj * u[i] + 2.0f * sv) - bb : (cj * u[i] + sv) - bb;

@@ -46,7 +46,7 @@ extern "C" {
 #define ORC_ROWS_MAX 256  /* max rows of one SpMV row block (one per thread) */
 #define ORC_NNZ_CAP 2048  /* max nonzeros staged per CRS row block */
 #define ORC_SPMV_GMAX 2048 /* max blocks of the CRS SpMV and FP32 ELL kernels */
-#define ORC_ELL_GMAX 8192  /* max blocks of the FP64 ELL SpMV kernels */
+#define ORC_ELL_GMAX 0     /* the FP64 SpMV kernels: one block per row block, no cap */
 #define ORC_DIA_GMAX 4096  /* max blocks of the FP32 DIA kernels */
 #define ORC_DIA_MAX 8      /* max diagonals of the FP32 DIA layout */
 
@@ -95,6 +95,10 @@ double orc_dot_tree(int N, const double *a, const double *b);
  * (CRS: orc_crs_rowblocks; ELL: 256-row chunks). */
 double orc_rowblock_dot_tree(int nrb, const int *starts, const double *p,
                              const double *q);
+/* The FP64 fused loop's order: one block per row block, the block partials
+ * folded in groups of 256, the group sums folded as block partials. */
+double orc_rowblock_dot_tree_groups(int nrb, const int *starts, const double *p,
+                                    const double *q);
 /* The same with at most gmax blocks (G = min(nrb, gmax)). */
 double orc_rowblock_dot_tree_g(int nrb, const int *starts, const double *p,
                                const double *q, int gmax);
@@ -193,6 +197,8 @@ typedef struct {
   int degree;
   float c[ORC_POLY_MAX + 1];
   float two_over_b;
+  int fp32_values; /* 1: the Clenshaw steps read the FP32 values even in DIA
+                      (cghip_set_poly_values(ctx, 32)) */
 } orc_poly;
 
 /* v rounded to binary16 (nearest, ties to even), returned as a float. */

@@ -93,13 +93,15 @@ def lib() -> C.CDLL:
         L.orc_rowblock_dot_tree.restype = C.c_double
         L.orc_rowblock_dot_tree_walk.argtypes = [C.c_int, _i32p, _f64p, _f64p, C.c_int, C.c_int]
         L.orc_rowblock_dot_tree_walk.restype = C.c_double
+        L.orc_rowblock_dot_tree_groups.argtypes = [C.c_int, _i32p, _f64p, _f64p]
+        L.orc_rowblock_dot_tree_groups.restype = C.c_double
         L.orc_pcg.argtypes = [C.POINTER(Problem), _f64p, _f64p, C.POINTER(Result)] + [C.c_void_p] * 5
         _lib = L
     return _lib
 
 
 GEOMETRY = {"block": 256, "vec_gmax": 2048, "rows_max": 256, "nnz_cap": 2048, "spmv_gmax": 2048,
-            "ell_gmax": 8192, "dia_max": 8}
+            "ell_gmax": 0, "dia_max": 8}
 
 
 class MMFile:
@@ -212,12 +214,16 @@ class MixedResult(C.Structure):
 
 
 class Poly(C.Structure):
-    _fields_ = [("degree", C.c_int), ("c", C.c_float * 17), ("two_over_b", C.c_float)]
+    _fields_ = [("degree", C.c_int), ("c", C.c_float * 17), ("two_over_b", C.c_float),
+                ("fp32_values", C.c_int)]
 
 
-def make_poly(coeffs, b):
-    """orc_poly from Chebyshev coefficients on [0, b], rounded as the library does."""
+def make_poly(coeffs, b, fp32_values=False):
+    """orc_poly from Chebyshev coefficients on [0, b], rounded as the library
+    does; fp32_values: the Clenshaw steps read the FP32 operator values
+    (cghip_set_poly_values 32) instead of the FP16 copy of a DIA operator."""
     pl = Poly()
+    pl.fp32_values = int(fp32_values)
     pl.degree = len(coeffs) - 1
     if not 1 <= pl.degree <= 16:
         raise ValueError("degree must be in [1, 16]")
@@ -228,7 +234,7 @@ def make_poly(coeffs, b):
 
 
 def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max_iter=100000,
-              delta=0.0, history=True, poly=None):
+              delta=0.0, history=True, poly=None, poly_fp32_values=False):
     """The oracle's FP32-inner / FP64 reliable-update PCG (CRS).
 
     precond "jacobi" and "poly" run on the scaled operator
@@ -248,7 +254,7 @@ def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max
     if precond == "poly":
         if poly is None:
             raise ValueError("precond 'poly' needs poly=(coeffs, b)")
-        pl = make_poly(*poly)
+        pl = make_poly(*poly, fp32_values=poly_fp32_values)
     p.max_iter = max_iter
     p.tol = tol
     p.N = n
@@ -273,7 +279,7 @@ def mixed_pcg(*, n, crs, k, precond="jacobi", C_jac=None, x0=None, tol=1e-9, max
     return x, info, h
 
 
-def poly_apply(crs, r32, coeffs, b):
+def poly_apply(crs, r32, coeffs, b, fp32_values=False):
     """z = p(A_hat) r in FP32 on the scaled operator A_hat = D^-1/2 A D^-1/2,
     the library's expression order."""
     L = lib()
@@ -283,7 +289,7 @@ def poly_apply(crs, r32, coeffs, b):
     n = ptr.size - 1
     r = np.ascontiguousarray(r32, dtype=np.float32)
     z = np.zeros(n, dtype=np.float32)
-    pl = make_poly(coeffs, b)
+    pl = make_poly(coeffs, b, fp32_values)
     L.orc_poly_apply.argtypes = [C.c_int] + [C.c_void_p] * 6
     rc = L.orc_poly_apply(n, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, r.ctypes.data,
                           C.cast(C.byref(pl), C.c_void_p), z.ctypes.data)

@@ -154,7 +154,9 @@ def test_solve_record(mtx, mode, tmp_path):
         if mode in ("fused", "hooks"):
             assert h[-1] == r["residual"]
         assert r["rhs"] == "ones" and r["matvec_bytes"] > 0
-        assert r["peak_gbps"] == 8000.0 and 0 < r["matvec_peak_fraction"] < 2
+        assert r["peak_gbps"] == 8000.0
+        if mode != "learned":  # the learned driver does not time its kernels: null
+            assert 0 < r["matvec_peak_fraction"] < 2
         b = r["backend"]
         assert b["name"] == "hip" and b["dot"] == "tree"
         assert b["loop"] == {"fused": "fused", "hooks": "hooks", "mixed": "mixed", "learned": "mixed"}[mode]

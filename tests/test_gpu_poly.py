@@ -64,6 +64,46 @@ def test_poly_bitwise_vs_oracle(ctx, name, degree, tmp_path):
     assert np.abs(x - 1.0).max() <= 1e-5
 
 
+@pytest.mark.parametrize("degree", [2, 8])
+@pytest.mark.parametrize("name", ["lap2d_64", "diff3d_24"])
+def test_poly_fp32_values_bitwise_vs_oracle(ctx, name, degree, tmp_path):
+    """cghip_set_poly_values(32): the Clenshaw steps read the DIA operator's
+    FP32 values instead of its FP16 copy; bit-exact with the oracle's
+    restatement of that variant, and a different preconditioner from the
+    default FP16 one.  Changing the precision drops the polynomial."""
+    from cgxx_amd.cghip import CGHIPError
+
+    fx = [f for f in load_golden() if f["case"] == name][0]
+    case = Case(fx, str(tmp_path))
+    ptr, idx, val = case.crs
+    _setup(ctx, ptr, idx, val, case.k, case.C)
+    assert ctx.fp32_layout() == "DIA"
+    c, b = _coeffs(ptr, idx, val, degree)
+    xs = {}
+    try:
+        for bits in (32, 16):
+            ctx.set_poly_values(bits)
+            with pytest.raises(CGHIPError, match="set_poly"):
+                ctx.solve_mixed(precond="poly", max_iter=20000, tol=1e-9)
+            ctx.set_poly(c, b)
+            ctx.fill("X", 0.0)
+            info = ctx.solve_mixed(precond="poly", max_iter=20000, tol=1e-9)
+            xs[bits] = ctx.download("X")
+            ox, oinfo, oh = oracle.mixed_pcg(n=case.n, crs=case.crs, k=case.k, precond="poly",
+                                             poly=(c, b), max_iter=20000, tol=1e-9,
+                                             poly_fp32_values=bits == 32)
+            assert (info["iterations"], info["updates"]) == (oinfo["iterations"], oinfo["updates"])
+            it = info["iterations"]
+            assert_bitwise(ctx.history("rho", it + 1), oh["rho"], f"rho ({bits}-bit values)")
+            assert_bitwise(xs[bits], ox, f"x ({bits}-bit values)")
+            assert info["true_residual"] <= 1e-9
+    finally:
+        ctx.set_poly_values(16)
+    # the 2D Laplacian's scaled off-diagonal values (-1/4) are exact in FP16:
+    # the same preconditioner; the variable coefficients are not
+    assert np.array_equal(xs[16], xs[32]) == (name == "lap2d_64")
+
+
 def test_poly_profile_and_batch_agree(ctx, tmp_path):
     fx = [f for f in load_golden() if f["case"] == "diff3d_24"][0]
     case = Case(fx, str(tmp_path))
@@ -106,6 +146,27 @@ def test_set_poly_rejects(ctx, tmp_path):
     ctx.prepare_fp32()
     with pytest.raises(CGHIPError):
         ctx.solve_mixed(precond="poly")
+
+
+@pytest.mark.parametrize("side", [96, 128])
+def test_poly_strip_walk_bitwise_vs_oracle(ctx, side):
+    """Grids whose x-y plane is a whole number of 256-row chunks: the
+    Clenshaw steps without a dot walk the DIA operator strip by strip
+    (2 strips of 18 chunks at 96^3, of 32 at 128^3); every row is computed
+    alone, so 12 iterations stay bitwise equal to the oracle."""
+    m = problems.anisotropic_family(100011, sides=(side,))
+    k, C = m.ones_rhs(), 1.0 / m.diagonal()
+    _setup(ctx, m.ptr, m.index, m.value, k, C)
+    assert ctx.fp32_layout() == "DIA"
+    c, b = _coeffs(m.ptr, m.index, m.value, 8, ratio=300.0)
+    ctx.set_poly(c, b)
+    info = ctx.solve_mixed(precond="poly", max_iter=12, tol=1e-300)
+    ox, oinfo, oh = oracle.mixed_pcg(n=m.n, crs=(m.ptr, m.index, m.value), k=k, precond="poly",
+                                     C_jac=C, poly=(c, b), max_iter=12, tol=1e-300)
+    assert (info["iterations"], info["updates"]) == (oinfo["iterations"], oinfo["updates"])
+    assert info["iterations"] == 12
+    assert_bitwise(ctx.history("rho", 13), oh["rho"], "rho = <r, z>")
+    assert_bitwise(ctx.download("X"), ox, "x")
 
 
 def test_poly_full_size_c2(ctx):

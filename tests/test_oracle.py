@@ -269,3 +269,31 @@ def test_rowblock_dot_tree_walk_covers_every_row_once(n, gmax, xcd):
         q2 = np.ldexp(1.0, (rows // 256).astype(np.int32))
         got2 = oracle.lib().orc_rowblock_dot_tree_walk(nrb, starts, p, q2, gmax, xcd)
         assert got2 == float(q2.sum())
+
+
+@pytest.mark.parametrize("n", [1, 255, 256 * 256, 256 * 256 + 1, 256 * 700 + 13, 3 * 256 * 256 + 5])
+def test_rowblock_dot_tree_groups_covers_every_row_once(n):
+    """The FP64 fused loop's <p,q> tree (one block per 256-row block, the
+    block sums folded in groups of 256, then the group sums) adds every row
+    exactly once: exact integer sums, and a 2^k weight per row block for
+    small cases; on CRS row blocks of a matrix with a long row too."""
+    L = oracle.lib()
+    nrb = (n + 255) // 256
+    starts = np.minimum(np.arange(nrb + 1, dtype=np.int64) * 256, n).astype(np.int32)
+    rows = np.arange(n)
+    p = np.ones(n)
+    q = ((rows % 7) + 1).astype(np.float64)
+    assert L.orc_rowblock_dot_tree_groups(nrb, starts, p, q) == float(q.sum())
+    if nrb <= 50:
+        q2 = np.ldexp(1.0, (rows // 256).astype(np.int32))
+        assert L.orc_rowblock_dot_tree_groups(nrb, starts, p, q2) == float(q2.sum())
+    # CRS row blocks (orc_crs_rowblocks) with one 3000-entry row in its own block
+    m = 3000
+    lens = np.full(m, 3)
+    lens[17] = 2999
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    st = np.zeros(m + 1, np.int32)
+    nb = L.orc_crs_rowblocks(m, ptr, st)
+    pr = np.arange(1, m + 1, dtype=np.float64)
+    assert L.orc_rowblock_dot_tree_groups(nb, st, pr, np.ones(m)) == float(pr.sum())
+
