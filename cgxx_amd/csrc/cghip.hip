@@ -883,7 +883,7 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     if (c->dia_ndiag > 0 && dia_pays(c->dia_ndiag, n, c->nnz_nd)) {
       const size_t slots = (size_t)c->dia_ndiag * (size_t)n;
       rc = dalloc(c, &c->dia_val, slots);
-      if (!rc) rc = dalloc(c, &c->dia_val16, slots);
+      if (!rc) rc = dalloc(c, &c->dia_val16, 2 * (size_t)half_words(c->dia_ndiag) * (size_t)n);
       if (!rc) rc = dalloc(c, &c->dia_mask, (size_t)n);
       if (rc) {
         free_fp32(c);
@@ -892,7 +892,6 @@ int cghip_prepare_fp32(cghip_ctx *c) {
       int *d_over = reinterpret_cast<int *>(c->dot_out);
       HIPCHK(c, hipMemsetAsync(d_over, 0, sizeof(int), c->stream));
       HIPCHK(c, hipMemsetAsync(c->dia_val, 0, sizeof(float) * slots, c->stream));
-      HIPCHK(c, hipMemsetAsync(c->dia_val16, 0, sizeof(uint16_t) * slots, c->stream));
       HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_val16,
                            c->dia_mask, d_over, c->stream));
       int over = 0;

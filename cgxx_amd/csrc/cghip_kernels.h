@@ -235,9 +235,17 @@ struct PolyParams {
 // column indices: 4 bytes per slot and one byte per row, and x is read at
 // i + off[d], contiguous across a wave.  Slots are addressed by 32-bit byte
 // offsets, so dia_fits requires ndiag * n * 4 <= 2^32.  val16 holds the
-// same values rounded to FP16 (round to nearest even): the polynomial
-// preconditioner's Clenshaw steps read them (2 bytes per slot); the CG
+// same values rounded to FP16 (round to nearest even), row by row: row i's
+// record is half_words(ndiag) 32-bit words at val16 + 2 i half_words, value
+// d in half d, and an absent entry (or the padding half) holds kHalfAbsent,
+// so the polynomial preconditioner's Clenshaw steps, which read them, need
+// neither the mask nor one stream per diagonal (2 bytes per slot); the CG
 // operator itself stays FP32.
+// FP16 bits of a signalling NaN: no FP32 -> FP16 conversion produces it (a
+// converted NaN is quiet), so it marks an absent entry of val16.
+constexpr uint16_t kHalfAbsent = 0x7D01;
+// 32-bit words of one row's FP16 record.
+__host__ __device__ inline int half_words(int ndiag) { return (ndiag + 1) / 2; }
 struct Dia {
   const float *val;
   const uint16_t *val16;
@@ -307,7 +315,8 @@ hipError_t scaled_copy(const Crs<double> &A, const double *s, int *index_nd, flo
 // k * n + i, length[i].
 hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64_t *slot,
                       hipStream_t st);
-// The same entries in DIA (Dia): val and val16 zeroed by the caller, mask
+// The same entries in DIA (Dia): val zeroed by the caller (val16 is written
+// whole, absent entries as kHalfAbsent), mask
 // written; overflow[0] = 1 (zeroed by the caller) if a value's magnitude is
 // 65520 or more (its FP16 copy would be infinite).
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
 
 // DIA copy of the scaled off-diagonal entries: entry (i, c) goes to
 // diagonal d with D.off[d] == c - i (every off-diagonal offset of the
-// matrix is in D.off, checked on the host); val is zero where no entry is.
+// matrix is in D.off, checked on the host); val is zero where no entry is,
+// and row i's FP16 record holds kHalfAbsent there.
 __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const double *__restrict__ s,
                                                        Dia D, float *__restrict__ val,
                                                        uint16_t *__restrict__ val16,
@@ -116,9 +117,12 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
                                                        int *__restrict__ overflow) {
   const int64_t T = (int64_t)gridDim.x * kBlock;
   const int64_t n = A.n;
+  const int hw = half_words(D.ndiag);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
     uint32_t m = 0;
     const double si = s[i];
+    uint16_t *rec = val16 + (int64_t)i * 2 * hw;
+    for (int h = 0; h < 2 * hw; h++) rec[h] = kHalfAbsent;
     for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
       const int c = A.index[e];
       if (c == (int)i) continue;
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
       const float v = (float)((A.value[e] * si) * s[c]);
       if (!(fabsf(v) < 65520.0f)) overflow[0] = 1;  // FP16 would round it to inf
       val[(int64_t)d * n + i] = v;
-      val16[(int64_t)d * n + i] = __builtin_bit_cast(uint16_t, (_Float16)v);
+      rec[d] = __builtin_bit_cast(uint16_t, (_Float16)v);
       m |= 1u << d;
     }
     mask[i] = (uint8_t)m;

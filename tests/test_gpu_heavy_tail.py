@@ -23,3 +23,50 @@ def test_heavy_tail_crs_ell_identical(capsys):
         assert f["residual"] <= 1e-9 and f["true_relative_residual"] <= 2e-9
     mj = out["mixed_jacobi"]
     assert mj["layout"] == "CRS" and mj["true_relative_residual"] <= 1e-9
+
+
+@pytest.mark.parametrize("n,cap,formats", [(30_000, 1000, ("CRS", "ELL")), (100_000, 3000, ("CRS",))],
+                         ids=["n30000-cap1000", "n100000-cap3000"])
+def test_heavy_tail_fused_loop_bitwise_vs_oracle(n, cap, formats):
+    """The fused FP64 Jacobi-PCG loop on the heavy-tailed matrix with the
+    seeded right-hand side, bit for bit against the oracle's restatement
+    of its order (order 1): every r2 and alpha, the iteration count and x.
+    At cap 3000 the longest rows exceed the 2048 nonzeros one CRS row block
+    stages in LDS, so they run the chunked path."""
+    import numpy as np
+
+    from cgxx_amd import problems
+    from cgxx_amd.cghip import Context
+    from oracle import oracle
+    from tests.cases import assert_bitwise, to_ell
+
+    m = problems.zipf_spd(n=n, cap=cap, seed=7)
+    k = problems.random_rhs(n, 11)
+    C = 1.0 / m.diagonal()
+    crs = (m.ptr.astype(np.int32), m.index.astype(np.int32), m.value)
+    assert cap < 2048 or int(np.diff(m.ptr).max()) > 2048
+    ctx = Context(0)
+    try:
+        ctx.set_size(n)
+        ctx.upload_crs(*crs)
+        ell = None
+        if "ELL" in formats:
+            ell = to_ell(*crs)
+            ctx.upload_ell(*ell)
+        ctx.upload_jacobi(C)
+        ctx.upload("K", k)
+        for fmt in formats:
+            print(f"heavy tail n={n} cap={cap} {fmt}", flush=True)
+            ctx.set_format(fmt)
+            ctx.fill("X", 0.0)
+            info = ctx.solve(precond="jacobi", max_iter=2000, tol=1e-9)
+            it = info["iterations"]
+            ox, oit, ores, oh = oracle.pcg(n=n, fmt=fmt, precond="jacobi", order=1, k=k, tol=1e-9,
+                                           max_iter=2000, crs=crs, ell=ell, C_jac=C)
+            assert it == oit and 0 < it < 2000
+            assert_bitwise(ctx.history("r2", it + 1), oh["r2"], f"r2 ({fmt})")
+            assert_bitwise(ctx.history("alpha", it), oh["alpha"], f"alpha ({fmt})")
+            assert_bitwise(ctx.download("X"), ox, f"x ({fmt})")
+            assert info["residual"] == ores <= 1e-9
+    finally:
+        ctx.close()
