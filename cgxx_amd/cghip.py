@@ -401,6 +401,12 @@ class Mlp:
             self.L.cghip_mlp_destroy(self._h)
             self._h = C.c_void_p()
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
         try:
             self.close()
