@@ -25,14 +25,19 @@ def test_heavy_tail_crs_ell_identical(capsys):
     assert mj["layout"] == "CRS" and mj["true_relative_residual"] <= 1e-9
 
 
-@pytest.mark.parametrize("n,cap,formats", [(30_000, 1000, ("CRS", "ELL")), (100_000, 3000, ("CRS",))],
-                         ids=["n30000-cap1000", "n100000-cap3000"])
-def test_heavy_tail_fused_loop_bitwise_vs_oracle(n, cap, formats):
+@pytest.mark.parametrize("n,cap,seed,formats", [(30_000, 1000, 7, ("CRS", "ELL")),
+                                                (100_000, 3000, 7, ("CRS",)),
+                                                (1_000_000, 2000, 20261202, ("CRS",))],
+                         ids=["n30000-cap1000", "n100000-cap3000", "c3-full-crs"])
+def test_heavy_tail_fused_loop_bitwise_vs_oracle(n, cap, seed, formats):
     """The fused FP64 Jacobi-PCG loop on the heavy-tailed matrix with the
     seeded right-hand side, bit for bit against the oracle's restatement
     of its order (order 1): every r2 and alpha, the iteration count and x.
     At cap 3000 the longest rows exceed the 2048 nonzeros one CRS row block
-    stages in LDS, so they run the chunked path."""
+    stages in LDS, so they run the chunked path; the third case is the full
+    heavy-tailed configuration (10^6 rows, the generator's default seed) in
+    CRS (its 24 GB ELL form is compared with CRS by heavy_tail above at
+    n = 30,000 and in the measurement module at full size)."""
     import numpy as np
 
     from cgxx_amd import problems
@@ -40,7 +45,7 @@ def test_heavy_tail_fused_loop_bitwise_vs_oracle(n, cap, formats):
     from oracle import oracle
     from tests.cases import assert_bitwise, to_ell
 
-    m = problems.zipf_spd(n=n, cap=cap, seed=7)
+    m = problems.zipf_spd(n=n, cap=cap, seed=seed)
     k = problems.random_rhs(n, 11)
     C = 1.0 / m.diagonal()
     crs = (m.ptr.astype(np.int32), m.index.astype(np.int32), m.value)
