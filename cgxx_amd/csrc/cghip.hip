@@ -33,7 +33,7 @@ struct cghip_ctx {
 
   // CRS (MatrixDataCRS) + row blocks of the CSR-stream kernel
   bool has_crs = false;
-  int crs_nnz = 0, crs_nrb = 0;
+  int crs_nnz = 0, crs_nrb = 0, crs_short = 0;
   int *crs_ptr = nullptr, *crs_index = nullptr, *crs_rb = nullptr;
   double *crs_value = nullptr;
 
@@ -78,7 +78,7 @@ struct cghip_ctx {
   bool has_fp32 = false;
   float *value32 = nullptr;
   bool has_scaled = false;
-  int nnz_nd = 0, nrb_nd = 0;
+  int nnz_nd = 0, nrb_nd = 0, short_nd = 0;
   int *ptr_nd = nullptr, *index_nd = nullptr, *rb_nd = nullptr;
   float *value_nd = nullptr;
   double *scale64 = nullptr;
@@ -191,7 +191,7 @@ void free_fp32(cghip_ctx *c) {
   for (auto &v : c->v32) dfree(v);
   for (auto &v : c->w32) dfree(v);
   c->has_scaled = false;
-  c->nnz_nd = c->nrb_nd = 0;
+  c->nnz_nd = c->nrb_nd = c->short_nd = 0;
   c->has_fp32 = false;
   c->has_poly = false;
   c->poly_degree = 0;
@@ -202,6 +202,7 @@ void free_crs(cghip_ctx *c) {
   dfree(c->crs_index);
   dfree(c->crs_value);
   dfree(c->crs_rb);
+  c->crs_short = 0;
   c->has_crs = false;
 }
 
@@ -241,7 +242,8 @@ int ensure_pq1(cghip_ctx *c, int G) {
 }
 
 Crs<double> crs_of(cghip_ctx *c) {
-  return Crs<double>{c->crs_ptr, c->crs_index, c->crs_value, c->crs_rb, c->n, c->crs_nrb};
+  return Crs<double>{c->crs_ptr, c->crs_index, c->crs_value, c->crs_rb, c->n, c->crs_nrb, 0,
+                     c->crs_short};
 }
 
 Ell<double> ell_of(cghip_ctx *c) {
@@ -263,12 +265,13 @@ Op32 op32_of(cghip_ctx *c, bool scaled) {
   Op32 A{};
   const int n = c->n;
   if (scaled) {
-    A.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, n, c->nrb_nd, 1};
+    A.crs = Crs<float>{c->ptr_nd, c->index_nd, c->value_nd, c->rb_nd, n, c->nrb_nd, 1, c->short_nd};
     A.ell = EllPacked{c->ell_len_nd, c->ell_slot_nd, n, c->ell_w_nd, 1};
     A.dia = dia_of(c);
     A.kind = c->dia_nd ? kOpDia : c->ell_nd ? kOpEll : kOpCrs;
   } else {
-    A.crs = Crs<float>{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb, 0};
+    A.crs = Crs<float>{c->crs_ptr, c->crs_index, c->value32, c->crs_rb, n, c->crs_nrb, 0,
+                       c->crs_short};
     A.kind = kOpCrs;
   }
   return A;
@@ -469,6 +472,7 @@ int cghip_upload_crs(cghip_ctx *c, int n, int nnz, const int32_t *ptr, const int
   }
   c->crs_nnz = nnz;
   c->crs_nrb = nrb;
+  c->crs_short = crs_short_rows(n, ptr, nrb);
   c->dia_ndiag = ndiag;
   for (int d = 0; d < kDiaMax; d++) c->dia_off[d] = d < ndiag ? off[d] : 0;
   c->has_crs = true;
@@ -866,6 +870,7 @@ int cghip_prepare_fp32(cghip_ctx *c) {
     c->nnz_nd = ptr[n];
     std::vector<int> starts(n + 1);
     c->nrb_nd = crs_rowblocks(n, ptr.data(), starts.data());
+    c->short_nd = crs_short_rows(n, ptr.data(), c->nrb_nd);
     rc = dalloc(c, &c->ptr_nd, (size_t)n + 1);
     if (!rc) rc = dalloc(c, &c->index_nd, (size_t)c->nnz_nd);
     if (!rc) rc = dalloc(c, &c->value_nd, (size_t)c->nnz_nd);

@@ -16,6 +16,7 @@ constexpr int kBlock = 256;      // threads per block (4 wave64s)
 constexpr int kVecGMax = 2048;   // max blocks of grid-stride vector kernels
 constexpr int kRowsMax = 256;    // max rows in one CRS row block (one per thread)
 constexpr int kNnzCap = 2048;    // max nonzeros staged in LDS per row block
+constexpr int kShortRow = 8;     // rows of at most this many entries: SpMV without LDS
 constexpr int kSpmvGMax = 2048;  // max blocks of the CRS SpMV and FP32 ELL kernels
 constexpr int kEllGMax = 0;      // the FP64 SpMV kernels are not capped: one block per row block
 constexpr int kDiaGMax = 4096;   // max blocks of the FP32 DIA kernels (scaled operator)
@@ -106,7 +107,12 @@ struct Crs {
   const int *rb_start;  // row-block starts, nrb + 1 entries
   int n, nrb;
   int unit;  // 1: implicit unit diagonal, y_i = x_i + (sum of the stored products)
+  // 1: no row has more than kShortRow entries, so (kShortRow * kRowsMax <=
+  // kNnzCap) row block rb is exactly rows kRowsMax * rb .. + kRowsMax - 1
+  int short_rows;
 };
+static_assert(kShortRow * kRowsMax <= kNnzCap && kRowsMax == kBlock,
+              "short-row blocks are full kRowsMax-row blocks, one row per thread");
 
 template <typename V>
 struct Ell {
@@ -120,6 +126,9 @@ struct Ell {
 // kNnzCap nonzeros, except that a block always takes its first row (a row
 // longer than kNnzCap is then streamed through LDS in chunks).
 int crs_rowblocks(int n, const int *ptr, int *starts);
+// 1 if every row of ptr has at most kShortRow entries and the row blocks
+// from crs_rowblocks are the full kRowsMax-row blocks that implies.
+int crs_short_rows(int n, const int *ptr, int nrb);
 
 // Message returned by cghip_last_error(NULL) (errors without a context).
 void set_global_error(const char *msg);

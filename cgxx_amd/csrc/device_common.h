@@ -58,17 +58,45 @@ __device__ inline double fold_partials(const double *partial, int G, double *lds
 // ---------------------------------------------------------------------------
 // SpMV.
 
+// s + p[j0] + p[j0 + 1] + ... + p[j1 - 1], added left to right, with 8 LDS
+// reads in flight (clamped to [j0, j1); the extra reads are dropped by a
+// select).
+template <typename X>
+__device__ inline X lds_sum8(const X *p, int j0, int j1, X s) {
+  for (int j = j0; j < j1; j += 8) {
+    X w[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) w[u] = p[min(j + u, j1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s = j + u < j1 ? s + w[u] : s;
+  }
+  return s;
+}
+
 // CSR-stream of row block rb with an epilogue: the thread owning row r
 // calls pv = pre(r) as soon as it knows its row (before the products, so
 // the epilogue's own loads are in flight with the matrix stream), then
 // epi(r, s, pv) with the row's sum s (products added left to right from 0)
 // and gets back its contribution to a fused dot (0 for threads without a
 // row).
+//
+// The block stages its entry range through LDS (coalesced loads, products
+// into prod), one barrier, then each owner adds its row from LDS: a row of
+// at most kShortRow products one read at a time, a longer one 8 reads at a
+// time (the 8-wide form measured 60 us slower on 7-entry rows at 256^3 and
+// 1.3x faster on the heavy-tailed rows).  Measured and not adopted at
+// 256^3: each thread loading its own row's entries with no LDS (352 against
+// 334 us: strided loads), and straight-line staging with every gather in
+// flight at once (347 against 327 us; the gathers of consecutive entries
+// spread over the stencil's distant columns).  For short-row matrices
+// (A.short_rows) row block rb is rows kBlock * rb .. + kBlock - 1, which
+// saves the dependent load of the row-block table (about 1%).
 template <typename V, typename X, typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restrict__ x, int rb,
                                               X *__restrict__ prod, Pre pre, Epi epi, XF xf = XF()) {
   const int t = threadIdx.x;
-  const int r0 = A.rb_start[rb], r1 = A.rb_start[rb + 1];
+  const int r0 = A.short_rows ? rb * kBlock : A.rb_start[rb];
+  const int r1 = A.short_rows ? min(r0 + kBlock, A.n) : A.rb_start[rb + 1];
   const int e0 = A.ptr[r0], e1 = A.ptr[r1];
   const int r = r0 + t;
   const bool own = r < r1;
@@ -99,7 +127,11 @@ __device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restri
     __syncthreads();
     if (own) {
       X s = X(0);
-      for (int j = rb0; j < re0; j++) s += prod[j - e0];
+      if (re0 - rb0 <= kShortRow) {
+        for (int j = rb0; j < re0; j++) s += prod[j - e0];
+      } else {
+        s = lds_sum8(prod, rb0 - e0, re0 - e0, s);
+      }
       dp = epi(r, s, pv);
     }
     __syncthreads();
@@ -112,8 +144,7 @@ __device__ inline double crs_rowblock_pre_epi(const Crs<V> &A, const X *__restri
       const int ce = min(c0 + kNnzCap, e1);
       for (int e = c0 + t; e < ce; e += kBlock) prod[e - c0] = X(A.value[e]) * xf(x[A.index[e]]);
       __syncthreads();
-      if (t == 0)
-        for (int j = 0; j < ce - c0; j++) s += prod[j];
+      if (t == 0) s = lds_sum8(prod, 0, ce - c0, s);
       __syncthreads();
     }
     if (t == 0) dp = epi(r0, s, pv);
@@ -151,57 +182,109 @@ __device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, 
 // 32-bit byte offsets from the two array bases, which keeps the FP64
 // kernels at 58 VGPRs (8 waves per SIMD) instead of 76 with 64-bit
 // addresses per slot.
+//
+// Slots j0 .. j0 + R - 1 of row r (len > j0), added to s left to right.
+// Straight-line code as in crs_stage: a slot past the row's length loads
+// the row's last live slot again (in bounds; the gather then reads x[r]) and
+// its product is dropped by a select (a conditional add let the compiler
+// sink the slot's loads under it).
+template <int R, typename V, typename X, bool O32>
+__device__ inline X ell_chunk(const Ell<V> &A, const X *__restrict__ x, int r, int len, int j0,
+                              X s) {
+  const int64_t n = A.n;
+  const char *ib = reinterpret_cast<const char *>(A.index);
+  const char *vb = reinterpret_cast<const char *>(A.data);
+  int c[R];
+  V v[R];
+  X xv[R];
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    const int j = min(j0 + u, len - 1);
+    if (O32) {
+      const uint32_t k = (uint32_t)j * (uint32_t)n + (uint32_t)r;
+      c[u] = __builtin_nontemporal_load(reinterpret_cast<const int *>(ib + k * 4u));
+      v[u] = __builtin_nontemporal_load(reinterpret_cast<const V *>(vb + k * (uint32_t)sizeof(V)));
+    } else {
+      const int64_t k = (int64_t)j * n + r;
+      c[u] = __builtin_nontemporal_load(A.index + k);
+      v[u] = __builtin_nontemporal_load(A.data + k);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < R; u++) xv[u] = x[j0 + u < len ? c[u] : r];
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    const X p = X(v[u]) * xv[u];
+    s = j0 + u < len ? s + p : s;
+  }
+  return s;
+}
+
 template <typename V, typename X, bool O32, typename Pre, typename Epi>
 __device__ inline double ell_row_pre_epi(const Ell<V> &A, const X *__restrict__ x, int r, Pre pre,
                                          Epi epi) {
-  const int64_t n = A.n;
   const int len = __builtin_nontemporal_load(A.length + r);
   const auto pv = pre(r);
-  const char *ib = reinterpret_cast<const char *>(A.index);
-  const char *vb = reinterpret_cast<const char *>(A.data);
   X s = X(0);
-  for (int j0 = 0; j0 < len; j0 += kEllChunk) {
-    int c[kEllChunk];
-    V v[kEllChunk];
-#pragma unroll
-    for (int u = 0; u < kEllChunk; u++) {
-      if (j0 + u < len) {
-        if (O32) {
-          const uint32_t k = (uint32_t)(j0 + u) * (uint32_t)n + (uint32_t)r;
-          c[u] = __builtin_nontemporal_load(reinterpret_cast<const int *>(ib + k * 4u));
-          v[u] = __builtin_nontemporal_load(
-              reinterpret_cast<const V *>(vb + k * (uint32_t)sizeof(V)));
-        } else {
-          const int64_t k = (int64_t)(j0 + u) * n + r;
-          c[u] = __builtin_nontemporal_load(A.index + k);
-          v[u] = __builtin_nontemporal_load(A.data + k);
-        }
-      }
+  // Chunks of up to kEllChunk slots, as many as the wave's longest row
+  // needs (uniform); a lane whose row ends inside a chunk sits it out.
+  const int W = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(len));
+  for (int j0 = 0; j0 < W; j0 += kEllChunk) {
+    if (j0 >= len) continue;
+    switch (min(kEllChunk, W - j0)) {
+      case 1: s = ell_chunk<1, V, X, O32>(A, x, r, len, j0, s); break;
+      case 2: s = ell_chunk<2, V, X, O32>(A, x, r, len, j0, s); break;
+      case 3: s = ell_chunk<3, V, X, O32>(A, x, r, len, j0, s); break;
+      case 4: s = ell_chunk<4, V, X, O32>(A, x, r, len, j0, s); break;
+      case 5: s = ell_chunk<5, V, X, O32>(A, x, r, len, j0, s); break;
+      case 6: s = ell_chunk<6, V, X, O32>(A, x, r, len, j0, s); break;
+      case 7: s = ell_chunk<7, V, X, O32>(A, x, r, len, j0, s); break;
+      default: s = ell_chunk<8, V, X, O32>(A, x, r, len, j0, s); break;
     }
-#pragma unroll
-    for (int u = 0; u < kEllChunk; u++)
-      if (j0 + u < len) s += X(v[u]) * x[c[u]];
   }
   return epi(r, s, pv);
 }
 
 // The same over packed FP32 slots (index | value bits << 32): one 8-byte
 // non-temporal load per slot.
+template <int R, typename XF>
+__device__ inline float ell_chunk(const EllPacked &A, const float *__restrict__ x, int r, int len,
+                                  int j0, float s, XF xf) {
+  const int64_t n = A.n;
+  uint64_t e[R];
+  float xv[R];
+#pragma unroll
+  for (int u = 0; u < R; u++)
+    e[u] = __builtin_nontemporal_load(A.slot + ((int64_t)min(j0 + u, len - 1) * n + r));
+#pragma unroll
+  for (int u = 0; u < R; u++) xv[u] = x[j0 + u < len ? (int)(uint32_t)e[u] : r];
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    const float p = __uint_as_float((uint32_t)(e[u] >> 32)) * xf(xv[u]);
+    s = j0 + u < len ? s + p : s;
+  }
+  return s;
+}
+
 template <typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__restrict__ x, int r,
                                          Pre pre, Epi epi, XF xf = XF()) {
-  const int64_t n = A.n;
   const int len = __builtin_nontemporal_load(A.length + r);
   const auto pv = pre(r);
   float s = 0.0f;
-  for (int j0 = 0; j0 < len; j0 += kEllChunk) {
-    uint64_t e[kEllChunk];
-#pragma unroll
-    for (int u = 0; u < kEllChunk; u++)
-      if (j0 + u < len) e[u] = __builtin_nontemporal_load(A.slot + ((int64_t)(j0 + u) * n + r));
-#pragma unroll
-    for (int u = 0; u < kEllChunk; u++)
-      if (j0 + u < len) s += __uint_as_float((uint32_t)(e[u] >> 32)) * xf(x[(int)(uint32_t)e[u]]);
+  const int W = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(len));
+  for (int j0 = 0; j0 < W; j0 += kEllChunk) {
+    if (j0 >= len) continue;
+    switch (min(kEllChunk, W - j0)) {
+      case 1: s = ell_chunk<1>(A, x, r, len, j0, s, xf); break;
+      case 2: s = ell_chunk<2>(A, x, r, len, j0, s, xf); break;
+      case 3: s = ell_chunk<3>(A, x, r, len, j0, s, xf); break;
+      case 4: s = ell_chunk<4>(A, x, r, len, j0, s, xf); break;
+      case 5: s = ell_chunk<5>(A, x, r, len, j0, s, xf); break;
+      case 6: s = ell_chunk<6>(A, x, r, len, j0, s, xf); break;
+      case 7: s = ell_chunk<7>(A, x, r, len, j0, s, xf); break;
+      default: s = ell_chunk<8>(A, x, r, len, j0, s, xf); break;
+    }
   }
   return epi(r, s, pv);
 }

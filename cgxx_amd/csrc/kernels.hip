@@ -470,6 +470,12 @@ int crs_rowblocks(int n, const int *ptr, int *starts) {
   return nb;
 }
 
+int crs_short_rows(int n, const int *ptr, int nrb) {
+  for (int i = 0; i < n; i++)
+    if (ptr[i + 1] - ptr[i] > kShortRow) return 0;
+  return nrb == (n + kRowsMax - 1) / kRowsMax;
+}
+
 static inline dim3 vgrid(int n) { return dim3(vec_grid(n)); }
 
 template <typename V, typename X>
