@@ -302,18 +302,27 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
 // 2-byte stream per diagonal and a mask stream: 256^3 Clenshaw step
 // 83.2 -> 75.9 us (the absent mark alone, per-diagonal streams kept:
 // 78.5 us; the record read as three 4-byte loads: 97.4 us).
-template <bool HALF, typename Pre, typename Epi, typename XF = XIdent>
-__device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__ x, int r, Pre pre,
-                                         Epi epi, XF xf = XF()) {
-  const uint32_t m = HALF ? 0u : __builtin_nontemporal_load(A.mask + r);
-  const auto pv = pre(r);
+// The loads of one DIA row (dia_row_load), kept apart from its products
+// (dia_row_finish) so that a walk can issue two rows' loads before it uses
+// either: dia_row_pre_epi is the two in sequence.
+template <typename PV>
+struct DiaRowLoads {
+  uint32_t v[kDiaMax];
+  float xv[kDiaMax];
+  uint32_t m;
+  PV pv;
+};
+
+template <bool HALF, typename Pre>
+__device__ inline auto dia_row_load(const Dia &A, const float *__restrict__ x, int r, Pre pre) {
+  DiaRowLoads<decltype(pre(0))> l;
+  l.m = HALF ? 0u : __builtin_nontemporal_load(A.mask + r);
+  l.pv = pre(r);
   const char *xb = reinterpret_cast<const char *>(x);
   // Every load is issued before any is used and none waits for the mask:
   // x is read at the column clamped into [0, n) for every diagonal, so
   // the mask (or the absent mark) only selects the products.  Raw value
   // bits first: a conversion next to its load would wait for it.
-  uint32_t v[kDiaMax];
-  float xv[kDiaMax];
   const int last = A.n - 1;
   if (HALF) {
     const int hw = half_words(A.ndiag);
@@ -332,31 +341,43 @@ __device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__
     }
 #pragma unroll
     for (int d = 0; d < kDiaMax; d++)
-      if (d < A.ndiag) v[d] = (w[d >> 1] >> (16 * (d & 1))) & 0xffffu;
+      if (d < A.ndiag) l.v[d] = (w[d >> 1] >> (16 * (d & 1))) & 0xffffu;
   } else {
     const char *vb = reinterpret_cast<const char *>(A.val);
     const uint32_t nv = (uint32_t)A.n * 4u, rv = (uint32_t)r * 4u;
 #pragma unroll
     for (int d = 0; d < kDiaMax; d++)
       if (d < A.ndiag)
-        v[d] = __builtin_nontemporal_load(
+        l.v[d] = __builtin_nontemporal_load(
             reinterpret_cast<const uint32_t *>(vb + (rv + (uint32_t)d * nv)));
   }
 #pragma unroll
   for (int d = 0; d < kDiaMax; d++)
     if (d < A.ndiag) {
       const int col = min(max(r + A.off[d], 0), last);
-      xv[d] = *reinterpret_cast<const float *>(xb + (uint32_t)col * 4u);
+      l.xv[d] = *reinterpret_cast<const float *>(xb + (uint32_t)col * 4u);
     }
+  return l;
+}
+
+template <bool HALF, typename L, typename Epi, typename XF>
+__device__ inline double dia_row_finish(const Dia &A, int r, const L &l, Epi epi, XF xf) {
   float s = 0.0f;
 #pragma unroll
   for (int d = 0; d < kDiaMax; d++)
-    if (d < A.ndiag && (HALF ? v[d] != kHalfAbsent : ((m >> d) & 1u) != 0u)) {
-      const float a = HALF ? (float)__builtin_bit_cast(_Float16, (uint16_t)v[d])
-                           : __builtin_bit_cast(float, v[d]);
-      s += a * xf(xv[d]);
+    if (d < A.ndiag && (HALF ? l.v[d] != kHalfAbsent : ((l.m >> d) & 1u) != 0u)) {
+      const float a = HALF ? (float)__builtin_bit_cast(_Float16, (uint16_t)l.v[d])
+                           : __builtin_bit_cast(float, l.v[d]);
+      s += a * xf(l.xv[d]);
     }
-  return epi(r, s, pv);
+  return epi(r, s, l.pv);
+}
+
+template <bool HALF, typename Pre, typename Epi, typename XF = XIdent>
+__device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__ x, int r, Pre pre,
+                                         Epi epi, XF xf = XF()) {
+  const auto l = dia_row_load<HALF>(A, x, r, pre);
+  return dia_row_finish<HALF>(A, r, l, epi, xf);
 }
 
 // ELL rows rb*256 + t: y = A x; with WITH_DOT, returns the owner's

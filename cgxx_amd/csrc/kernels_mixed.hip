@@ -193,7 +193,19 @@ __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, f
   double tpart = 0.0;
   int q0, qs, qend;
   if (K == kOpDia && STRIP && strip_walk(A.dia, q0, qs, qend)) {
-    for (int q = q0; q < qend; q += qs) {
+    // Two rows per trip, both rows' loads issued before either is used
+    // (their contributions added in walk order): twice the bytes in flight
+    // per wave for the steps that stream little besides the matrix.
+    int q = q0;
+    for (; q + qs < qend; q += 2 * qs) {
+      const int ra = strip_chunk(A.dia, q) * kBlock + (int)threadIdx.x;
+      const int rb = strip_chunk(A.dia, q + qs) * kBlock + (int)threadIdx.x;
+      const auto la = dia_row_load<HALF>(A.dia, x, ra, pre);
+      const auto lb = dia_row_load<HALF>(A.dia, x, rb, pre);
+      tpart += dia_row_finish<HALF>(A.dia, ra, la, epi, xf);
+      tpart += dia_row_finish<HALF>(A.dia, rb, lb, epi, xf);
+    }
+    if (q < qend) {
       const int r = strip_chunk(A.dia, q) * kBlock + (int)threadIdx.x;
       tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi, xf);
     }
