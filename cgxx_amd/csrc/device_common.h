@@ -417,6 +417,27 @@ __device__ inline double2 v2_stream(const double *p, int64_t j) {
   return make_double2(v.x, v.y);
 }
 
+// Pair j stored non-temporally: for a vector the next kernels do not read
+// (the iterate), so that its stream does not displace the vector the next
+// SpMV gathers from the last-level cache.
+__device__ inline void v2_store_stream(double *p, int64_t j, double2 v) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<d2 *>(p) + j);
+}
+
+// The same for quads of a float vector.
+__device__ inline float4 v4_stream(const float *p, int64_t j) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p) + j);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void v4_store_stream(float *p, int64_t j, float4 v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<f4 *>(p) + j);
+}
+
 // Quads (4q .. 4q+3) of a float vector: thread gid of T = 256*G visits quads
 // gid, gid + T, ... in order, elements in ascending order within a quad.
 struct Quads {

@@ -412,7 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
     st->next = s;
   }
   // x += a p always; p = z + b p unless the loop stops here.  Four streams
-  // per element: unrolled twice to stay within 64 VGPRs.
+  // per element: unrolled twice to stay within 64 VGPRs.  x, r and C are
+  // streamed non-temporally: only p stays cached for the next SpMV.
   constexpr int U = 2;
   Pairs P(n);
   if (converged) {
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int64_t j = P.q + u * P.T;
-      xv[u] = v2(x)[j];
+      xv[u] = v2_stream(x, j);
       pv[u] = v2(p)[j];
       rv[u] = v2_stream(r, j);
       if (JAC) cv[u] = v2_stream(C, j);
@@ -439,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__
       const double zy = JAC ? cv[u].y * rv[u].y : rv[u].y;
       pv[u].x = zx + b * pv[u].x;
       pv[u].y = zy + b * pv[u].y;
-      v2(x)[j] = xv[u];
+      v2_store_stream(x, j, xv[u]);
       v2(p)[j] = pv[u];
     }
   }

@@ -407,7 +407,9 @@ __global__ __launch_bounds__(kBlock) void k_mp_update(int n, int gpq, float *__r
   if (threadIdx.x == 0) prr.a[blockIdx.x] = b0;
 }
 
-// y32 += a p32 and p32 = u + b p32 over quads (the old p read once).
+// y32 += a p32 and p32 = u + b p32 over quads (the old p read once); y32,
+// read again only by a reliable update, is streamed non-temporally so that
+// p32 stays cached for the next SpMV.
 __device__ inline void mp_yp_update(int n, float *__restrict__ y, float *__restrict__ p,
                                     const float *__restrict__ u, float a, float b) {
   Quads Q(n);
@@ -416,7 +418,7 @@ __device__ inline void mp_yp_update(int n, float *__restrict__ y, float *__restr
 #pragma unroll
     for (int k = 0; k < kQuadUnroll; k++) {
       const int64_t j = Q.q + k * Q.T;
-      yv[k] = v4(y)[j];
+      yv[k] = v4_stream(y, j);
       pv[k] = v4(p)[j];
       uv[k] = v4(u)[j];
     }
@@ -432,7 +434,7 @@ __device__ inline void mp_yp_update(int n, float *__restrict__ y, float *__restr
       po.y = uv[k].y + b * pv[k].y;
       po.z = uv[k].z + b * pv[k].z;
       po.w = uv[k].w + b * pv[k].w;
-      v4(y)[j] = yo;
+      v4_store_stream(y, j, yo);
       v4(p)[j] = po;
     }
   }
