@@ -96,7 +96,6 @@ struct cghip_ctx {
   bool dia_nd = false;
   float *dia_val = nullptr;
   uint16_t *dia_val16 = nullptr;  // FP16 copy for the polynomial's Clenshaw steps
-  uint8_t *dia_mask = nullptr;
   float *v32[4] = {};
   MixedState *d_mstate = nullptr;
   MixedState *h_mstate = nullptr;  // pinned
@@ -184,7 +183,6 @@ void free_fp32(cghip_ctx *c) {
   dfree(c->ell_slot_nd);
   dfree(c->dia_val);
   dfree(c->dia_val16);
-  dfree(c->dia_mask);
   c->dia_nd = false;
   c->ell_nd = false;
   c->ell_w_nd = 0;
@@ -251,8 +249,7 @@ Ell<double> ell_of(cghip_ctx *c) {
 }
 
 Dia dia_of(cghip_ctx *c) {
-  Dia d{c->dia_val, c->dia_val16, c->dia_mask, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1,
-        0, 0};
+  Dia d{c->dia_val, c->dia_val16, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1, 0, 0};
   for (int k = 0; k < kDiaMax; k++) d.off[k] = c->dia_off[k];
   dia_strips(d);
   return d;
@@ -889,16 +886,14 @@ int cghip_prepare_fp32(cghip_ctx *c) {
       const size_t slots = (size_t)c->dia_ndiag * (size_t)n;
       rc = dalloc(c, &c->dia_val, slots);
       if (!rc) rc = dalloc(c, &c->dia_val16, 2 * (size_t)half_words(c->dia_ndiag) * (size_t)n);
-      if (!rc) rc = dalloc(c, &c->dia_mask, (size_t)n);
       if (rc) {
         free_fp32(c);
         return rc;
       }
       int *d_over = reinterpret_cast<int *>(c->dot_out);
       HIPCHK(c, hipMemsetAsync(d_over, 0, sizeof(int), c->stream));
-      HIPCHK(c, hipMemsetAsync(c->dia_val, 0, sizeof(float) * slots, c->stream));
-      HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_val16,
-                           c->dia_mask, d_over, c->stream));
+      HIPCHK(c, scaled_dia(crs_of(c), c->scale64, dia_of(c), c->dia_val, c->dia_val16, d_over,
+                           c->stream));
       int over = 0;
       HIPCHK(c, hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       if (int r = sync(c)) return r;
@@ -906,7 +901,6 @@ int cghip_prepare_fp32(cghip_ctx *c) {
       if (over) {  // no FP16 copy possible: the next layout instead
         dfree(c->dia_val);
         dfree(c->dia_val16);
-        dfree(c->dia_mask);
         c->dia_nd = false;
       }
     }

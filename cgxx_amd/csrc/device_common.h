@@ -293,35 +293,35 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
 // ascending diagonal order (the CRS order of a row with ascending columns)
 // summed left to right from 0, then epi(r, s, pv).  Values are read once
 // (non-temporal); x at r + off[d] is contiguous across a wave.
-// FP32 values: one stream per diagonal, the mask byte says which entries
-// are stored.  HALF: the row's FP16 record (half_words(ndiag) words: one
-// 12-byte load for the 6 diagonals of a 3D 7-point stencil, else one load
-// per word), an entry stored unless its bits are kHalfAbsent; converted
-// exactly to FP32.  Both skip absent entries (never a product with zero),
-// so both give the row sums of the CRS order.  The record instead of one
-// 2-byte stream per diagonal and a mask stream: 256^3 Clenshaw step
-// 83.2 -> 75.9 us (the absent mark alone, per-diagonal streams kept:
-// 78.5 us; the record read as three 4-byte loads: 97.4 us).
-// The loads of one DIA row (dia_row_load), kept apart from its products
+// FP32 values: one stream per diagonal, an entry stored unless its bits are
+// kF32Absent (a row record of the six FP32 values instead measured slower:
+// 118 against 111 us).  HALF: the row's FP16 record (half_words(ndiag)
+// words: one 12-byte load for the 6 diagonals of a 3D 7-point stencil, else
+// one load per word), an entry stored unless its bits are kHalfAbsent;
+// converted exactly to FP32.  Both skip absent entries (never a product
+// with zero), so both give the row sums of the CRS order.  The record
+// instead of one 2-byte stream per diagonal and a mask stream: 256^3
+// Clenshaw step 83.2 -> 75.9 us (the absent mark alone, per-diagonal
+// streams kept: 78.5 us; the record read as three 4-byte loads: 97.4 us).
+//
+// The loads of one DIA row (dia_row_load) are kept apart from its products
 // (dia_row_finish) so that a walk can issue two rows' loads before it uses
 // either: dia_row_pre_epi is the two in sequence.
 template <typename PV>
 struct DiaRowLoads {
   uint32_t v[kDiaMax];
   float xv[kDiaMax];
-  uint32_t m;
   PV pv;
 };
 
 template <bool HALF, typename Pre>
 __device__ inline auto dia_row_load(const Dia &A, const float *__restrict__ x, int r, Pre pre) {
   DiaRowLoads<decltype(pre(0))> l;
-  l.m = HALF ? 0u : __builtin_nontemporal_load(A.mask + r);
   l.pv = pre(r);
   const char *xb = reinterpret_cast<const char *>(x);
-  // Every load is issued before any is used and none waits for the mask:
-  // x is read at the column clamped into [0, n) for every diagonal, so
-  // the mask (or the absent mark) only selects the products.  Raw value
+  // Every load is issued before any is used and none waits for the
+  // values: x is read at the column clamped into [0, n) for every
+  // diagonal, so the absent mark only selects the products.  Raw value
   // bits first: a conversion next to its load would wait for it.
   const int last = A.n - 1;
   if (HALF) {
@@ -365,7 +365,7 @@ __device__ inline double dia_row_finish(const Dia &A, int r, const L &l, Epi epi
   float s = 0.0f;
 #pragma unroll
   for (int d = 0; d < kDiaMax; d++)
-    if (d < A.ndiag && (HALF ? l.v[d] != kHalfAbsent : ((l.m >> d) & 1u) != 0u)) {
+    if (d < A.ndiag && (HALF ? l.v[d] != kHalfAbsent : l.v[d] != kF32Absent)) {
       const float a = HALF ? (float)__builtin_bit_cast(_Float16, (uint16_t)l.v[d])
                            : __builtin_bit_cast(float, l.v[d]);
       s += a * xf(l.xv[d]);

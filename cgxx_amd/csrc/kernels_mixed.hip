@@ -108,21 +108,21 @@ __global__ __launch_bounds__(kBlock) void k_scaled_ell(Crs<double> A, const doub
 
 // DIA copy of the scaled off-diagonal entries: entry (i, c) goes to
 // diagonal d with D.off[d] == c - i (every off-diagonal offset of the
-// matrix is in D.off, checked on the host); val is zero where no entry is,
-// and row i's FP16 record holds kHalfAbsent there.
+// matrix is in D.off, checked on the host); row i's slots of val hold
+// kF32Absent and its FP16 record kHalfAbsent where no entry is.
 __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const double *__restrict__ s,
                                                        Dia D, float *__restrict__ val,
                                                        uint16_t *__restrict__ val16,
-                                                       uint8_t *__restrict__ mask,
                                                        int *__restrict__ overflow) {
   const int64_t T = (int64_t)gridDim.x * kBlock;
   const int64_t n = A.n;
   const int hw = half_words(D.ndiag);
+  const float absent = __builtin_bit_cast(float, kF32Absent);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
-    uint32_t m = 0;
     const double si = s[i];
     uint16_t *rec = val16 + (int64_t)i * 2 * hw;
     for (int h = 0; h < 2 * hw; h++) rec[h] = kHalfAbsent;
+    for (int d = 0; d < D.ndiag; d++) val[(int64_t)d * n + i] = absent;
     for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
       const int c = A.index[e];
       if (c == (int)i) continue;
@@ -133,9 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
       if (!(fabsf(v) < 65520.0f)) overflow[0] = 1;  // FP16 would round it to inf
       val[(int64_t)d * n + i] = v;
       rec[d] = __builtin_bit_cast(uint16_t, (_Float16)v);
-      m |= 1u << d;
     }
-    mask[i] = (uint8_t)m;
   }
 }
 
@@ -607,9 +605,9 @@ hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64
 }
 
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
-                      uint16_t *val16, uint8_t *mask, int *overflow, hipStream_t st) {
+                      uint16_t *val16, int *overflow, hipStream_t st) {
   hipLaunchKernelGGL(k_scaled_dia, dim3(row_grid(A.n)), dim3(kBlock), 0, st, A, s, D, val, val16,
-                     mask, overflow);
+                     overflow);
   return hipGetLastError();
 }
 
