@@ -46,13 +46,59 @@ __device__ inline double block_fold(double v, double *lds) {
   return r;
 }
 
+// Thread t's left-to-right sum of partials t, t + 256, ... (G >= 1 of
+// them, written by an earlier kernel).  Loads go out 8 at a time, clamped
+// to the last partial and dropped by a select past G: a loop of load then
+// add waited for every load (up to 24 cache round trips before a vector
+// kernel could start streaming).
+__device__ inline double thread_partials(const double *partial, int G) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < G; i += 8 * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = partial[min(i + u * kBlock, G - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s = i + u * kBlock < G ? s + v[u] : s;
+  }
+  return s;
+}
+
+// The same for two arrays (ga, gb >= 1 partials) with both arrays' loads in
+// flight together.
+__device__ inline void thread_partials2(const double *a, int ga, const double *b, int gb,
+                                        double &sa, double &sb) {
+  sa = 0.0;
+  sb = 0.0;
+  const int g = max(ga, gb);
+  for (int i = threadIdx.x; i < g; i += 8 * kBlock) {
+    double va[8], vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      va[u] = a[min(i + u * kBlock, ga - 1)];
+      vb[u] = b[min(i + u * kBlock, gb - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      sa = i + u * kBlock < ga ? sa + va[u] : sa;
+      sb = i + u * kBlock < gb ? sb + vb[u] : sb;
+    }
+  }
+}
+
 // Fold of G partials written by an earlier kernel: thread t adds partials
 // t, t + 256, ... left to right, then the block fold.  Every block that calls
 // it gets the same value.
 __device__ inline double fold_partials(const double *partial, int G, double *lds) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < G; i += kBlock) s += partial[i];
-  return block_fold(s, lds);
+  return block_fold(thread_partials(partial, G), lds);
+}
+
+// Two such folds, their loads in flight together.
+__device__ inline void fold_partials2(const double *a, int ga, const double *b, int gb,
+                                      double *lds, double &ra, double &rb) {
+  double sa, sb;
+  thread_partials2(a, ga, b, gb, sa, sb);
+  ra = block_fold(sa, lds);
+  rb = block_fold(sb, lds);
 }
 
 // ---------------------------------------------------------------------------

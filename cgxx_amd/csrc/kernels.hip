@@ -292,8 +292,8 @@ template <bool JAC>
 __global__ __launch_bounds__(kBlock) void k_pcg_init_finish(Partials pa, int G, SolveState *st,
                                                             History h) {
   __shared__ double lds[4];
-  const double r2 = fold_partials(pa.a, G, lds);
-  const double pr = fold_partials(pa.b, G, lds);
+  double r2, pr;
+  fold_partials2(pa.a, G, pa.b, G, lds, r2, pr);
   if (threadIdx.x == 0) {
     Scalars s{};
     s.r2 = r2;
@@ -375,17 +375,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 // b = rho / rho_old (CG.cpp:357-370) and p = z + b p with z = C o r
 // recomputed (CG.cpp:375-381).  Writes `next`.
 template <bool JAC>
-__global__ __launch_bounds__(kBlock) void k_pcg_xpay(int n, double *__restrict__ x,
-                                                     double *__restrict__ p,
-                                                     const double *__restrict__ r,
-                                                     const double *__restrict__ C, Partials prr,
-                                                     SolveState *st, History h) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pcg_xpay(
+    int n, double *__restrict__ x, double *__restrict__ p, const double *__restrict__ r,
+    const double *__restrict__ C, Partials prr, SolveState *st, History h) {
   __shared__ double lds[4];
   const Scalars c = st->cur;
   if (c.done) return;
   const int G = gridDim.x;  // the update kernel ran on the same grid
-  const double r2 = fold_partials(prr.a, G, lds);
-  const double rz = fold_partials(prr.b, G, lds);
+  double r2, rz;
+  fold_partials2(prr.a, G, prr.b, G, lds, r2, rz);
   const double res = sqrt(r2) / st->nrm2_0;
   const bool converged = res <= st->tol;
   const double rho = JAC ? rz : r2;

@@ -292,8 +292,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart_fold(Partials prt, int gt
                                                             const double *rzp, int gz, int first,
                                                             MixedState *st, History h) {
   __shared__ double lds[4];
-  const double rt2 = fold_partials(prt.a, gt, lds);
-  const double rr = fold_partials(prr.a, gr, lds);
+  double rt2, rr;
+  fold_partials2(prt.a, gt, prr.a, gr, lds, rt2, rr);
   const double rz = fold_partials(rzp, gz, lds);
   if (threadIdx.x == 0) {
     MixedScalars s = st->next;
@@ -553,8 +553,8 @@ __global__ __launch_bounds__(kBlock) void k_mp_xpay_poly(int n, float *__restric
   __shared__ double lds[4];
   const MixedScalars c = st->cur;
   if (c.done) return;
-  const double rr = fold_partials(prr.a, gridDim.x, lds);
-  const double rz = fold_partials(prz.a, gz, lds);
+  double rr, rz;
+  fold_partials2(prr.a, gridDim.x, prz.a, gz, lds, rr, rz);
   const double beta = rz / c.rho;
   const float a = st->alpha32;
   if (blockIdx.x == 0 && threadIdx.x == 0) mp_advance(st, c, rr, rz, beta, h);
