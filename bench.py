@@ -91,7 +91,7 @@ def algorithmic_bytes(short: str, meta: dict):
     table = [("k_pcg_matvec_dot_crs", crs_bytes(n, nnz, 8, 8)), ("k_matvec_crs<double", crs_bytes(n, nnz, 8, 8)),
              ("k_pcg_matvec_dot_ell", ell_bytes(n, nnz, 8, 8)), ("k_matvec_ell<double", ell_bytes(n, nnz, 8, 8)),
              ("k_pcg_update<true>", 32 * n), ("k_pcg_update<false>", 24 * n), ("k_pcg_xpay<true>", 48 * n),
-             ("k_pcg_xpay<false>", 40 * n), ("k_mp_matvec_dot<2>", 4 * D * n + n + 8 * n),
+             ("k_pcg_xpay<false>", 40 * n), ("k_mp_matvec_dot<2>", 4 * D * n + 8 * n),
              ("k_mp_update", 12 * n), ("k_mp_xpay_poly", 20 * n), ("k_mp_xpay", 20 * n),
              ("k_poly_step<false, 2, 1,", rec + 8 * n),
              ("k_poly_step<false, 2, 2,", rec + 12 * n),
@@ -361,9 +361,9 @@ def main() -> int:
     ndiag = int(offs.size - (offs == 0).sum())
     del rows
     # operator stream of one FP32 SpMV of the scaled operator (without x, y)
-    op_bytes = {"DIA": 4 * ndiag * n + n, "ELL": 8 * nnz_nd + 4 * n,
+    op_bytes = {"DIA": 4 * ndiag * n, "ELL": 8 * nnz_nd + 4 * n,
                 "CRS": 8 * nnz_nd + 4 * (n + 1)}[layout]
-    # the Clenshaw steps read each row's FP16 record (no mask byte)
+    # the Clenshaw steps read each row's FP16 record
     poly_op_bytes = 4 * ((ndiag + 1) // 2) * n if layout == "DIA" else op_bytes
     kind = {"CRS": 0, "ELL": 1, "DIA": 2}[layout]
     log(f"uploaded FP64 CRS + ELL, Jacobi, FP32 copy and scaled FP32 operator ({layout}"

@@ -1303,11 +1303,12 @@ int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) 
   if (precond == CGHIP_PRECOND_POLY && !c->has_poly)
     return fail(cc, CGHIP_ERR_STATE, "call cghip_set_poly first");
   const double nnz = c->nnz_nd, d = c->dia_ndiag;
-  // operator stream of one SpMV: FP32 values in the CG SpMV; the Clenshaw
-  // steps read DIA's FP16 copy
-  const double op32 = c->dia_nd ? 4.0 * d * n + n : c->ell_nd ? 8.0 * nnz + 4.0 * n
-                                                              : 8.0 * nnz + 4.0 * (n + 1);
-  const double op16 = c->dia_nd && c->poly_fp16 ? 2.0 * d * n + n : op32;
+  // operator stream of one SpMV: FP32 values in the CG SpMV (DIA: one
+  // stream per diagonal, no mask); the Clenshaw steps read DIA's FP16 row
+  // records
+  const double op32 = c->dia_nd ? 4.0 * d * n : c->ell_nd ? 8.0 * nnz + 4.0 * n
+                                                          : 8.0 * nnz + 4.0 * (n + 1);
+  const double op16 = c->dia_nd && c->poly_fp16 ? 4.0 * half_words(c->dia_ndiag) * n : op32;
   const double spmv = op32 + 8.0 * n;
   if (precond == CGHIP_PRECOND_JACOBI) {
     *bytes = spmv + 12.0 * n + 20.0 * n;  // + r update (r, q; r) + y, p update (y, p, r; y, p)
