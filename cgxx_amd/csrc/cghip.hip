@@ -648,13 +648,17 @@ int enqueue_iteration(cghip_ctx *c, const double *C, History h, hipEvent_t *ev) 
          *x = c->vec[CGHIP_VEC_X];
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   const int G = spmv_blocks(c);
+  // G <= 256: the update folds the SpMV's partials itself (no group fold)
+  const bool direct = G <= kBlock;
   const Partials pq1{c->pq1, nullptr}, pq2{c->pq1 + G, nullptr};
+  double *grp = direct ? nullptr : pq2.a;
   if (c->format == CGHIP_FORMAT_CRS)
-    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, pq1, pq2.a, c->d_state, c->stream));
+    HIPCHK(c, pcg_matvec_dot_crs(crs_of(c), p, q, pq1, grp, c->d_state, c->stream));
   else
-    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, pq1, pq2.a, c->d_state, c->stream));
+    HIPCHK(c, pcg_matvec_dot_ell(ell_of(c), p, q, pq1, grp, c->d_state, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  HIPCHK(c, pcg_update(c->n, fold_groups(G), r, q, C, pq2, pr_of(c), c->d_state, h, c->stream));
+  HIPCHK(c, pcg_update(c->n, direct ? -G : fold_groups(G), r, q, C, direct ? pq1 : pq2, pr_of(c),
+                       c->d_state, h, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   HIPCHK(c, pcg_xpay(c->n, x, p, r, C, pr_of(c), c->d_state, h, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[3], c->stream));

@@ -155,13 +155,17 @@ hipError_t residual_norms(int n, const double *b, const double *q, Partials pa, 
 hipError_t pcg_init(int n, const double *k, double *r, double *p, const double *C, Partials pa,
                     SolveState *st, History h, hipStream_t s);
 // q = A p with one partial of <p,q> per row block (CRS) or 256-row chunk
-// (ELL) in pq.a, then those partials folded in groups of 256 into
-// pq2[0 .. fold_groups(G)), G the number of row blocks / chunks.
+// (ELL) in pq.a, then (pq2 non-null) those partials folded in groups of
+// 256 into pq2[0 .. fold_groups(G)), G the number of row blocks / chunks.
+// With G <= 256 the loop passes pq2 = null and the update folds the G
+// partials itself (pcg_update with gpq = -G).
 hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, Partials pq,
                               double *pq2, SolveState *st, hipStream_t s);
 hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, Partials pq,
                               double *pq2, SolveState *st, hipStream_t s);
-// alpha = rho / <p,q>; r -= alpha q; partials of <r,r> and <r,Cr>
+// alpha = rho / <p,q>; r -= alpha q; partials of <r,r> and <r,Cr>.  <p,q>
+// from gpq > 0 group sums in pq.a, or (gpq = -G, G <= 256) from the G
+// block partials themselves, folded to the same bits as their one group.
 hipError_t pcg_update(int n, int gpq, double *r, const double *q, const double *C, Partials pq,
                       Partials pr, SolveState *st, History h, hipStream_t s);
 // x += alpha p (the old p: each p element is read once outside the SpMV);

@@ -320,16 +320,22 @@ __device__ inline void update_elem(double &rv, double qv, double cv, double na, 
 // a = rho / <p,q> (CG.cpp:334), kept in st->alpha; r += (-a) q (CG.cpp:342;
 // the x += a p of CG.cpp:340 is done by the last kernel, which reads p
 // anyway); partials of <r,r> and <r,z> with z = C o r (CG.cpp:344, 363-366).
+// gpq > 0: <p,q> folds the gpq group sums in ppq.a (k_fold_groups).
+// gpq = -G (G <= 256, small systems, one launch fewer per iteration): the
+// block folds the G SpMV partials as k_fold_groups folds its one group,
+// and 0.0 + that sum is what folding the single group sum gives (a sum
+// with only zeros added, except that -0.0 becomes +0.0), so the bits are
+// those of the two-level fold.
 template <bool JAC>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_pcg_update(
-    int n, int gpq, double *__restrict__ r,
-                                                       const double *__restrict__ q,
-                                                       const double *__restrict__ C, Partials ppq,
-                                                       Partials prr, SolveState *st, History h) {
+    int n, int gpq, double *__restrict__ r, const double *__restrict__ q,
+    const double *__restrict__ C, Partials ppq, Partials prr, SolveState *st, History h) {
   __shared__ double lds[4];
   const Scalars c = st->cur;
   if (c.done) return;
-  const double pq = fold_partials(ppq.a, gpq, lds);
+  const double pq =
+      gpq > 0 ? fold_partials(ppq.a, gpq, lds)
+              : 0.0 + block_fold((int)threadIdx.x < -gpq ? ppq.a[threadIdx.x] : 0.0, lds);
   const double a = c.rho / pq;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha = a;
@@ -563,7 +569,8 @@ hipError_t pcg_matvec_dot_crs(const Crs<double> &A, const double *p, double *q, 
                               double *pq2, SolveState *st, hipStream_t s) {
   const int G = A.nrb > 0 ? A.nrb : 1;
   hipLaunchKernelGGL(k_pcg_matvec_dot_crs, dim3(G), dim3(kBlock), 0, s, A, p, q, pq, st);
-  hipLaunchKernelGGL(k_fold_groups, dim3(fold_groups(G)), dim3(kBlock), 0, s, pq.a, G, pq2);
+  if (pq2)
+    hipLaunchKernelGGL(k_fold_groups, dim3(fold_groups(G)), dim3(kBlock), 0, s, pq.a, G, pq2);
   return hipGetLastError();
 }
 
@@ -574,7 +581,8 @@ hipError_t pcg_matvec_dot_ell(const Ell<double> &A, const double *p, double *q, 
     hipLaunchKernelGGL(k_pcg_matvec_dot_ell<true>, dim3(G), dim3(kBlock), 0, s, A, p, q, pq, st);
   else
     hipLaunchKernelGGL(k_pcg_matvec_dot_ell<false>, dim3(G), dim3(kBlock), 0, s, A, p, q, pq, st);
-  hipLaunchKernelGGL(k_fold_groups, dim3(fold_groups(G)), dim3(kBlock), 0, s, pq.a, G, pq2);
+  if (pq2)
+    hipLaunchKernelGGL(k_fold_groups, dim3(fold_groups(G)), dim3(kBlock), 0, s, pq.a, G, pq2);
   return hipGetLastError();
 }
 
