@@ -232,21 +232,62 @@ __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, f
 // blocks of 256 per CU; the CRS and ELL instances use fewer registers.
 #define CGHIP_SGPR_CAP __attribute__((amdgpu_num_sgpr(74)))
 
+// x64 += scale o (double)y32 (scale nullable: plain add), over element
+// pairs, two pairs in flight per thread (an element at a time ran at
+// 5.5 TB/s; four pairs need more than 64 VGPRs).
 __global__ __launch_bounds__(kBlock) void k_mp_accum(int n, double *__restrict__ x64,
                                                      const float *__restrict__ y32,
                                                      const double *__restrict__ scale) {
-  const int64_t T = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T)
-    x64[i] = x64[i] + (scale ? scale[i] * (double)y32[i] : (double)y32[i]);
+  constexpr int U = 2;
+  Pairs P(n);
+  const float2 *y2 = reinterpret_cast<const float2 *>(y32);
+  for (; P.q + (U - 1) * P.T < P.full; P.q += U * P.T) {
+    double2 xv[U], sv[U];
+    float2 yv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t j = P.q + u * P.T;
+      xv[u] = v2(x64)[j];
+      yv[u] = y2[j];
+      sv[u] = scale ? v2_stream(scale, j) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t j = P.q + u * P.T;
+      xv[u].x = xv[u].x + (scale ? sv[u].x * (double)yv[u].x : (double)yv[u].x);
+      xv[u].y = xv[u].y + (scale ? sv[u].y * (double)yv[u].y : (double)yv[u].y);
+      v2(x64)[j] = xv[u];
+    }
+  }
+  for (; P.q < P.npair; P.q += P.T)
+    for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++)
+      x64[e] = x64[e] + (scale ? scale[e] * (double)y32[e] : (double)y32[e]);
 }
 
-// r64 = k - q64 and the partials of ||r64||^2 (pairs order).
+// r64 = k - q64 and the partials of ||r64||^2 (pairs order: a thread's
+// pairs in turn, element 2j before 2j + 1), kUnroll pairs in flight.
 __global__ __launch_bounds__(kBlock) void k_mp_residual(int n, const double *__restrict__ k,
                                                         const double *__restrict__ q,
                                                         double *__restrict__ r, Partials pa) {
   __shared__ double lds[4];
   Pairs P(n);
   double s = 0.0;
+  for (; P.q + (kUnroll - 1) * P.T < P.full; P.q += kUnroll * P.T) {
+    double2 kv[kUnroll], qv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t j = P.q + u * P.T;
+      kv[u] = v2_stream(k, j);
+      qv[u] = v2_stream(q, j);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const double r0 = kv[u].x - qv[u].x, r1 = kv[u].y - qv[u].y;
+      v2(r)[P.q + u * P.T] = make_double2(r0, r1);
+      s += r0 * r0;
+      s += r1 * r1;
+    }
+  }
   for (; P.q < P.npair; P.q += P.T) {
     for (int64_t e = 2 * P.q; e < 2 * P.q + 2 && e < n; e++) {
       const double ri = k[e] - q[e];
@@ -258,12 +299,13 @@ __global__ __launch_bounds__(kBlock) void k_mp_residual(int n, const double *__r
   if (threadIdx.x == 0) pa.a[blockIdx.x] = bp;
 }
 
-__device__ inline float restart_value(const double *r64, const double *scale, int64_t e) {
-  return scale ? (float)(scale[e] * r64[e]) : (float)r64[e];
+__device__ inline float restart_value(double r, const double *scale, double sv) {
+  return scale ? (float)(sv * r) : (float)r;
 }
 
 // r32 = (float)(s o r64), y32 = 0, p32 = r32 when first; partials of
-// <r32,r32> (quads order).
+// <r32,r32> (quads order: a thread's quads in turn, elements ascending),
+// kQuadUnroll quads in flight (an element at a time: 144 us at 256^3).
 __global__ __launch_bounds__(kBlock) void k_mp_restart(int n, const double *__restrict__ r64,
                                                        const double *__restrict__ scale,
                                                        float *__restrict__ r32,
@@ -273,9 +315,39 @@ __global__ __launch_bounds__(kBlock) void k_mp_restart(int n, const double *__re
   __shared__ double lds[4];
   Quads Q(n);
   double s_rr = 0.0;
+  for (; Q.q + (kQuadUnroll - 1) * Q.T < Q.full; Q.q += kQuadUnroll * Q.T) {
+    double2 ra[kQuadUnroll], rb[kQuadUnroll], sa[kQuadUnroll], sb[kQuadUnroll];
+#pragma unroll
+    for (int u = 0; u < kQuadUnroll; u++) {
+      const int64_t j = Q.q + u * Q.T;
+      ra[u] = v2_stream(r64, 2 * j);
+      rb[u] = v2_stream(r64, 2 * j + 1);
+      if (scale) {
+        sa[u] = v2_stream(scale, 2 * j);
+        sb[u] = v2_stream(scale, 2 * j + 1);
+      } else {
+        sa[u] = sb[u] = make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kQuadUnroll; u++) {
+      const int64_t j = Q.q + u * Q.T;
+      const float4 rv = make_float4(restart_value(ra[u].x, scale, sa[u].x),
+                                    restart_value(ra[u].y, scale, sa[u].y),
+                                    restart_value(rb[u].x, scale, sb[u].x),
+                                    restart_value(rb[u].y, scale, sb[u].y));
+      v4(r32)[j] = rv;
+      v4(y32)[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (first) v4(p32)[j] = rv;
+      s_rr += (double)rv.x * (double)rv.x;
+      s_rr += (double)rv.y * (double)rv.y;
+      s_rr += (double)rv.z * (double)rv.z;
+      s_rr += (double)rv.w * (double)rv.w;
+    }
+  }
   for (; Q.q < Q.nquad; Q.q += Q.T) {
     for (int64_t e = 4 * Q.q; e < 4 * Q.q + 4 && e < n; e++) {
-      const float r = restart_value(r64, scale, e);
+      const float r = restart_value(r64[e], scale, scale ? scale[e] : 0.0);
       r32[e] = r;
       y32[e] = 0.0f;
       if (first) p32[e] = r;
