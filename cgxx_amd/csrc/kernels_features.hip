@@ -43,35 +43,52 @@ __device__ inline double block_max(double v, double *lds) {
 
 __global__ __launch_bounds__(kBlock) void k_matrix_stats(Crs<double> A, double *__restrict__ out) {
   __shared__ double lds[4];
-  double cnt[kStatQuantities] = {}, sum[kStatQuantities] = {}, sq[kStatQuantities] = {};
+  // counts in 32-bit integers (exact, half the registers of doubles;
+  // converted for the folds, which then add the same whole numbers)
+  int cnt[kStatQuantities] = {};
+  double sum[kStatQuantities] = {}, sq[kStatQuantities] = {};
   double mn[kStatQuantities], mx[kStatQuantities];
   for (int q = 0; q < kStatQuantities; q++) {
     mn[q] = INFINITY;
     mx[q] = -INFINITY;
   }
-  double hist[kStatHistBins] = {};
-  double bad = 0.0;
+  int hist[kStatHistBins] = {};
+  int bad = 0;
   const int T = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += T) {
     const int e0 = A.ptr[i], e1 = A.ptr[i + 1];
     double d = 0.0, off = 0.0, smax = 0.0, smin = INFINITY;
     bool has_d = false;
-    for (int e = e0; e < e1; e++) {
-      const double v = A.value[e];
-      if (A.index[e] == i) {
-        d = v;
-        has_d = true;
-      } else if (v != 0.0) {
-        const double a = fabs(v);
-        off += a;
-        smax = fmax(smax, a);
-        smin = fmin(smin, a);
+    // Entries 8 at a time, every load in flight before the first is used
+    // (clamped to the row's last entry, dropped past it), then taken in
+    // stored order through selects: a loop of load then use waited for
+    // every entry (593 us at 256^3).
+    for (int e = e0; e < e1; e += 8) {
+      int c[8];
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int k = min(e + u, e1 - 1);
+        c[u] = A.index[k];
+        v[u] = A.value[k];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const bool live = e + u < e1;
+        const bool diag = live & (c[u] == i);
+        const bool offd = live & !diag & (v[u] != 0.0);
+        const double a = fabs(v[u]);
+        d = diag ? v[u] : d;
+        has_d = has_d | diag;
+        off = offd ? off + a : off;
+        smax = offd ? fmax(smax, a) : smax;
+        smin = offd ? fmin(smin, a) : smin;
       }
     }
     const int len = e1 - e0;
-    hist[len >= kStatHistBins ? kStatHistBins - 1 : (len < 1 ? 0 : len - 1)] += 1.0;
+    hist[len >= kStatHistBins ? kStatHistBins - 1 : (len < 1 ? 0 : len - 1)] += 1;
     if (!has_d || !(d > 0.0)) {
-      bad += 1.0;
+      bad += 1;
       continue;
     }
     double qv[kStatQuantities];
@@ -86,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_matrix_stats(Crs<double> A, double *
     live[2] = live[3] = live[4] = has_off;
     for (int q = 0; q < kStatQuantities; q++) {
       if (!live[q]) continue;
-      cnt[q] += 1.0;
+      cnt[q] += 1;
       sum[q] += qv[q];
       sq[q] += qv[q] * qv[q];
       mn[q] = fmin(mn[q], qv[q]);
@@ -95,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_matrix_stats(Crs<double> A, double *
   }
   double *o = out + (size_t)blockIdx.x * kStatWidth;
   for (int q = 0; q < kStatQuantities; q++) {
-    const double c = block_fold(cnt[q], lds);
+    const double c = block_fold((double)cnt[q], lds);
     const double s = block_fold(sum[q], lds);
     const double s2 = block_fold(sq[q], lds);
     const double lo = block_min(mn[q], lds);
@@ -109,10 +126,10 @@ __global__ __launch_bounds__(kBlock) void k_matrix_stats(Crs<double> A, double *
     }
   }
   for (int b = 0; b < kStatHistBins; b++) {
-    const double h = block_fold(hist[b], lds);
+    const double h = block_fold((double)hist[b], lds);
     if (threadIdx.x == 0) o[5 * kStatQuantities + b] = h;
   }
-  const double nb = block_fold(bad, lds);
+  const double nb = block_fold((double)bad, lds);
   if (threadIdx.x == 0) o[5 * kStatQuantities + kStatHistBins] = nb;
 }
 
