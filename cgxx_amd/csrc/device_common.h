@@ -230,10 +230,13 @@ __device__ inline double crs_rowblock(const Crs<V> &A, const X *__restrict__ x, 
 // addresses per slot.
 //
 // Slots j0 .. j0 + R - 1 of row r (len > j0), added to s left to right.
-// Straight-line code as in crs_stage: a slot past the row's length loads
-// the row's last live slot again (in bounds; the gather then reads x[r]) and
+// Straight-line code, every slot load and then every gather in flight
+// before the first product (loads under per-lane branches had made each
+// gather wait for the one before): a slot past the row's length loads the
+// row's last live slot again (in bounds; the gather then reads x[r]) and
 // its product is dropped by a select (a conditional add let the compiler
-// sink the slot's loads under it).
+// sink the slot's loads under it).  R is uniform across the wave
+// (ell_row_pre_epi), so no load is spent on chunks no lane needs.
 template <int R, typename V, typename X, bool O32>
 __device__ inline X ell_chunk(const Ell<V> &A, const X *__restrict__ x, int r, int len, int j0,
                               X s) {
