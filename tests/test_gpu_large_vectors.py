@@ -8,15 +8,17 @@ pairs, grids of at most 2048 blocks) and 2 * 4 * 2048 * 256 for the FP32
 ones.  The other parity tests stay below that, so these run 30 iterations
 on 2D Laplacians of 2200^2 = 4,840,000 rows and 2201^2 = 4,844,401 rows
 (odd: the last pair and quad are partial): the fused FP64 loop in CRS and
-ELL against oracle order 1, and the mixed solver with a reliable update
-every few iterations (delta = 0.5: the restart, residual and accumulate
-kernels run at this size too) against the oracle's mixed restatement.
+ELL against oracle order 1, and the mixed solver (Jacobi, and the degree-4
+polynomial whose Clenshaw steps feed the y/p update) with a reliable
+update every few iterations (delta = 0.5: the restart, residual and
+accumulate kernels run at this size too) against the oracle's mixed
+restatement.
 Every r2 and alpha and the final x are compared bitwise.
 """
 import numpy as np
 import pytest
 
-from cgxx_amd import problems
+from cgxx_amd import problems, spectrum
 from oracle import oracle
 from tests.cases import assert_bitwise
 
@@ -61,8 +63,9 @@ def test_fused_fp64_loop_in_flight_path_bitwise(ctx, side):
         assert_bitwise(ctx.download("X"), ox, f"x ({fmt})")
 
 
+@pytest.mark.parametrize("precond", ["jacobi", "poly"])
 @pytest.mark.parametrize("side", [2200, 2201])
-def test_mixed_in_flight_path_and_updates_bitwise(ctx, side):
+def test_mixed_in_flight_path_and_updates_bitwise(ctx, side, precond):
     m = problems.laplace2d(side)
     k = m.ones_rhs()
     ctx.set_size(m.n)
@@ -71,13 +74,20 @@ def test_mixed_in_flight_path_and_updates_bitwise(ctx, side):
     ctx.upload_jacobi(1.0 / m.diagonal())
     ctx.prepare_fp32()
     ctx.upload("K", k)
+    poly = None
+    if precond == "poly":  # degree 4 on [b / 100, b]: the Clenshaw steps and their y/p update
+        b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+        poly = (spectrum.cheb_residual_coeffs(b / 100.0, b, 4), b)
+        ctx.set_poly(*poly)
     ctx.fill("X", 0.0)
-    print(f"\nmixed Jacobi, delta 0.5, n = {m.n}, layout {ctx.fp32_layout()}", flush=True)
-    info = ctx.solve_mixed(precond="jacobi", max_iter=ITERS, tol=1e-300, delta=0.5)
-    ox, oinfo, oh = oracle.mixed_pcg(n=m.n, crs=(m.ptr, m.index, m.value), k=k, precond="jacobi",
-                                     max_iter=ITERS, tol=1e-300, delta=0.5)
+    print(f"\nmixed {precond}, delta 0.5, n = {m.n}, layout {ctx.fp32_layout()}", flush=True)
+    info = ctx.solve_mixed(precond=precond, max_iter=ITERS, tol=1e-300, delta=0.5)
+    ox, oinfo, oh = oracle.mixed_pcg(n=m.n, crs=(m.ptr, m.index, m.value), k=k, precond=precond,
+                                     poly=poly, max_iter=ITERS, tol=1e-300, delta=0.5)
     assert info["iterations"] == oinfo["iterations"] == ITERS
     assert info["updates"] == oinfo["updates"] >= 2
     assert_bitwise(ctx.history("r2", ITERS + 1), oh["r2"], "inner r2")
+    if precond == "poly":
+        assert_bitwise(ctx.history("rho", ITERS + 1), oh["rho"], "rho = <r, z>")
     assert_bitwise(ctx.history("alpha", ITERS), oh["alpha"], "alpha")
     assert_bitwise(ctx.download("X"), ox, "x")
