@@ -184,21 +184,23 @@ __device__ inline int strip_chunk(const Dia &d, int q) {
   return z * Pc + s * Wc + (rem - z * Wc);
 }
 
-template <int K, bool HALF = false, bool XCD = false, bool STRIP = false, typename Pre,
-          typename Epi, typename XF = XIdent>
+template <int K, bool HALF = false, bool XCD = false, bool STRIP = false, int ROWS = 2,
+          typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
                                    Epi epi, XF xf = XF()) {
   double tpart = 0.0;
   int q0, qs, qend;
   if (K == kOpDia && STRIP && strip_walk(A.dia, q0, qs, qend)) {
-    // Four rows per trip (then two, then one), every row's loads issued
-    // before any is used (contributions added in walk order): more bytes in
-    // flight per wave for steps that stream little besides the matrix.
-    // 256^3 Clenshaw step 76.6 us with one row per trip, 75.4 with two,
-    // 74.3 with four (the Clenshaw instances then use 75-95 VGPRs, 5-6
-    // waves per SIMD); 320^3 unchanged.
+    // ROWS (2 or 4) rows per trip, then two, then one, every row's loads
+    // issued before any is used (contributions added in walk order): more
+    // bytes in flight per wave for steps that stream little besides the
+    // matrix.  256^3 Clenshaw step 76.6 us with one row per trip, 75.4 with
+    // two, 74.3 with four; the four-row instances hold 75-95 VGPRs (5-6
+    // waves per SIMD), which costs more than it gains when a block walks
+    // only a few chunks (128^3: 12% slower), so mp_poly_apply picks them by
+    // size.
     int q = q0;
-    for (; q + 3 * qs < qend; q += 4 * qs) {
+    for (; ROWS >= 4 && q + 3 * qs < qend; q += 4 * qs) {
       const int ra = strip_chunk(A.dia, q) * kBlock + (int)threadIdx.x;
       const int rb = strip_chunk(A.dia, q + qs) * kBlock + (int)threadIdx.x;
       const int rc = strip_chunk(A.dia, q + 2 * qs) * kBlock + (int)threadIdx.x;
@@ -589,7 +591,7 @@ enum { kStepPlain = 0, kStepFirst = 1, kStepSecond = 2 };
 // that, fewer than 8 blocks of 256 fit on a CU of gfx950 and the grid of
 // 2048 blocks runs in two rounds (295 us instead of 200 us per step at
 // 256^3).
-template <bool FINAL, int K, int MODE, bool HALF, bool NT = false>
+template <bool FINAL, int K, int MODE, bool HALF, bool NT = false, int ROWS = 2>
 __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
                                                       const float *__restrict__ r,
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
   const float ck = poly->c[poly->degree];
-  const double tpart = op32_rows<K, HALF, K == kOpDia || !FINAL, !FINAL>(
+  const double tpart = op32_rows<K, HALF, K == kOpDia || !FINAL, !FINAL, ROWS>(
       A, MODE == kStepFirst ? r : b1, prod,
       [&](int i) {
         const float ri = NT ? __builtin_nontemporal_load(r + i) : r[i];
@@ -725,7 +727,7 @@ hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, 
   return hipGetLastError();
 }
 
-template <int K, bool HALF, bool NT>
+template <int K, bool HALF, bool NT, int ROWS = 2>
 static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
@@ -734,23 +736,23 @@ static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly
   for (int j = degree - 1; j >= 1; j--) {
     const float *b1 = w[(j + 1) % 3], *b2 = (j + 2 <= degree - 1) ? w[(j + 2) % 3] : nullptr;
     if (j == degree - 1)
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst, HALF, NT>), grid, block, 0, s, A, nullptr,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst, HALF, NT, ROWS>), grid, block, 0, s, A, nullptr,
                          nullptr, r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
     else if (j == degree - 2)
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond, HALF, NT>), grid, block, 0, s, A, b1, nullptr,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond, HALF, NT, ROWS>), grid, block, 0, s, A, b1, nullptr,
                          r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
     else
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain, HALF, NT>), grid, block, 0, s, A, b1, b2, r32,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain, HALF, NT, ROWS>), grid, block, 0, s, A, b1, b2, r32,
                          poly, j, w[j % 3], (float *)nullptr, prz, st);
   }
   if (degree == 1)
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst, HALF, NT>), grid, block, 0, s, A, nullptr, nullptr,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst, HALF, NT, ROWS>), grid, block, 0, s, A, nullptr, nullptr,
                        r32, poly, 0, w[0], pcopy, prz, st);
   else if (degree == 2)
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond, HALF, NT>), grid, block, 0, s, A, w[1], nullptr,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond, HALF, NT, ROWS>), grid, block, 0, s, A, w[1], nullptr,
                        r32, poly, 0, w[0], pcopy, prz, st);
   else
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain, HALF, NT>), grid, block, 0, s, A, w[1], w[2], r32,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain, HALF, NT, ROWS>), grid, block, 0, s, A, w[1], w[2], r32,
                        poly, 0, w[0], pcopy, prz, st);
 }
 
@@ -760,16 +762,29 @@ static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly
 // they would cost 83 -> 87 us.
 static bool poly_stream(int n) { return n > (1 << 24); }
 
+// Four rows per strip-walk trip when every block walks at least 8 chunks
+// (256^3: 16 per block of 4096); below that the two-row instances, with
+// fewer registers and more waves, are faster (128^3: 2 chunks per block).
+static bool poly_rows4(const Op32 &A) {
+  return (int64_t)ell_chunks(A.dia.n) >= 8 * (int64_t)op32_grid(A);
+}
+
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s, bool half) {
   switch (A.kind) {
     case kOpDia:
       if (A.dia.unit != 1) return hipErrorInvalidValue;
-      if (half && poly_stream(A.dia.n))
+      if (half && poly_stream(A.dia.n) && poly_rows4(A))
+        poly_apply_t<kOpDia, true, true, 4>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      else if (half && poly_stream(A.dia.n))
         poly_apply_t<kOpDia, true, true>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      else if (half && poly_rows4(A))
+        poly_apply_t<kOpDia, true, false, 4>(A, r32, poly, degree, w, pcopy, prz, st, s);
       else if (half)
         poly_apply_t<kOpDia, true, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      else if (poly_rows4(A))
+        poly_apply_t<kOpDia, false, false, 4>(A, r32, poly, degree, w, pcopy, prz, st, s);
       else
         poly_apply_t<kOpDia, false, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
       break;

@@ -91,3 +91,41 @@ def test_mixed_in_flight_path_and_updates_bitwise(ctx, side, precond):
         assert_bitwise(ctx.history("rho", ITERS + 1), oh["rho"], "rho = <r, z>")
     assert_bitwise(ctx.history("alpha", ITERS), oh["alpha"], "alpha")
     assert_bitwise(ctx.download("X"), ox, "x")
+
+
+@pytest.mark.parametrize("bits", [16, 32])
+def test_poly_four_row_strip_walk_bitwise(ctx, bits):
+    """The Clenshaw steps' four-rows-per-trip strip walk, which the library
+    picks only when every block walks at least 8 chunks of 256 rows: a 3D
+    Laplacian of 208^3 = 8,998,912 rows (35,152 chunks over 4096 blocks; its
+    x-y plane of 169 chunks gives the strip geometry), the degree-4
+    polynomial on FP16 and on FP32 operator values, 6 iterations bit for
+    bit against the oracle."""
+    m = problems.laplace3d(208)
+    assert m.n // 256 >= 8 * 4096
+    k = m.ones_rhs()
+    ctx.set_size(m.n)
+    ctx.upload_crs(m.ptr, m.index, m.value)
+    ctx.set_format("CRS")
+    ctx.upload_jacobi(1.0 / m.diagonal())
+    ctx.prepare_fp32()
+    ctx.upload("K", k)
+    assert ctx.fp32_layout() == "DIA"
+    b = spectrum.gershgorin_bound(m.ptr, m.index, m.value)
+    poly = (spectrum.cheb_residual_coeffs(b / 100.0, b, 4), b)
+    ctx.set_poly_values(bits)
+    ctx.set_poly(*poly)
+    ctx.fill("X", 0.0)
+    print(f"\npolynomial, FP{bits} values, n = {m.n}", flush=True)
+    its = 6
+    info = ctx.solve_mixed(precond="poly", max_iter=its, tol=1e-300)
+    ox, oinfo, oh = oracle.mixed_pcg(n=m.n, crs=(m.ptr, m.index, m.value), k=k, precond="poly",
+                                     poly=poly, max_iter=its, tol=1e-300,
+                                     poly_fp32_values=bits == 32)
+    assert info["iterations"] == oinfo["iterations"] == its
+    assert info["updates"] == oinfo["updates"]
+    assert_bitwise(ctx.history("r2", its + 1), oh["r2"], "inner r2")
+    assert_bitwise(ctx.history("rho", its + 1), oh["rho"], "rho = <r, z>")
+    assert_bitwise(ctx.history("alpha", its), oh["alpha"], "alpha")
+    assert_bitwise(ctx.download("X"), ox, "x")
+    ctx.set_poly_values(16)
