@@ -6,7 +6,11 @@ and 64 VGPRs a CU holds 8 blocks of 256 threads and the step takes 200 us;
 at 82-91 SGPRs it takes 275-295 us although the compiler still reports 8
 waves per SIMD, because fewer blocks are resident and the grid runs in two
 rounds.  Every kernel of the FP64 and mixed-precision loops therefore stays
-at <= 80 SGPRs, <= 64 VGPRs and no scratch.
+at <= 80 SGPRs, <= 64 VGPRs and no scratch, with one measured exception:
+the Clenshaw steps' four-rows-per-trip instances (k_poly_step<..., 4>) hold
+up to 96 VGPRs (5-6 waves per SIMD) for twice the loads in flight, 75.4 ->
+74.3 us per 256^3 step; the library takes them only where every block
+walks at least 8 chunks and the 64-VGPR two-row instances elsewhere.
 """
 import json
 import os
@@ -36,14 +40,26 @@ def test_every_hot_kernel_is_recorded(resources):
         assert any(re.search(pat, k) for k in resources), pat
 
 
+FOUR_ROWS = re.compile(r"k_poly_step<.*, 4>\(")
+
+
 @pytest.mark.parametrize("pat", HOT)
 def test_hot_kernel_register_budget(resources, pat):
     for name, r in resources.items():
         if not re.search(pat, name):
             continue
+        vgpr_max = 96 if FOUR_ROWS.search(name) else 64
         assert r["sgpr"] <= 80, (name, r)
-        assert r["vgpr"] + r.get("agpr", 0) <= 64, (name, r)
+        assert r["vgpr"] + r.get("agpr", 0) <= vgpr_max, (name, r)
         assert r["scratch"] == 0, (name, r)
+
+
+def test_four_row_instances_exist_and_two_row_ones_keep_the_budget(resources):
+    four = [k for k in resources if FOUR_ROWS.search(k)]
+    two = [k for k in resources if re.search(r"k_poly_step<", k) and not FOUR_ROWS.search(k)]
+    assert four and two
+    for name in two:
+        assert resources[name]["vgpr"] <= 64, name
 
 
 def test_remark_parser():
