@@ -363,8 +363,11 @@ def main() -> int:
     # operator stream of one FP32 SpMV of the scaled operator (without x, y)
     op_bytes = {"DIA": 4 * ndiag * n, "ELL": 8 * nnz_nd + 4 * n,
                 "CRS": 8 * nnz_nd + 4 * (n + 1)}[layout]
-    # the Clenshaw steps read each row's FP16 record
-    poly_op_bytes = 4 * ((ndiag + 1) // 2) * n if layout == "DIA" else op_bytes
+    # the Clenshaw steps read each row's FP16 record, or with symmetric
+    # storage the upper diagonals' values only (2 bytes per stored pair)
+    poly_values = ctx.poly_values_layout()
+    poly_op_bytes = (ndiag * n if poly_values == "FP16-SYM" else 4 * ((ndiag + 1) // 2) * n) \
+        if layout == "DIA" else op_bytes
     kind = {"CRS": 0, "ELL": 1, "DIA": 2}[layout]
     log(f"uploaded FP64 CRS + ELL, Jacobi, FP32 copy and scaled FP32 operator ({layout}"
         + (f", {ndiag} diagonals)" if layout == "DIA" else ")"))

@@ -29,6 +29,7 @@ SYMBOLS = [
     "cghip_apply_preconditioner", "cghip_solve", "cghip_history", "cghip_time_matvec", "cghip_time_hook",
     "cghip_fill_vector", "cghip_true_residual", "cghip_prepare_fp32", "cghip_solve_mixed",
     "cghip_set_poly", "cghip_set_poly_values", "cghip_matrix_stats", "cghip_time_poly_step", "cghip_fp32_layout",
+    "cghip_poly_values_layout",
     "cghip_mixed_iteration_bytes",
     "cghip_mlp_param_count", "cghip_mlp_create", "cghip_mlp_destroy", "cghip_mlp_last_error",
     "cghip_mlp_set_params", "cghip_mlp_get_params", "cghip_mlp_forward", "cghip_mlp_gradient",
@@ -123,6 +124,7 @@ def load() -> C.CDLL:
     L.cghip_matrix_stats.argtypes = [vp, _f64p, C.c_int]
     L.cghip_time_poly_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.cghip_fp32_layout.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cghip_poly_values_layout.argtypes = [vp, C.POINTER(C.c_int)]
     L.cghip_mixed_iteration_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.cghip_mlp_param_count.argtypes = [C.c_int] * 4
@@ -272,6 +274,13 @@ class Context:
         v = C.c_int()
         self._chk(self.L.cghip_fp32_layout(self._h, C.byref(v)), "cghip_fp32_layout")
         return {0: "CRS", 1: "ELL", 2: "DIA"}.get(v.value)
+
+    def poly_values_layout(self) -> str | None:
+        """Values the Clenshaw steps read: "FP32", "FP16" (DIA row records),
+        "FP16-SYM" (upper diagonals only) or None."""
+        v = C.c_int()
+        self._chk(self.L.cghip_poly_values_layout(self._h, C.byref(v)), "cghip_poly_values_layout")
+        return {0: "FP32", 1: "FP16", 2: "FP16-SYM"}.get(v.value)
 
     def set_poly(self, coeffs, b: float):
         """Polynomial preconditioner p(D^-1 A) D^-1, p = sum_j coeffs[j] T_j(2 l / b - 1)."""

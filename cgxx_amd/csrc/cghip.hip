@@ -96,6 +96,7 @@ struct cghip_ctx {
   bool dia_nd = false;
   float *dia_val = nullptr;
   uint16_t *dia_val16 = nullptr;  // FP16 copy for the polynomial's Clenshaw steps
+  uint16_t *dia_up16 = nullptr;   // its upper diagonals when symmetric (Dia::up16), or null
   float *v32[4] = {};
   MixedState *d_mstate = nullptr;
   MixedState *h_mstate = nullptr;  // pinned
@@ -183,6 +184,7 @@ void free_fp32(cghip_ctx *c) {
   dfree(c->ell_slot_nd);
   dfree(c->dia_val);
   dfree(c->dia_val16);
+  dfree(c->dia_up16);
   c->dia_nd = false;
   c->ell_nd = false;
   c->ell_w_nd = 0;
@@ -249,7 +251,8 @@ Ell<double> ell_of(cghip_ctx *c) {
 }
 
 Dia dia_of(cghip_ctx *c) {
-  Dia d{c->dia_val, c->dia_val16, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1, 0, 0};
+  Dia d{c->dia_val, c->dia_val16, c->n, c->dia_ndiag > 0 ? c->dia_ndiag : 0, {}, 1, 0, 0,
+        c->dia_up16};
   for (int k = 0; k < kDiaMax; k++) d.off[k] = c->dia_off[k];
   dia_strips(d);
   return d;
@@ -906,6 +909,23 @@ int cghip_prepare_fp32(cghip_ctx *c) {
         dfree(c->dia_val);
         dfree(c->dia_val16);
         c->dia_nd = false;
+      } else if (c->dia_ndiag == 2 * kSymPairs && poly_sym_pays(n) && dia_mirrored(dia_of(c))) {
+        // symmetric FP16 storage, kept only if every value equals its
+        // transpose's bit for bit
+        rc = dalloc(c, &c->dia_up16, (size_t)kSymPairs * ((size_t)n + kSymPad));
+        if (rc) {
+          free_fp32(c);
+          return rc;
+        }
+        int *d_mis = reinterpret_cast<int *>(c->dot_out);
+        HIPCHK(c, hipMemsetAsync(d_mis, 0, sizeof(int), c->stream));
+        Dia d = dia_of(c);
+        d.up16 = nullptr;
+        HIPCHK(c, dia_upper16(d, c->dia_up16, d_mis, c->stream));
+        int mis = 0;
+        HIPCHK(c, hipMemcpyAsync(&mis, d_mis, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        if (int r = sync(c)) return r;
+        if (mis) dfree(c->dia_up16);
       }
     }
     if (!c->dia_nd && ell_pays(w, n, c->nnz_nd)) {
@@ -944,6 +964,15 @@ int cghip_fp32_layout(const cghip_ctx *c, int *layout) {
             : c->dia_nd    ? CGHIP_LAYOUT_DIA
             : c->ell_nd    ? CGHIP_LAYOUT_ELL
                            : CGHIP_LAYOUT_CRS;
+  return CGHIP_OK;
+}
+
+int cghip_poly_values_layout(const cghip_ctx *c, int *layout) {
+  if (!c || !layout) return CGHIP_ERR_ARG;
+  *layout = !c->has_scaled                  ? -1
+            : !c->dia_nd || !c->poly_fp16   ? CGHIP_POLY_VALUES_FP32
+            : c->dia_up16                   ? CGHIP_POLY_VALUES_FP16_SYM
+                                            : CGHIP_POLY_VALUES_FP16;
   return CGHIP_OK;
 }
 
@@ -1309,10 +1338,13 @@ int cghip_mixed_iteration_bytes(const cghip_ctx *c, int precond, double *bytes) 
   const double nnz = c->nnz_nd, d = c->dia_ndiag;
   // operator stream of one SpMV: FP32 values in the CG SpMV (DIA: one
   // stream per diagonal, no mask); the Clenshaw steps read DIA's FP16 row
-  // records
+  // records, or with symmetric storage its upper diagonals (2 bytes per
+  // stored pair; the lower values are those of earlier rows)
   const double op32 = c->dia_nd ? 4.0 * d * n : c->ell_nd ? 8.0 * nnz + 4.0 * n
                                                           : 8.0 * nnz + 4.0 * (n + 1);
-  const double op16 = c->dia_nd && c->poly_fp16 ? 4.0 * half_words(c->dia_ndiag) * n : op32;
+  const double op16 = !(c->dia_nd && c->poly_fp16) ? op32
+                      : c->dia_up16                ? 1.0 * d * n
+                                                   : 4.0 * half_words(c->dia_ndiag) * n;
   const double spmv = op32 + 8.0 * n;
   if (precond == CGHIP_PRECOND_JACOBI) {
     *bytes = spmv + 12.0 * n + 20.0 * n;  // + r update (r, q; r) + y, p update (y, p, r; y, p)

@@ -274,7 +274,40 @@ struct Dia {
   // largest offset P is a plane of plane_chunks 256-row chunks, cut into
   // strips of strip_chunks; 0 = not a plane-aligned stencil.
   int plane_chunks, strip_chunks;
+  // Symmetric FP16 storage (dia_upper16), or null: when the kSymPairs * 2
+  // diagonals come in mirrored pairs (off[d] == -off[ndiag - 1 - d]) and every FP16 value
+  // equals its transpose's bit for bit (absent marks included), up16 holds
+  // only the upper diagonals, one stream of n halves each: stream k at
+  // up16 + k (n + kSymPad) is diagonal ndiag / 2 + k.  Row i reads its lower value of
+  // diagonal ndiag / 2 - 1 - k as entry (i - o, i) of stream k (o = off[ndiag
+  // / 2 + k]), so the Clenshaw steps load 2 bytes per stored pair instead of
+  // 4 and form the same products in the same order.
+  const uint16_t *up16;
 };
+// Diagonal pairs of the symmetric FP16 storage the Clenshaw steps read:
+// those of a 3D 7-point stencil.  A compile-time count keeps the row's
+// loads free of branches (with the count known only at run time each
+// stream's loads sat in a branch of their own and waited for the previous
+// stream's: 256^3 step 117 against 75.5 us from the records).
+constexpr int kSymPairs = 3;
+// Stream k of the symmetric storage starts at up16 + k (n + kSymPad): an
+// odd number of 128-byte lines between streams, so that the streams' loads
+// for one row do not all fall on one HBM channel when 2n bytes is a large
+// power of two.
+constexpr int kSymPad = 1088;
+// Sizes where the symmetric storage is used: per Clenshaw step (degree 12,
+// against the records) 128^3 8.8 against 12.3 us (its vectors and values
+// stay in the last-level cache: fewer load bytes and 62 instead of 86
+// VGPRs, 8 waves per SIMD instead of 5), 320^3 157.6 against 161.7 us, but
+// 256^3 75.8 against 75.5 us; the stride padding changed nothing there.
+inline bool poly_sym_pays(int64_t n) { return n <= (1 << 22) || n > (1 << 24); }
+// Diagonals in mirrored pairs around the (implicit) main diagonal.
+inline bool dia_mirrored(const Dia &d) {
+  if (d.ndiag < 2 || d.ndiag % 2 != 0 || d.off[d.ndiag / 2] <= 0) return false;
+  for (int k = 0; k < d.ndiag; k++)
+    if (d.off[k] != -d.off[d.ndiag - 1 - k]) return false;
+  return true;
+}
 // Strip walk geometry of a DIA operator whose largest offset P is a whole
 // number of chunks dividing n (a 3D stencil whose x-y plane is P rows):
 // strips of the largest divisor of P / 256 not above 32 chunks.  Sets both
@@ -338,6 +371,11 @@ hipError_t scaled_ell(const Crs<double> &A, const double *s, int *length, uint64
 // infinite).
 hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float *val,
                       uint16_t *val16, int *overflow, hipStream_t st);
+// up16 (kSymPairs streams of n halves, kSymPad apart, Dia::up16) from D.val16 of a
+// mirrored D; mismatch[0] = 1 (zeroed by the caller) if any FP16 value
+// differs from its transpose's or a lower value lies outside the matrix:
+// up16 must then not be used.
+hipError_t dia_upper16(const Dia &D, uint16_t *up16, int *mismatch, hipStream_t st);
 // x64 += scale o (double)y32 (scale nullable: plain add)
 hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s);
 // r64 = k - q64; partials.a = ||r64||^2 (pairs order)

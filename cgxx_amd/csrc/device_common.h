@@ -353,6 +353,12 @@ __device__ inline double ell_row_pre_epi(const EllPacked &A, const float *__rest
 // Clenshaw step 83.2 -> 75.9 us (the absent mark alone, per-diagonal
 // streams kept: 78.5 us; the record read as three 4-byte loads: 97.4 us).
 //
+// SYM (HALF only, Dia::up16 set, kSymPairs diagonal pairs): slot k holds
+// upper stream k at r and slot kSymPairs + k the same stream at r - o (row
+// r's value on the mirrored lower diagonal), each with x at its column; the
+// products are added in the stored order (lower diagonals from the
+// farthest, then the upper ones), so the row sums are those of the records.
+//
 // The loads of one DIA row (dia_row_load) are kept apart from its products
 // (dia_row_finish) so that a walk can issue two rows' loads before it uses
 // either: dia_row_pre_epi is the two in sequence.
@@ -363,7 +369,7 @@ struct DiaRowLoads {
   PV pv;
 };
 
-template <bool HALF, typename Pre>
+template <bool HALF, bool SYM = false, typename Pre>
 __device__ inline auto dia_row_load(const Dia &A, const float *__restrict__ x, int r, Pre pre) {
   DiaRowLoads<decltype(pre(0))> l;
   l.pv = pre(r);
@@ -373,6 +379,27 @@ __device__ inline auto dia_row_load(const Dia &A, const float *__restrict__ x, i
   // diagonal, so the absent mark only selects the products.  Raw value
   // bits first: a conversion next to its load would wait for it.
   const int last = A.n - 1;
+  if (SYM) {
+    // both loads of a stream unconditional (r - o clamped to 0, its value
+    // then replaced by the absent mark); up16 is not streamed
+    // non-temporally: rows r + o read it again
+    constexpr int Q = kSymPairs;
+    const char *ub = reinterpret_cast<const char *>(A.up16);
+    const uint32_t nb = ((uint32_t)A.n + (uint32_t)kSymPad) * 2u;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      const int o = A.off[Q + k];
+      const int rl = max(r - o, 0);
+      const char *sk = ub + (uint32_t)k * nb;
+      const uint16_t up = *reinterpret_cast<const uint16_t *>(sk + (uint32_t)r * 2u);
+      const uint16_t lo = *reinterpret_cast<const uint16_t *>(sk + (uint32_t)rl * 2u);
+      l.v[k] = up;
+      l.v[Q + k] = r >= o ? lo : kHalfAbsent;
+      l.xv[k] = *reinterpret_cast<const float *>(xb + (uint32_t)min(r + o, last) * 4u);
+      l.xv[Q + k] = *reinterpret_cast<const float *>(xb + (uint32_t)rl * 4u);
+    }
+    return l;
+  }
   if (HALF) {
     const int hw = half_words(A.ndiag);
     const uint32_t *rec = reinterpret_cast<const uint32_t *>(A.val16) + (int64_t)r * hw;
@@ -409,9 +436,21 @@ __device__ inline auto dia_row_load(const Dia &A, const float *__restrict__ x, i
   return l;
 }
 
-template <bool HALF, typename L, typename Epi, typename XF>
+template <bool HALF, bool SYM = false, typename L, typename Epi, typename XF>
 __device__ inline double dia_row_finish(const Dia &A, int r, const L &l, Epi epi, XF xf) {
   float s = 0.0f;
+  if (SYM) {
+    constexpr int Q = kSymPairs;
+#pragma unroll
+    for (int k = Q - 1; k >= 0; k--)
+      if (l.v[Q + k] != kHalfAbsent)
+        s += (float)__builtin_bit_cast(_Float16, (uint16_t)l.v[Q + k]) * xf(l.xv[Q + k]);
+#pragma unroll
+    for (int k = 0; k < Q; k++)
+      if (l.v[k] != kHalfAbsent)
+        s += (float)__builtin_bit_cast(_Float16, (uint16_t)l.v[k]) * xf(l.xv[k]);
+    return epi(r, s, l.pv);
+  }
 #pragma unroll
   for (int d = 0; d < kDiaMax; d++)
     if (d < A.ndiag && (HALF ? l.v[d] != kHalfAbsent : l.v[d] != kF32Absent)) {
@@ -422,11 +461,11 @@ __device__ inline double dia_row_finish(const Dia &A, int r, const L &l, Epi epi
   return epi(r, s, l.pv);
 }
 
-template <bool HALF, typename Pre, typename Epi, typename XF = XIdent>
+template <bool HALF, bool SYM = false, typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double dia_row_pre_epi(const Dia &A, const float *__restrict__ x, int r, Pre pre,
                                          Epi epi, XF xf = XF()) {
-  const auto l = dia_row_load<HALF>(A, x, r, pre);
-  return dia_row_finish<HALF>(A, r, l, epi, xf);
+  const auto l = dia_row_load<HALF, SYM>(A, x, r, pre);
+  return dia_row_finish<HALF, SYM>(A, r, l, epi, xf);
 }
 
 // ELL rows rb*256 + t: y = A x; with WITH_DOT, returns the owner's

@@ -137,6 +137,29 @@ __global__ __launch_bounds__(kBlock) void k_scaled_dia(Crs<double> A, const doub
   }
 }
 
+// Symmetric FP16 storage of a mirrored DIA operator (h = ndiag / 2
+// diagonal pairs): stream k of up16 is diagonal h + k of the records (the
+// entries (i, i + o), o = off[h + k]); row i's value on the mirrored
+// diagonal h - 1 - k, the entry (i, i - o), must equal stream k at i - o,
+// or be absent where i < o.
+__global__ __launch_bounds__(kBlock) void k_dia_upper16(Dia D, uint16_t *__restrict__ up16,
+                                                        int *__restrict__ mismatch) {
+  const int64_t T = (int64_t)gridDim.x * kBlock;
+  const int64_t n = D.n, w = 2 * half_words(D.ndiag);
+  const int h = D.ndiag / 2;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += T) {
+    const uint16_t *rec = D.val16 + i * w;
+    bool bad = false;
+    for (int k = 0; k < h; k++) {
+      const int64_t o = D.off[h + k];
+      up16[k * (n + kSymPad) + i] = rec[h + k];
+      const uint16_t mirror = i >= o ? D.val16[(i - o) * w + h + k] : kHalfAbsent;
+      bad = bad || rec[h - 1 - k] != mirror;
+    }
+    if (bad) mismatch[0] = 1;
+  }
+}
+
 // Row-block loop of the FP32 operator of kind K: for every row i the
 // thread owns, pv = pre(i) early, then epi(i, t, pv) with the sum t of the
 // stored products of row i (from 0, in stored order); returns the sum of
@@ -185,7 +208,7 @@ __device__ inline int strip_chunk(const Dia &d, int q) {
 }
 
 template <int K, bool HALF = false, bool XCD = false, bool STRIP = false, int ROWS = 2,
-          typename Pre, typename Epi, typename XF = XIdent>
+          bool SYM = false, typename Pre, typename Epi, typename XF = XIdent>
 __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, float *prod, Pre pre,
                                    Epi epi, XF xf = XF()) {
   double tpart = 0.0;
@@ -205,33 +228,33 @@ __device__ inline double op32_rows(const Op32 &A, const float *__restrict__ x, f
       const int rb = strip_chunk(A.dia, q + qs) * kBlock + (int)threadIdx.x;
       const int rc = strip_chunk(A.dia, q + 2 * qs) * kBlock + (int)threadIdx.x;
       const int rd = strip_chunk(A.dia, q + 3 * qs) * kBlock + (int)threadIdx.x;
-      const auto la = dia_row_load<HALF>(A.dia, x, ra, pre);
-      const auto lb = dia_row_load<HALF>(A.dia, x, rb, pre);
-      const auto lc = dia_row_load<HALF>(A.dia, x, rc, pre);
-      const auto ld = dia_row_load<HALF>(A.dia, x, rd, pre);
-      tpart += dia_row_finish<HALF>(A.dia, ra, la, epi, xf);
-      tpart += dia_row_finish<HALF>(A.dia, rb, lb, epi, xf);
-      tpart += dia_row_finish<HALF>(A.dia, rc, lc, epi, xf);
-      tpart += dia_row_finish<HALF>(A.dia, rd, ld, epi, xf);
+      const auto la = dia_row_load<HALF, SYM>(A.dia, x, ra, pre);
+      const auto lb = dia_row_load<HALF, SYM>(A.dia, x, rb, pre);
+      const auto lc = dia_row_load<HALF, SYM>(A.dia, x, rc, pre);
+      const auto ld = dia_row_load<HALF, SYM>(A.dia, x, rd, pre);
+      tpart += dia_row_finish<HALF, SYM>(A.dia, ra, la, epi, xf);
+      tpart += dia_row_finish<HALF, SYM>(A.dia, rb, lb, epi, xf);
+      tpart += dia_row_finish<HALF, SYM>(A.dia, rc, lc, epi, xf);
+      tpart += dia_row_finish<HALF, SYM>(A.dia, rd, ld, epi, xf);
     }
     for (; q + qs < qend; q += 2 * qs) {
       const int ra = strip_chunk(A.dia, q) * kBlock + (int)threadIdx.x;
       const int rb = strip_chunk(A.dia, q + qs) * kBlock + (int)threadIdx.x;
-      const auto la = dia_row_load<HALF>(A.dia, x, ra, pre);
-      const auto lb = dia_row_load<HALF>(A.dia, x, rb, pre);
-      tpart += dia_row_finish<HALF>(A.dia, ra, la, epi, xf);
-      tpart += dia_row_finish<HALF>(A.dia, rb, lb, epi, xf);
+      const auto la = dia_row_load<HALF, SYM>(A.dia, x, ra, pre);
+      const auto lb = dia_row_load<HALF, SYM>(A.dia, x, rb, pre);
+      tpart += dia_row_finish<HALF, SYM>(A.dia, ra, la, epi, xf);
+      tpart += dia_row_finish<HALF, SYM>(A.dia, rb, lb, epi, xf);
     }
     if (q < qend) {
       const int r = strip_chunk(A.dia, q) * kBlock + (int)threadIdx.x;
-      tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi, xf);
+      tpart += dia_row_pre_epi<HALF, SYM>(A.dia, x, r, pre, epi, xf);
     }
   } else if (K == kOpDia) {
     int rb0, stride, rend;
     chunk_walk<XCD>(ell_chunks(A.dia.n), rb0, stride, rend);
     for (int rb = rb0; rb < rend; rb += stride) {
       const int r = rb * kBlock + (int)threadIdx.x;
-      if (r < A.dia.n) tpart += dia_row_pre_epi<HALF>(A.dia, x, r, pre, epi, xf);
+      if (r < A.dia.n) tpart += dia_row_pre_epi<HALF, SYM>(A.dia, x, r, pre, epi, xf);
     }
   } else if (K == kOpEll) {
     int rb0, stride, rend;
@@ -586,12 +609,14 @@ enum { kStepPlain = 0, kStepFirst = 1, kStepSecond = 2 };
 // off.  HALF (DIA only): the operator values come from the FP16 copy.
 // A_hat is the scaled operator: its unit diagonal is implicit (the launcher
 // refuses any other).  NT: r and B_{j+2}, read once per row, are loaded
-// non-temporally (poly_stream picks it by size).  The kernel keeps at most
+// non-temporally (poly_stream picks it by size).  SYM (with HALF): the
+// values come from the upper-diagonal streams Dia::up16, the same values
+// and products as the records in 2 instead of 4 bytes per stored pair.  The kernel keeps at most
 // 80 SGPRs: above
 // that, fewer than 8 blocks of 256 fit on a CU of gfx950 and the grid of
 // 2048 blocks runs in two rounds (295 us instead of 200 us per step at
 // 256^3).
-template <bool FINAL, int K, int MODE, bool HALF, bool NT = false, int ROWS = 2>
+template <bool FINAL, int K, int MODE, bool HALF, bool NT = false, int ROWS = 2, bool SYM = false>
 __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, const float *__restrict__ b1,
                                                       const float *__restrict__ b2,
                                                       const float *__restrict__ r,
@@ -605,7 +630,7 @@ __global__ __launch_bounds__(kBlock) CGHIP_SGPR_CAP void k_poly_step(Op32 A, con
   const float cj = poly->c[j];
   const float tb = poly->two_over_b;
   const float ck = poly->c[poly->degree];
-  const double tpart = op32_rows<K, HALF, K == kOpDia || !FINAL, !FINAL, ROWS>(
+  const double tpart = op32_rows<K, HALF, K == kOpDia || !FINAL, !FINAL, ROWS, SYM>(
       A, MODE == kStepFirst ? r : b1, prod,
       [&](int i) {
         const float ri = NT ? __builtin_nontemporal_load(r + i) : r[i];
@@ -702,6 +727,12 @@ hipError_t scaled_dia(const Crs<double> &A, const double *s, const Dia &D, float
   return hipGetLastError();
 }
 
+hipError_t dia_upper16(const Dia &D, uint16_t *up16, int *mismatch, hipStream_t st) {
+  if (D.ndiag != 2 * kSymPairs || !dia_mirrored(D)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_dia_upper16, dim3(row_grid(D.n)), dim3(kBlock), 0, st, D, up16, mismatch);
+  return hipGetLastError();
+}
+
 hipError_t mp_accumulate(int n, double *x64, const float *y32, const double *scale, hipStream_t s) {
   hipLaunchKernelGGL(k_mp_accum, dim3(vec_grid(n)), dim3(kBlock), 0, s, n, x64, y32, scale);
   return hipGetLastError();
@@ -727,7 +758,7 @@ hipError_t mp_restart_fold(int n, Partials prt, Partials prr, const double *rz, 
   return hipGetLastError();
 }
 
-template <int K, bool HALF, bool NT, int ROWS = 2>
+template <int K, bool HALF, bool NT, int ROWS = 2, bool SYM = false>
 static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s) {
@@ -736,23 +767,23 @@ static void poly_apply_t(const Op32 &A, const float *r32, const PolyParams *poly
   for (int j = degree - 1; j >= 1; j--) {
     const float *b1 = w[(j + 1) % 3], *b2 = (j + 2 <= degree - 1) ? w[(j + 2) % 3] : nullptr;
     if (j == degree - 1)
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst, HALF, NT, ROWS>), grid, block, 0, s, A, nullptr,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepFirst, HALF, NT, ROWS, SYM>), grid, block, 0, s, A, nullptr,
                          nullptr, r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
     else if (j == degree - 2)
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond, HALF, NT, ROWS>), grid, block, 0, s, A, b1, nullptr,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepSecond, HALF, NT, ROWS, SYM>), grid, block, 0, s, A, b1, nullptr,
                          r32, poly, j, w[j % 3], (float *)nullptr, prz, st);
     else
-      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain, HALF, NT, ROWS>), grid, block, 0, s, A, b1, b2, r32,
+      hipLaunchKernelGGL((k_poly_step<false, K, kStepPlain, HALF, NT, ROWS, SYM>), grid, block, 0, s, A, b1, b2, r32,
                          poly, j, w[j % 3], (float *)nullptr, prz, st);
   }
   if (degree == 1)
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst, HALF, NT, ROWS>), grid, block, 0, s, A, nullptr, nullptr,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepFirst, HALF, NT, ROWS, SYM>), grid, block, 0, s, A, nullptr, nullptr,
                        r32, poly, 0, w[0], pcopy, prz, st);
   else if (degree == 2)
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond, HALF, NT, ROWS>), grid, block, 0, s, A, w[1], nullptr,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepSecond, HALF, NT, ROWS, SYM>), grid, block, 0, s, A, w[1], nullptr,
                        r32, poly, 0, w[0], pcopy, prz, st);
   else
-    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain, HALF, NT, ROWS>), grid, block, 0, s, A, w[1], w[2], r32,
+    hipLaunchKernelGGL((k_poly_step<true, K, kStepPlain, HALF, NT, ROWS, SYM>), grid, block, 0, s, A, w[1], w[2], r32,
                        poly, 0, w[0], pcopy, prz, st);
 }
 
@@ -769,20 +800,31 @@ static bool poly_rows4(const Op32 &A) {
   return (int64_t)ell_chunks(A.dia.n) >= 8 * (int64_t)op32_grid(A);
 }
 
+// The FP16 DIA steps: records, or the upper-diagonal streams (SYM).
+template <bool SYM>
+static void poly_apply_half(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
+                            float *const w[3], float *pcopy, Partials prz, const MixedState *st,
+                            hipStream_t s) {
+  if (poly_stream(A.dia.n) && poly_rows4(A))
+    poly_apply_t<kOpDia, true, true, 4, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);
+  else if (poly_stream(A.dia.n))
+    poly_apply_t<kOpDia, true, true, 2, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);
+  else if (poly_rows4(A))
+    poly_apply_t<kOpDia, true, false, 4, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);
+  else
+    poly_apply_t<kOpDia, true, false, 2, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);
+}
+
 hipError_t mp_poly_apply(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                          float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                          hipStream_t s, bool half) {
   switch (A.kind) {
     case kOpDia:
       if (A.dia.unit != 1) return hipErrorInvalidValue;
-      if (half && poly_stream(A.dia.n) && poly_rows4(A))
-        poly_apply_t<kOpDia, true, true, 4>(A, r32, poly, degree, w, pcopy, prz, st, s);
-      else if (half && poly_stream(A.dia.n))
-        poly_apply_t<kOpDia, true, true>(A, r32, poly, degree, w, pcopy, prz, st, s);
-      else if (half && poly_rows4(A))
-        poly_apply_t<kOpDia, true, false, 4>(A, r32, poly, degree, w, pcopy, prz, st, s);
+      if (half && A.dia.up16)
+        poly_apply_half<true>(A, r32, poly, degree, w, pcopy, prz, st, s);
       else if (half)
-        poly_apply_t<kOpDia, true, false>(A, r32, poly, degree, w, pcopy, prz, st, s);
+        poly_apply_half<false>(A, r32, poly, degree, w, pcopy, prz, st, s);
       else if (poly_rows4(A))
         poly_apply_t<kOpDia, false, false, 4>(A, r32, poly, degree, w, pcopy, prz, st, s);
       else

@@ -40,7 +40,8 @@ def test_every_hot_kernel_is_recorded(resources):
         assert any(re.search(pat, k) for k in resources), pat
 
 
-FOUR_ROWS = re.compile(r"k_poly_step<.*, 4>\(")
+# template arguments FINAL, K, MODE, HALF, NT, ROWS, SYM
+FOUR_ROWS = re.compile(r"k_poly_step<[^(]*, 4, (true|false)>\(")
 
 
 @pytest.mark.parametrize("pat", HOT)
