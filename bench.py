@@ -87,7 +87,11 @@ def kernel_short_name(name: str) -> str:
 def algorithmic_bytes(short: str, meta: dict):
     """Bytes a launch must move at least (each array once, x once per SpMV)."""
     n, nnz, D = meta["n"], meta["nnz"], meta["ndiag"]
-    rec = 4 * ((D + 1) // 2) * n  # the FP16 records of the Clenshaw steps
+    # the Clenshaw steps' FP16 values: the row records, or with the symmetric
+    # storage (template arguments FINAL, K, MODE, HALF, NT, ROWS, SYM; SYM
+    # true) the upper diagonals only, 2 bytes per stored pair
+    args = short[short.find("<") + 1:-1].split(", ") if short.startswith("k_poly_step<") else []
+    rec = D * n if len(args) >= 7 and args[6] == "true" else 4 * ((D + 1) // 2) * n
     table = [("k_pcg_matvec_dot_crs", crs_bytes(n, nnz, 8, 8)), ("k_matvec_crs<double", crs_bytes(n, nnz, 8, 8)),
              ("k_pcg_matvec_dot_ell", ell_bytes(n, nnz, 8, 8)), ("k_matvec_ell<double", ell_bytes(n, nnz, 8, 8)),
              ("k_pcg_update<true>", 32 * n), ("k_pcg_update<false>", 24 * n), ("k_pcg_xpay<true>", 48 * n),

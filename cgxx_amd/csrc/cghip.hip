@@ -909,7 +909,7 @@ int cghip_prepare_fp32(cghip_ctx *c) {
         dfree(c->dia_val);
         dfree(c->dia_val16);
         c->dia_nd = false;
-      } else if (c->dia_ndiag == 2 * kSymPairs && poly_sym_pays(n) && dia_mirrored(dia_of(c))) {
+      } else if (c->dia_ndiag == 2 * kSymPairs && dia_mirrored(dia_of(c))) {
         // symmetric FP16 storage, kept only if every value equals its
         // transpose's bit for bit
         rc = dalloc(c, &c->dia_up16, (size_t)kSymPairs * ((size_t)n + kSymPad));

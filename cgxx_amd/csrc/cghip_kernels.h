@@ -295,12 +295,13 @@ constexpr int kSymPairs = 3;
 // for one row do not all fall on one HBM channel when 2n bytes is a large
 // power of two.
 constexpr int kSymPad = 1088;
-// Sizes where the symmetric storage is used: per Clenshaw step (degree 12,
-// against the records) 128^3 8.8 against 12.3 us (its vectors and values
-// stay in the last-level cache: fewer load bytes and 62 instead of 86
-// VGPRs, 8 waves per SIMD instead of 5), 320^3 157.6 against 161.7 us, but
-// 256^3 75.8 against 75.5 us; the stride padding changed nothing there.
-inline bool poly_sym_pays(int64_t n) { return n <= (1 << 22) || n > (1 << 24); }
+// Non-temporal r and B_{j+2} in the Clenshaw steps on the symmetric storage
+// once a step's 22 bytes per row outgrow the 256 MB last-level cache
+// (per step, degree 12: 224^3 37.8 us cached against 46.7 non-temporal,
+// 240^3 61.2 against 58.1, 256^3 75.6 against 67.8; without them the
+// 256^3 step read 4 bytes per row less than from the records, PMC
+// FETCH_SIZE, in the same time).
+inline bool poly_stream_sym(int64_t n) { return 22 * n > (int64_t(1) << 28); }
 // Diagonals in mirrored pairs around the (implicit) main diagonal.
 inline bool dia_mirrored(const Dia &d) {
   if (d.ndiag < 2 || d.ndiag % 2 != 0 || d.off[d.ndiag / 2] <= 0) return false;

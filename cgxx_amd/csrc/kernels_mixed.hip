@@ -805,9 +805,10 @@ template <bool SYM>
 static void poly_apply_half(const Op32 &A, const float *r32, const PolyParams *poly, int degree,
                             float *const w[3], float *pcopy, Partials prz, const MixedState *st,
                             hipStream_t s) {
-  if (poly_stream(A.dia.n) && poly_rows4(A))
+  const bool nt = SYM ? poly_stream_sym(A.dia.n) : poly_stream(A.dia.n);
+  if (nt && poly_rows4(A))
     poly_apply_t<kOpDia, true, true, 4, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);
-  else if (poly_stream(A.dia.n))
+  else if (nt)
     poly_apply_t<kOpDia, true, true, 2, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);
   else if (poly_rows4(A))
     poly_apply_t<kOpDia, true, false, 4, SYM>(A, r32, poly, degree, w, pcopy, prz, st, s);

@@ -224,11 +224,10 @@ int cghip_set_poly_values(cghip_ctx *ctx, int bits);
  * CGHIP_POLY_VALUES_FP16_SYM (symmetric FP16 storage: only the upper
  * diagonals, one stream each, row i reading its lower values from the
  * rows i - o; chosen by cghip_prepare_fp32 when the DIA layout holds the
- * three mirrored diagonal pairs of a 3D 7-point stencil, every FP16 value
- * equals its transpose's bit for bit and N <= 2^22 or N > 2^24, the sizes
- * where it measured faster; the products and their order, hence the
- * results, are those of the records, for half the value bytes); -1 without
- * a scaled operator. */
+ * three mirrored diagonal pairs of a 3D 7-point stencil and every FP16
+ * value equals its transpose's bit for bit; the products and their order,
+ * hence the results, are those of the records, for half the value bytes);
+ * -1 without a scaled operator. */
 #define CGHIP_POLY_VALUES_FP32 0
 #define CGHIP_POLY_VALUES_FP16 1
 #define CGHIP_POLY_VALUES_FP16_SYM 2

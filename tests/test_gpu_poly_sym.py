@@ -83,18 +83,19 @@ def test_two_diagonal_pairs_keep_the_records(ctx):
     _solve_and_compare(ctx, m, m.value, "FP16")
 
 
-def test_mid_sizes_keep_the_records(ctx):
-    """Between 2^22 and 2^24 rows (where it measured no faster) the steps
-    read the records: 176^3 = 5,451,776 rows."""
+def test_symmetric_storage_two_row_strip_walk(ctx):
+    """176^3 = 5,451,776 rows: the strip walk two rows per trip (5.2 chunks
+    per block), r and B_{j+2} cached."""
     m = problems.laplace3d(176)
-    _solve_and_compare(ctx, m, m.value, "FP16", iters=3, degree=4)
+    _solve_and_compare(ctx, m, m.value, "FP16-SYM", iters=3, degree=4)
 
 
-def test_symmetric_storage_above_2_24_rows(ctx):
-    """Above 2^24 rows: non-temporal r and B_{j+2}, four rows per strip-walk
-    trip (272^3 = 20,123,648 rows: plane 73,984 rows = 289 chunks)."""
+def test_symmetric_storage_non_temporal(ctx):
+    """Above 2^28 / 22 rows: non-temporal r and B_{j+2}, four rows per
+    strip-walk trip, as at 256^3 (272^3 = 20,123,648 rows: plane 73,984
+    rows = 289 chunks)."""
     m = problems.laplace3d(272)
-    assert m.n > 1 << 24
+    assert 22 * m.n > 1 << 28
     _solve_and_compare(ctx, m, m.value, "FP16-SYM", iters=3, degree=4)
 
 
